@@ -1,0 +1,240 @@
+"""Benchmark: Galerkin triple product R*A*P on one MI355X.
+
+Workload (default): 3D 7-point Poisson operator on a 128^3 grid (2,097,152
+rows), 2x2x2 smoothed aggregation to 262,144 coarse rows, FP64 values, int32
+column indices; synthetic inputs from sp3mm4amg_amd.problems (deterministic).
+A step is one complete product R*A*P on device-resident inputs: upper bounds,
+binning, symbolic, scan, output allocation from the pool and numeric phase.
+
+Prints ONE JSON line (rank 0): throughput in GFLOP/s (FLOP = 2 x multiply-adds
+of the pair form, the reference's flop count), ms per product, the HBM
+roofline of the dominant kernel and the reference CPU path timed on a bounded
+sample on this host's cores.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--path direct|pair]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def alg_bytes(R, A, P, mac1, mac2, nnz_ra, nnz_out, vbytes=8):
+    """Row-gather traffic of the product (stream R, gather A rows by R, gather P
+    rows by RA, write RAP), 4-byte column indices and row offsets."""
+    i = p = 4
+    m = R.M
+    return (R.nnz * (vbytes + i) + (m + 1) * p
+            + 2 * p * R.nnz + (vbytes + i) * mac1
+            + 2 * p * nnz_ra + (vbytes + i) * mac2
+            + nnz_out * (vbytes + i) + (m + 1) * p)
+
+
+def cpu_baseline(R, A, P, flop_per_row_fn, budget_s=20.0):
+    """Reference CPU path (oracle/_ref: the reference library compiled from its
+    sources) on a bounded row sample of the same workload: R[rows]*A*P."""
+    import numpy as np
+    from oracle import oracle as O
+    from oracle import ref
+    from sp3mm4amg_amd.csr import CSR
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    kind = "reference" if ref.available() else "port"
+    # probe: 1/64 of the rows, then scale the sample to the budget
+    def sample(frac):
+        k = max(1, int(R.M * frac))
+        Rs = CSR(k, R.N, R.irp[:k + 1], R.ja[:R.irp[k]], R.vals[:R.irp[k]])
+        return Rs
+
+    def run(Rs):
+        t0 = time.perf_counter()
+        if kind == "reference":
+            ref.sp3mm_pair(Rs, A, P, spmm="spmmRowByRow_SymbNum_", threads=threads)
+        else:
+            O.set_threads(threads)
+            O.sp3mm_pair(Rs, A, P, faithful=True)
+        return time.perf_counter() - t0
+
+    Rs = sample(1 / 64)
+    t = run(Rs)
+    frac = min(1.0, (1 / 64) * budget_s / max(t, 1e-3))
+    Rs = sample(frac)
+    t = run(Rs)
+    mac1 = int(O.ub(Rs, A)[-1])
+    RAs = O.spmm(Rs, A)
+    mac2 = int(O.ub(RAs, P)[-1])
+    flop = 2 * (mac1 + mac2)
+    return {"value": round(flop / t / 1e9, 4), "unit": "GFLOP/s", "cores": threads, "kind": kind,
+            "sample": f"rows 0..{Rs.M - 1} of R ({Rs.M / R.M:.3%}) times A times P, "
+                      f"sp3mmRowByRowPair_0 + spmmRowByRow_SymbNum_0 (IDXMAP), "
+                      f"{t:.2f} s for {flop} FLOP",
+            "seconds": round(t, 3)}
+
+
+def time_form(lib, dR, dA, dP, steps, warmup, **kw):
+    """warmup + steps products; returns (wall ms per product, mean main-kernel ms,
+    median device ms, last stats)."""
+    import numpy as np
+    for _ in range(warmup):
+        out, st = lib.rap(dR, dA, dP, **kw)
+        out.free()
+    lib.lib().sp3mm_sync()
+    main_ms, total_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        out, st = lib.rap(dR, dA, dP, **kw)
+        main_ms.append(st["ms_main"])
+        total_ms.append(st["ms_total"])
+        out.free()
+    lib.lib().sp3mm_sync()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    return wall, float(np.mean(main_ms)), float(np.median(total_ms)), st
+
+
+def mixed_leg(lib, problems, steps, warmup, n=100):
+    """Mixed precision on the 27-point random stencil (100^3, 3x3x3 aggregates):
+    FP32 operands + FP32-pair accumulation vs the FP64 product of the same
+    FP32-rounded operands."""
+    import numpy as np
+    prob = problems.stencil27_sa(n, seed=3)
+    R, A, P = prob["R"], prob["A"], prob["P"]
+    r32 = [m.with_vals(m.vals.astype(np.float32).astype(np.float64)) for m in (R, A, P)]
+    d64 = [lib.DeviceCSR.upload(m) for m in r32]
+    d32 = [lib.DeviceCSR.upload(m, vtype=lib.F32) for m in r32]
+    out, st_pair = lib.rap(*d64, path=lib.PAIR)
+    out.free()
+    mac1, mac2, nnz_ra = st_pair["mac1"], st_pair["mac2"], st_pair["nnz_mid"]
+    flop = 2 * (mac1 + mac2)
+    w64, m64, dev64, st64 = time_form(lib, *d64, steps, warmup, path=lib.DIRECT)
+    wmx, mmx, devmx, stmx = time_form(lib, *d32, steps, warmup, path=lib.DIRECT, precision=lib.MIXED)
+    ref, _ = lib.rap(*d64, path=lib.DIRECT)
+    got, _ = lib.rap(*d32, path=lib.DIRECT, precision=lib.MIXED)
+    a, b = ref.download(), got.download()
+    ref.free()
+    got.free()
+    same = np.array_equal(a.irp, b.irp) and np.array_equal(a.ja, b.ja)
+    rel = np.abs(b.vals - a.vals) / np.maximum(np.abs(a.vals), 1e-300)
+    for m in d64 + d32:
+        m.free()
+    nnz_out = stmx["nnz_out"]
+    return {"workload": f"{prob['name']} Galerkin R*A*P, direct path",
+            "flop": flop, "mac1": mac1, "mac2": mac2, "nnz_RAP": nnz_out,
+            "fp64": {"ms_per_product": round(w64, 4), "device_ms_median": round(dev64, 4),
+                     "gflops": round(flop / w64 / 1e6, 2),
+                     "bytes_alg": alg_bytes(R, A, P, mac1, mac2, nnz_ra, nnz_out, 8)},
+            "mixed": {"ms_per_product": round(wmx, 4), "device_ms_median": round(devmx, 4),
+                      "gflops": round(flop / wmx / 1e6, 2),
+                      "bytes_alg": alg_bytes(R, A, P, mac1, mac2, nnz_ra, nnz_out, 4),
+                      "fp64_rows": int(stmx["guarded_rows"]),
+                      "pattern_equal": bool(same),
+                      "max_rel_err_vs_fp64": float(rel.max()) if rel.size else 0.0,
+                      "median_rel_err_vs_fp64": float(np.median(rel)) if rel.size else 0.0},
+            "speedup_mixed_vs_fp64": round(w64 / wmx, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--path", choices=["direct", "pair"], default="direct")
+    ap.add_argument("--n", type=int, default=128, help="grid side of the 3D Poisson workload")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; the product is sharded by rows "
+                                   "of R across GPUs only in design (--gpus 1)"}))
+        sys.exit(2)
+
+    import numpy as np
+    from sp3mm4amg_amd import lib, problems
+
+    lib.init(0)
+    t0 = time.perf_counter()
+    prob = problems.poisson3d_sa(args.n)
+    R, A, P = prob["R"], prob["A"], prob["P"]
+    log(f"generated {prob['name']} in {time.perf_counter() - t0:.1f}s: nnz R/A/P "
+        f"{R.nnz}/{A.nnz}/{P.nnz}")
+    dR, dA, dP = (lib.DeviceCSR.upload(m) for m in (R, A, P))
+    path = lib.DIRECT if args.path == "direct" else lib.PAIR
+
+    # MAC counts and RA size from the pair form (one untimed product)
+    out, st_pair = lib.rap(dR, dA, dP, path=lib.PAIR)
+    out.free()
+    mac1, mac2, nnz_ra = st_pair["mac1"], st_pair["mac2"], st_pair["nnz_mid"]
+    flop = 2 * (mac1 + mac2)
+
+    for _ in range(args.warmup):
+        out, st = lib.rap(dR, dA, dP, path=path)
+        out.free()
+    lib.lib().sp3mm_sync()
+    main_ms, total_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out, st = lib.rap(dR, dA, dP, path=path)
+        main_ms.append(st["ms_main"])
+        total_ms.append(st["ms_total"])
+        nnz_out = st["nnz_out"]
+        out.free()
+    lib.lib().sp3mm_sync()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall * 1e3 / args.steps
+    value = flop * args.steps / wall / 1e9
+
+    bytes_alg = alg_bytes(R, A, P, mac1, mac2, nnz_ra, nnz_out)
+    main_avg = float(np.mean(main_ms))
+    achieved = bytes_alg / (main_avg * 1e-3) / 1e9
+    dev_med = float(np.median(total_ms))
+    copy_ms = lib.copy_probe(1 << 30, 20)
+    copy_gbs = 2 * (1 << 30) / (copy_ms * 1e-3) / 1e9 if copy_ms > 0 else None
+
+    rec = {
+        "metric": "Galerkin RAP ms & GFLOP/s: FP64 vs mixed-precision vs batched, 1 MI355X",
+        "value": round(value, 3),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{prob['name']} Galerkin R*A*P, FP64, {args.path} path",
+                   "rows_fine": A.M, "rows_coarse": R.M, "nnz_R": R.nnz, "nnz_A": A.nnz,
+                   "nnz_P": P.nnz, "nnz_RA": nnz_ra, "nnz_RAP": nnz_out, "mac1": mac1,
+                   "mac2": mac2, "flop": flop},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": f"numeric {args.path} kernel(s)", "kernel_ms": round(main_avg, 4),
+                     "bytes_alg": bytes_alg},
+        "product": {"device_ms_median": round(dev_med, 4),
+                    "roofline_frac": round(bytes_alg / (dev_med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                    "gflops_device": round(flop / (dev_med * 1e-3) / 1e9, 2)},
+        "hbm_copy_probe_GBs": round(copy_gbs, 1) if copy_gbs else None,
+    }
+    if not args.no_mixed:
+        log("mixed-precision leg (27-point stencil) ...")
+        rec["mixed_precision"] = mixed_leg(lib, problems, max(3, args.steps // 2), 1)
+    if not args.no_cpu:
+        log("timing the reference CPU path on a sample ...")
+        rec["cpu_baseline"] = cpu_baseline(R, A, P, None)
+    for m in (dR, dA, dP):
+        m.free()
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,278 @@
+/*
+ * sp3mm_amd.h -- C ABI of the MI355X (gfx950) Galerkin triple product R*A*P.
+ *
+ * Part 1 is a drop-in for the Sp3MM4AMG C library: the same structs with the
+ * same byte layout, and the same entry-point names and signatures, in the 0-based
+ * (_0) and 1-based Fortran (_1) variants.  Every entry point computes on the
+ * GPU; host arrays are borrowed inputs, outputs are malloc'd host memory that
+ * the caller frees with free()/freeSpmat(), and errors return NULL with a
+ * message on stderr, as in the reference.
+ *
+ * Part 2 is a device-resident interface (no counterpart in the reference) for
+ * callers that keep matrices in HBM: uploads, products on device-resident CSR,
+ * downloads, a batched form and run statistics.
+ *
+ * Reference paths below are relative to the Sp3MM4AMG repository.
+ */
+#ifndef SP3MM_AMD_H
+#define SP3MM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ======================================================================= */
+/* Part 1: reference-compatible types and entry points                     */
+/* ======================================================================= */
+
+/* code/src/include/config.h:51 -- 64-bit unsigned index type */
+typedef unsigned long idx_t;
+
+/* code/src/include/config.h:36-40 -- symbolic row implementation selector.
+ * Both values select the same GPU symbolic phase (LDS hash / bitmap). */
+typedef enum { RBTREE, IDXMAP } ROW_MMSYM_IMPL_MODE;
+
+/* code/src/include/config.h:42-49 -- run configuration.  gridRows/gridCols,
+ * threadNum and chunkDistrbFunc steer the reference's OpenMP loops; they are
+ * accepted and ignored (chunkDistrbFunc is never called). */
+typedef struct {
+    unsigned short gridRows;
+    unsigned short gridCols;
+    ROW_MMSYM_IMPL_MODE symbMMRowImplID;
+    unsigned int threadNum;
+    void *chunkDistrbFunc;
+} CONFIG;
+
+/* code/src/include/sparseMatrix.h:41-53 -- CSR matrix (built without ROWLENS) */
+typedef struct {
+    idx_t NZ, M, N;
+    double *AS;
+    idx_t *JA;
+    idx_t *IRP;
+    idx_t MAX_ROW_NZ;
+} spmat;
+
+/* code/src/include/Sp3MM_CSR_OMP_Multi.h:50-53 */
+typedef spmat *(*SPMM_INTERF)(spmat *, spmat *, CONFIG *);
+typedef spmat *(*SP3MM_INTERF)(spmat *, spmat *, spmat *, CONFIG *, SPMM_INTERF);
+
+/* code/src/Sp3MM_CSR_OMP_Multi.c:54 -- audit timers (seconds).  ElapsedInternal
+ * receives the device compute time of the last product (uploads/downloads
+ * excluded); Elapsed the wall time of the last call. */
+extern double Start, End, Elapsed, ElapsedInternal;
+
+/* --- triple products ---------------------------------------------------- */
+/* code/src/Sp3MM_CSR_OMP_UB_Generic.c:456-494 (declared in
+ * code/src/include/Sp3MM_CSR_OMP_UB_Generic.h:44-50): (R*AC)*P as two SpMMs.
+ * When spmm is NULL or one of this library's SpMM entry points the whole pair
+ * runs on the GPU with the intermediate kept in HBM; any other function pointer
+ * is called twice on the host, as in the reference. */
+spmat *sp3mmRowByRowPair_0(spmat *R, spmat *AC, spmat *P, CONFIG *cfg, SPMM_INTERF spmm);
+spmat *sp3mmRowByRowPair_1(spmat *R, spmat *AC, spmat *P, CONFIG *cfg, SPMM_INTERF spmm);
+
+/* code/src/Sp3MM_CSR_OMP_UB_Generic.c:498-573 (header :52-57): direct row
+ * forwarding; each row of R*AC is built in LDS and forwarded into P without
+ * an intermediate matrix in HBM.  spmm is ignored, as in the reference. */
+spmat *sp3mmRowByRowMerged_0(spmat *R, spmat *AC, spmat *P, CONFIG *cfg, SPMM_INTERF spmm);
+spmat *sp3mmRowByRowMerged_1(spmat *R, spmat *AC, spmat *P, CONFIG *cfg, SPMM_INTERF spmm);
+
+/* --- SpMM C = A*B --------------------------------------------------------- */
+/* All return the same sorted CSR product; in the reference they differ only in
+ * their OpenMP decomposition:
+ *   spmmSerial_              code/src/Sp3MM_CSR_OMP_UB_Generic.c:39-53
+ *   spmmRowByRow_            code/src/Sp3MM_CSR_OMP_UB_Generic.c:56-124
+ *   spmmRowByRow1DBlocks_    code/src/Sp3MM_CSR_OMP_UB_Generic.c:126-201
+ *   spmmRowByRow2DBlocks_    code/src/Sp3MM_CSR_OMP_UB_Generic.c:205-323
+ *   spmmRowByRow2DBlocksAllocated_  code/src/Sp3MM_CSR_OMP_UB_Generic.c:325-454
+ *   spmmRowByRow_SymbNum_    code/src/Sp3MM_CSR_OMP_Num_Generic.c:93-147
+ *   spmmRowByRow1DBlocks_SymbNum_   code/src/Sp3MM_CSR_OMP_Num_Generic.c:148-210
+ *   spmmRowByRow2DBlocks_SymbNum_   code/src/Sp3MM_CSR_OMP_Num_Generic.c:211-239
+ *       (a stub that always returns NULL in the reference; here it computes the
+ *        product like the others). */
+spmat *spmmSerial_0(spmat *A, spmat *B, CONFIG *cfg);
+spmat *spmmSerial_1(spmat *A, spmat *B, CONFIG *cfg);
+spmat *spmmRowByRow_0(spmat *A, spmat *B, CONFIG *cfg);
+spmat *spmmRowByRow_1(spmat *A, spmat *B, CONFIG *cfg);
+spmat *spmmRowByRow1DBlocks_0(spmat *A, spmat *B, CONFIG *cfg);
+spmat *spmmRowByRow1DBlocks_1(spmat *A, spmat *B, CONFIG *cfg);
+spmat *spmmRowByRow2DBlocks_0(spmat *A, spmat *B, CONFIG *cfg);
+spmat *spmmRowByRow2DBlocks_1(spmat *A, spmat *B, CONFIG *cfg);
+spmat *spmmRowByRow2DBlocksAllocated_0(spmat *A, spmat *B, CONFIG *cfg);
+spmat *spmmRowByRow2DBlocksAllocated_1(spmat *A, spmat *B, CONFIG *cfg);
+spmat *spmmRowByRow_SymbNum_0(spmat *A, spmat *B, CONFIG *cfg);
+spmat *spmmRowByRow_SymbNum_1(spmat *A, spmat *B, CONFIG *cfg);
+spmat *spmmRowByRow1DBlocks_SymbNum_0(spmat *A, spmat *B, CONFIG *cfg);
+spmat *spmmRowByRow1DBlocks_SymbNum_1(spmat *A, spmat *B, CONFIG *cfg);
+spmat *spmmRowByRow2DBlocks_SymbNum_0(spmat *A, spmat *B, CONFIG *cfg);
+spmat *spmmRowByRow2DBlocks_SymbNum_1(spmat *A, spmat *B, CONFIG *cfg);
+
+/* --- sizing and symbolic phases (arrays are malloc'd; caller frees) ----- */
+/* code/src/include/SpMMUtilsGeneric.h:112-139: ub[r] = sum_{k in A_r} nnz(B_k),
+ * ub[A->M] = total; A->M+1 entries. */
+idx_t *spMMSizeUpperbound_0(spmat *A, spmat *B);
+idx_t *spMMSizeUpperbound_1(spmat *A, spmat *B);
+
+/* code/src/include/SpMMUtilsGeneric.h:146-174: the same bound split over
+ * gridCols column ranges of B.  bColPartOffsets holds, for every row of B and
+ * every range, the index into B->JA where the range starts (B->M*gridCols+1
+ * entries, as built by colsOffsetsPartitioningUnifRanges_,
+ * code/src/commons/sparseUtilsGeneric.c:284-307); A->M*gridCols+1 entries out. */
+idx_t *spMMSizeUpperboundColParts_0(spmat *A, spmat *B, unsigned short gridCols,
+                                    idx_t *bColPartOffsets);
+idx_t *spMMSizeUpperboundColParts_1(spmat *A, spmat *B, unsigned short gridCols,
+                                    idx_t *bColPartOffsets);
+
+/* code/src/Sp3MM_CSR_OMP_SymbStep_Generic.c:281-417: exact row lengths of A*B
+ * plus the total in the last entry (A->M+1 entries).  Both modes are served by
+ * the same GPU symbolic kernels. */
+idx_t *SpMM_Symb___0(ROW_MMSYM_IMPL_MODE mode, spmat *a, spmat *b);
+idx_t *SpMM_Symb___1(ROW_MMSYM_IMPL_MODE mode, spmat *a, spmat *b);
+
+/* code/src/Sp3MM_CSR_OMP_SymbStep_Generic.c:281-417 built with OUT_IDXS
+ * (code/src/Sp3MM_CSR_OMP_SymbStep_Multi.c:59-92): row lengths as above and,
+ * when outIdxs != NULL, *outIdxs = malloc'd array of a->M pointers to each
+ * row's sorted column indices.  All rows live in one allocation starting at
+ * (*outIdxs)[0]; the caller frees (*outIdxs)[0] and then *outIdxs. */
+idx_t *SpMM_Symb_OutIdxs__0(ROW_MMSYM_IMPL_MODE mode, spmat *a, spmat *b, idx_t ***outIdxs);
+idx_t *SpMM_Symb_OutIdxs__1(ROW_MMSYM_IMPL_MODE mode, spmat *a, spmat *b, idx_t ***outIdxs);
+
+/* code/src/Sp3MM_CSR_OMP_SymbStep_Generic.c:453-579: exact row lengths of
+ * a*b*c by direct row forwarding, plus the total (a->M+1 entries), and the
+ * sorted row indices as for SpMM_Symb_OutIdxs__ when outIdxs != NULL. */
+idx_t *Sp3MM_Symb_OutIdxs_0(ROW_MMSYM_IMPL_MODE mode, spmat *a, spmat *b, spmat *c,
+                            idx_t ***outIdxs);
+idx_t *Sp3MM_Symb_OutIdxs_1(ROW_MMSYM_IMPL_MODE mode, spmat *a, spmat *b, spmat *c,
+                            idx_t ***outIdxs);
+
+/* --- helpers the reference's callers link against ------------------------ */
+/* code/src/include/sparseMatrix.h:188-214 */
+spmat *allocSpMatrix(idx_t rows, idx_t cols);
+int allocSpMatrixInternal(idx_t rows, idx_t cols, spmat *mat);
+/* code/src/include/sparseMatrix.h:154-168 */
+void freeSpmatInternal(spmat *mat);
+void freeSpmat(spmat *mat);
+/* code/src/include/sparseUtilsGeneric.h:64-72 */
+void C_FortranShiftIdxs(spmat *m);
+void Fortran_C_ShiftIdxs(spmat *m);
+
+/* ======================================================================= */
+/* Part 2: device-resident interface (no counterpart in the reference)     */
+/* ======================================================================= */
+
+enum { SP3MM_F64 = 0, SP3MM_F32 = 1 };
+enum { SP3MM_PATH_PAIR = 0, SP3MM_PATH_DIRECT = 1 };
+enum { SP3MM_PREC_FP64 = 0, SP3MM_PREC_MIXED = 1 };
+
+/* A sorted CSR matrix in HBM, 0-based.  rp has M+1 int64 offsets; rp32 holds
+ * the same offsets as int32 whenever nnz < 2^31 (it is what the gathers read);
+ * col holds int32 column indices (N < 2^31); val holds doubles (SP3MM_F64) or
+ * floats (SP3MM_F32). */
+typedef struct {
+    int64_t M, N, nnz;
+    int64_t *rp;
+    int32_t *rp32;
+    int32_t *col;
+    void *val;
+    int32_t vtype;
+    int32_t owned;
+} sp3mm_dmat;
+
+typedef struct {
+    int32_t path;        /* SP3MM_PATH_PAIR or SP3MM_PATH_DIRECT */
+    int32_t precision;   /* SP3MM_PREC_FP64, or SP3MM_PREC_MIXED (direct path, FP32
+                            operands, FP64 output) */
+    double guard;        /* MIXED: a row holding an off-diagonal entry with
+                            |value| < guard * (|R||A||P|)_ij is recomputed in FP64;
+                            diagonal entries are always summed in FP64.
+                            <= 0 selects the default 1e-2 */
+    int32_t timing;      /* 1: time each kernel with HIP events (adds syncs) */
+    int32_t reserved;
+    double *rowsum;      /* MIXED, optional: device array of R->M doubles that
+                            receives the FP64 row sums of R*A*P */
+} sp3mm_opts;
+
+typedef struct {
+    double ms_total;       /* device time of the whole product                 */
+    double ms_ub;          /* upper bounds and binning                          */
+    double ms_symbolic;    /* symbolic phase(s)                                */
+    double ms_scan;        /* row-offset scans                                  */
+    double ms_numeric;     /* numeric phase(s)                                  */
+    double ms_main;        /* the dominant kernel (numeric of the last product) */
+    int64_t mac1, mac2;    /* multiply-adds of R*A and of (RA)*P                */
+    int64_t nnz_mid;       /* nnz of R*A (pair path) or its symbolic count      */
+    int64_t nnz_out;
+    int64_t guarded_rows;  /* MIXED: rows recomputed in FP64                    */
+    int64_t guarded_entries;
+    int32_t main_kernel_launches;
+    int32_t reserved;
+} sp3mm_stats;
+
+/* Select (and initialise) the HIP device; called implicitly with device 0. */
+int sp3mm_init(int device);
+/* Text of the last error of the calling thread ("" if none). */
+const char *sp3mm_last_error(void);
+/* Block until all queued device work of the library has finished. */
+int sp3mm_sync(void);
+
+/* Upload host CSR (int64 row offsets and columns in index base `base`,
+ * values of type vtype) after validating shape, sortedness and ranges. */
+int sp3mm_dmat_upload(sp3mm_dmat *out, int64_t M, int64_t N, int64_t nnz,
+                      const int64_t *irp, const int64_t *ja, const void *val,
+                      int32_t vtype, int32_t base);
+/* Download to host arrays of M+1, nnz, nnz entries (values as stored). */
+int sp3mm_dmat_download(const sp3mm_dmat *m, int64_t *irp, int64_t *ja, void *val,
+                        int32_t base);
+/* Convert values to another type in place (allocates a new value array). */
+int sp3mm_dmat_convert(sp3mm_dmat *m, int32_t vtype);
+void sp3mm_dmat_free(sp3mm_dmat *m);
+
+/* out = R*A*P on device-resident operands. */
+int sp3mm_rap(const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P,
+              sp3mm_dmat *out, const sp3mm_opts *opts, sp3mm_stats *stats);
+/* out = A*B. */
+int sp3mm_spgemm(const sp3mm_dmat *A, const sp3mm_dmat *B, sp3mm_dmat *out,
+                 sp3mm_stats *stats);
+/* ub (host, A->M+1 int64) = row upper bounds of A*B, last entry the total. */
+int sp3mm_ub(const sp3mm_dmat *A, const sp3mm_dmat *B, int64_t *ub);
+/* rowlens (host, A->M+1 int64) = exact row lengths of A*B (C != NULL: of A*B*C). */
+int sp3mm_symbolic(const sp3mm_dmat *A, const sp3mm_dmat *B, const sp3mm_dmat *C,
+                   int64_t *rowlens);
+
+/* Batched products: `count` independent triples (R[i], A[i], P[i]) -- levels
+ * of an AMG hierarchy and/or unrelated problems of uneven sizes -- packed
+ * block-diagonally into one product.  create() packs the operands and runs the
+ * product once (sizing bins, scratch and output); with use_graph the whole
+ * launch sequence is captured into one HIP graph.  run() recomputes every
+ * product (operand values may change, patterns may not; a changed pattern is
+ * reported as an error) with one graph launch or a plain launch sequence, and
+ * returns the device time in *ms.  result() copies product i into a new
+ * device matrix.  Direct path; FP64 operands, or FP32 operands with
+ * opts->precision == SP3MM_PREC_MIXED. */
+typedef struct sp3mm_batch sp3mm_batch;
+int sp3mm_batch_create(int32_t count, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P,
+                       const sp3mm_opts *opts, int32_t use_graph, sp3mm_batch **out);
+int sp3mm_batch_run(sp3mm_batch *b, int32_t use_graph, double *ms);
+int sp3mm_batch_result(sp3mm_batch *b, int32_t i, sp3mm_dmat *out);
+int sp3mm_batch_info(sp3mm_batch *b, double *ms_pack, double *ms_build, int64_t *nnz, int32_t *has_graph);
+void sp3mm_batch_free(sp3mm_batch *b);
+
+/* Device memory from the library's pool (e.g. for sp3mm_opts.rowsum), its
+ * release, and a blocking device-to-host copy. */
+void *sp3mm_device_alloc(int64_t bytes);
+void sp3mm_device_free(void *p);
+int sp3mm_copy_to_host(void *dst, const void *src, int64_t bytes);
+
+/* Compute units, LDS bytes per workgroup on the device, LDS bytes the bin
+ * tables are allowed to use. */
+int sp3mm_device_info(int32_t *num_cu, int64_t *lds_per_block, int64_t *lds_used);
+
+/* Time a device-to-device copy of `bytes` bytes (HBM bandwidth probe); returns ms. */
+double sp3mm_copy_probe(int64_t bytes, int32_t reps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SP3MM_AMD_H */

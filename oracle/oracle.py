@@ -170,3 +170,22 @@ def rel_errors(got: CSR, ref: CSR, scale: CSR | None = None) -> np.ndarray:
         den = np.maximum(den, np.finfo(np.float64).eps * scale.vals)
     den = np.where(den == 0, 1.0, den)
     return diff / den
+
+
+def error_bound(R: CSR, A: CSR, P: CSR, want: CSR, rel: float = 1e-12) -> np.ndarray:
+    """Per-nonzero tolerance for comparing two FP64 evaluations of R*A*P that
+    sum the same products in different orders:
+
+        |x - y| <= rel * |y| + 4 * T_r * u * (|R||A||P|)_ij
+
+    u = 2^-53 and T_r = products of row r in both stages (ub of R*A plus ub of
+    (RA)*P).  The second term is the first-order bound of recursive summation,
+    which any order (the reference's own included) satisfies against the exact
+    value; it only matters for entries that cancel far below their scale."""
+    scale = abs_product(R, A, P)
+    if not scale.pattern_equal(want):
+        raise AssertionError("scale pattern differs")
+    ub1 = ub(R, A)[:-1]
+    ub2 = ub(spmm(R, A), P)[:-1]
+    T = (ub1 + ub2)[want.row_ids()].astype(np.float64)
+    return rel * np.abs(want.vals) + 4.0 * T * 2.0 ** -53 * scale.vals

@@ -1,0 +1,348 @@
+// api.hip -- the C ABI declared in include/sp3mm_amd.h.
+//
+// Device-resident functions (sp3mm_*) and the Sp3MM4AMG-compatible drop-in
+// entry points (_0 / _1).  A drop-in call uploads the borrowed host spmat
+// operands (64-bit indices, narrowed to int32 on the GPU and shifted to
+// 0-based there), runs the product on the GPU and downloads the result into
+// malloc'd host arrays, widening indices back to 64 bits (and to 1-based for
+// the _1 variants) on the GPU before the copy.
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <chrono>
+
+#include "engine.hpp"
+
+namespace sp3mm {
+int init_device(int dev);
+const char *last_error_text();
+}  // namespace sp3mm
+
+using namespace sp3mm;
+
+extern "C" {
+
+double Start = 0, End = 0, Elapsed = 0, ElapsedInternal = 0;
+
+// ---------------------------------------------------------------------------
+// device-resident interface
+// ---------------------------------------------------------------------------
+int sp3mm_init(int device) { return init_device(device); }
+
+const char *sp3mm_last_error(void) { return last_error_text(); }
+
+int sp3mm_sync(void)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    SP3MM_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sp3mm_dmat_upload(sp3mm_dmat *out, int64_t M, int64_t N, int64_t nnz, const int64_t *irp,
+                      const int64_t *ja, const void *val, int32_t vtype, int32_t base)
+{
+    clear_error();
+    if (!out || !irp || (nnz && (!ja || !val))) {
+        set_error("upload: null argument");
+        return -1;
+    }
+    if (M < 0 || N < 0 || nnz < 0 || N >= INT32_MAX || M >= INT32_MAX) {
+        set_error("upload: unsupported shape %lld x %lld (%lld nnz)", (long long)M, (long long)N,
+                  (long long)nnz);
+        return -1;
+    }
+    if ((vtype != SP3MM_F64 && vtype != SP3MM_F32) || (base != 0 && base != 1)) {
+        set_error("upload: bad value type or index base");
+        return -1;
+    }
+    // row offsets are checked on the host before any device work: monotone,
+    // from base to nnz+base (the column kernel trusts them for its bounds)
+    if (irp[0] != base || irp[M] != nnz + base) {
+        set_error("upload: row offsets must run from %d to nnz+%d (got %lld .. %lld, nnz %lld)", base, base,
+                  (long long)irp[0], (long long)irp[M], (long long)nnz);
+        return -1;
+    }
+    for (int64_t r = 0; r < M; r++) {
+        if (irp[r + 1] < irp[r]) {
+            set_error("upload: row offsets decrease at row %lld", (long long)r);
+            return -1;
+        }
+    }
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    hipStream_t s = c->stream;
+    sp3mm_dmat m;
+    if (dmat_alloc(m, M, N, nnz, vtype)) return -1;
+    Buf trp((size_t)(M + 1) * 8), tja((size_t)(nnz ? nnz : 1) * 8), err(64);
+    if (!trp.p || !tja.p || !err.p) {
+        dmat_release(m);
+        set_error("upload: staging allocation failed");
+        return -1;
+    }
+    int rc = 0;
+    do {
+        if (hipMemsetAsync(err.p, 0, 64, s) != hipSuccess) { rc = -1; break; }
+        if (hipMemcpyAsync(trp.p, irp, (size_t)(M + 1) * 8, hipMemcpyHostToDevice, s) != hipSuccess) {
+            rc = -1;
+            break;
+        }
+        if (nnz && hipMemcpyAsync(tja.p, ja, (size_t)nnz * 8, hipMemcpyHostToDevice, s) != hipSuccess) {
+            rc = -1;
+            break;
+        }
+        if (nnz && hipMemcpyAsync(m.val, val, (size_t)nnz * (vtype == SP3MM_F32 ? 4 : 8),
+                                  hipMemcpyHostToDevice, s) != hipSuccess) {
+            rc = -1;
+            break;
+        }
+        launch_rp_convert(trp.as<int64_t>(), M, nnz, base, m.rp, m.rp32, err.as<int>(), s);
+        launch_col_convert(tja.as<int64_t>(), m.rp, M, N, base, m.col, err.as<int>(), s);
+        int herr = 0;
+        if (hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, s) != hipSuccess) { rc = -1; break; }
+        if (hipStreamSynchronize(s) != hipSuccess) { rc = -1; break; }
+        if (herr) {
+            set_error("upload: matrix is not valid sorted CSR (offsets must be monotone from %d to "
+                      "nnz+%d, columns in [%d, N+%d) and strictly increasing inside each row)",
+                      base, base, base, base);
+            rc = -2;
+        }
+    } while (0);
+    if (rc) {
+        if (rc == -1) set_error("upload: HIP error %s", hipGetErrorString(hipGetLastError()));
+        dmat_release(m);
+        return -1;
+    }
+    *out = m;
+    return 0;
+}
+
+int sp3mm_dmat_download(const sp3mm_dmat *m, int64_t *irp, int64_t *ja, void *val, int32_t base)
+{
+    clear_error();
+    if (!m || !irp) {
+        set_error("download: null argument");
+        return -1;
+    }
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    hipStream_t s = c->stream;
+    const int64_t M = m->M, nnz = m->nnz;
+    Buf trp((size_t)(M + 1) * 8), tja((size_t)(nnz ? nnz : 1) * 8);
+    if (!trp.p || !tja.p) {
+        set_error("download: staging allocation failed");
+        return -1;
+    }
+    launch_rp_shift(m->rp, M + 1, base, trp.as<int64_t>(), s);
+    SP3MM_HIP(hipMemcpyAsync(irp, trp.p, (size_t)(M + 1) * 8, hipMemcpyDeviceToHost, s));
+    if (nnz) {
+        launch_col_widen(m->col, nnz, base, tja.as<int64_t>(), s);
+        if (ja) SP3MM_HIP(hipMemcpyAsync(ja, tja.p, (size_t)nnz * 8, hipMemcpyDeviceToHost, s));
+        if (val)
+            SP3MM_HIP(hipMemcpyAsync(val, m->val, (size_t)nnz * (m->vtype == SP3MM_F32 ? 4 : 8),
+                                     hipMemcpyDeviceToHost, s));
+    }
+    SP3MM_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int sp3mm_dmat_convert(sp3mm_dmat *m, int32_t vtype)
+{
+    clear_error();
+    if (!m) return -1;
+    if (m->vtype == vtype) return 0;
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    const int64_t n = m->nnz;
+    void *nv = c->pool.alloc((size_t)(n ? n : 1) * (vtype == SP3MM_F32 ? 4 : 8));
+    if (!nv) {
+        set_error("convert: allocation failed");
+        return -1;
+    }
+    if (vtype == SP3MM_F32)
+        launch_f64_to_f32(static_cast<const double *>(m->val), n, static_cast<float *>(nv), c->stream);
+    else
+        launch_f32_to_f64(static_cast<const float *>(m->val), n, static_cast<double *>(nv), c->stream);
+    SP3MM_HIP(hipStreamSynchronize(c->stream));
+    if (m->owned) c->pool.release(m->val);
+    m->val = nv;
+    m->vtype = vtype;
+    return 0;
+}
+
+void sp3mm_dmat_free(sp3mm_dmat *m)
+{
+    if (!m) return;
+    Context *c = ctx();
+    if (c) (void)hipStreamSynchronize(c->stream);
+    dmat_release(*m);
+}
+
+int sp3mm_rap(const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P, sp3mm_dmat *out,
+              const sp3mm_opts *opts, sp3mm_stats *stats)
+{
+    clear_error();
+    if (!R || !A || !P || !out) {
+        set_error("rap: null argument");
+        return -1;
+    }
+    sp3mm_opts o{};
+    if (opts) o = *opts;
+    memset(out, 0, sizeof *out);
+    if (o.precision == SP3MM_PREC_MIXED) {
+        if (o.path != SP3MM_PATH_DIRECT) {
+            set_error("rap: the mixed-precision product runs on the direct path only");
+            return -1;
+        }
+        return rap_direct_mixed(*R, *A, *P, *out, stats, o.guard, o.rowsum);
+    }
+    if (o.precision != SP3MM_PREC_FP64) {
+        set_error("rap: unknown precision %d", o.precision);
+        return -1;
+    }
+    if (o.path == SP3MM_PATH_DIRECT) return rap_direct(*R, *A, *P, *out, stats, o.timing != 0);
+    return rap_pair(*R, *A, *P, *out, stats, o.timing != 0);
+}
+
+int sp3mm_spgemm(const sp3mm_dmat *A, const sp3mm_dmat *B, sp3mm_dmat *out, sp3mm_stats *stats)
+{
+    clear_error();
+    if (!A || !B || !out) {
+        set_error("spgemm: null argument");
+        return -1;
+    }
+    memset(out, 0, sizeof *out);
+    return spgemm(*A, *B, *out, stats);
+}
+
+int sp3mm_ub(const sp3mm_dmat *A, const sp3mm_dmat *B, int64_t *ub)
+{
+    clear_error();
+    if (!A || !B || !ub) return -1;
+    return row_ub(*A, *B, ub);
+}
+
+int sp3mm_symbolic(const sp3mm_dmat *A, const sp3mm_dmat *B, const sp3mm_dmat *C, int64_t *rowlens)
+{
+    clear_error();
+    if (!A || !B || !rowlens) return -1;
+    sp3mm_dmat out;
+    memset(&out, 0, sizeof out);
+    int rc = C ? rap_direct(*A, *B, *C, out, nullptr, false, true) : spgemm(*A, *B, out, nullptr, true);
+    if (rc) return rc;
+    std::vector<int64_t> rp((size_t)out.M + 1);
+    Context *c = ctx();
+    if (hipMemcpy(rp.data(), out.rp, (size_t)(out.M + 1) * 8, hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("symbolic: copy failed");
+        dmat_release(out);
+        return -1;
+    }
+    (void)c;
+    for (int64_t r = 0; r < out.M; r++) rowlens[r] = rp[r + 1] - rp[r];
+    rowlens[out.M] = rp[out.M];
+    dmat_release(out);
+    return 0;
+}
+
+int sp3mm_device_info(int32_t *num_cu, int64_t *lds_per_block, int64_t *lds_used)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    if (num_cu) *num_cu = c->num_cu;
+    if (lds_per_block) *lds_per_block = (int64_t)c->lds_device;
+    if (lds_used) *lds_used = (int64_t)c->max_lds;
+    return 0;
+}
+
+int sp3mm_batch_create(int32_t count, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P,
+                       const sp3mm_opts *opts, int32_t use_graph, sp3mm_batch **out)
+{
+    clear_error();
+    if (!R || !A || !P || !out) {
+        set_error("batch: null argument");
+        return -1;
+    }
+    sp3mm_opts o{};
+    if (opts) o = *opts;
+    if (o.path != SP3MM_PATH_DIRECT) {
+        set_error("batch: batches run on the direct path");
+        return -1;
+    }
+    void *b = batch_create(count, R, A, P, o.precision == SP3MM_PREC_MIXED, o.guard, use_graph != 0);
+    *out = static_cast<sp3mm_batch *>(b);
+    return b ? 0 : -1;
+}
+
+int sp3mm_batch_run(sp3mm_batch *b, int32_t use_graph, double *ms)
+{
+    clear_error();
+    return batch_run(b, use_graph != 0, ms);
+}
+
+int sp3mm_batch_result(sp3mm_batch *b, int32_t i, sp3mm_dmat *out)
+{
+    clear_error();
+    if (!out) return -1;
+    memset(out, 0, sizeof *out);
+    return batch_result(b, i, *out);
+}
+
+int sp3mm_batch_info(sp3mm_batch *b, double *ms_pack, double *ms_build, int64_t *nnz, int32_t *has_graph)
+{
+    if (!b) return -1;
+    batch_info(b, ms_pack, ms_build, nnz, has_graph);
+    return 0;
+}
+
+void sp3mm_batch_free(sp3mm_batch *b) { batch_free(b); }
+
+void *sp3mm_device_alloc(int64_t bytes)
+{
+    Context *c = ctx();
+    if (!c || bytes < 0) return nullptr;
+    return c->pool.alloc((size_t)(bytes ? bytes : 1));
+}
+
+void sp3mm_device_free(void *p)
+{
+    Context *c = ctx();
+    if (!c || !p) return;
+    (void)hipStreamSynchronize(c->stream);
+    c->pool.release(p);
+}
+
+int sp3mm_copy_to_host(void *dst, const void *src, int64_t bytes)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    SP3MM_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, c->stream));
+    SP3MM_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+double sp3mm_copy_probe(int64_t bytes, int32_t reps)
+{
+    Context *c = ctx();
+    if (!c || bytes <= 0 || reps <= 0) return -1.0;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    Buf a((size_t)bytes), b((size_t)bytes);
+    if (!a.p || !b.p) return -1.0;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    launch_copy_probe(a.p, b.p, bytes, c->stream);
+    (void)hipEventRecord(e0, c->stream);
+    for (int i = 0; i < reps; i++) launch_copy_probe(a.p, b.p, bytes, c->stream);
+    (void)hipEventRecord(e1, c->stream);
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return (double)ms / reps;
+}
+
+}  // extern "C"

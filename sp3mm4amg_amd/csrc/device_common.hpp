@@ -1,0 +1,182 @@
+// device_common.hpp -- wavefront/workgroup helpers and LDS hash tables for the
+// gfx950 Sp3MM kernels.  CDNA wavefronts are 64 lanes; a "group" is either one
+// wavefront (G == 64, one output row per wavefront) or a whole workgroup
+// (G > 64, one output row per workgroup).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sp3mm {
+
+constexpr int kWave = 64;
+constexpr int kEmpty = -1;
+
+// Error bits raised by kernels instead of hanging or writing out of bounds.
+enum : int {
+    kErrHashFull = 1,     // an LDS hash table ran out of slots (sizing bug)
+    kErrOverflow = 2,     // an output row did not fit the space reserved for it
+    kErrBadInput = 4,     // input validation failed
+    kErrSpin = 8,         // a bounded wait gave up
+};
+
+__device__ __forceinline__ int lane() { return (int)__lane_id(); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+// number of set bits of `mask` strictly below this lane
+__device__ __forceinline__ int lanes_below(uint64_t mask)
+{
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+}
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int G>
+struct Group {
+    static_assert(G % kWave == 0, "group must be whole wavefronts");
+    static constexpr int kWaves = G / kWave;
+    __device__ static int tid() { return (int)(threadIdx.x % G); }
+    __device__ static void sync()
+    {
+        if constexpr (G == kWave) wave_sync();
+        else __syncthreads();
+    }
+};
+
+// sum of an int over the group; `red` is kWaves ints of LDS private to the group
+template <int G>
+__device__ __forceinline__ long long group_sum(long long v, long long *red)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if constexpr (G == kWave) {
+        return v;
+    } else {
+        const int w = Group<G>::tid() / kWave;
+        __syncthreads();
+        if (lane() == 0) red[w] = v;
+        __syncthreads();
+        long long t = 0;
+        for (int i = 0; i < Group<G>::kWaves; i++) t += red[i];
+        __syncthreads();
+        return t;
+    }
+}
+
+template <int G>
+__device__ __forceinline__ long long group_max(long long v, long long *red)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        long long u = __shfl_xor(v, o);
+        v = u > v ? u : v;
+    }
+    if constexpr (G == kWave) {
+        return v;
+    } else {
+        const int w = Group<G>::tid() / kWave;
+        __syncthreads();
+        if (lane() == 0) red[w] = v;
+        __syncthreads();
+        long long t = red[0];
+        for (int i = 1; i < Group<G>::kWaves; i++) t = red[i] > t ? red[i] : t;
+        __syncthreads();
+        return t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// LDS hash table with linear probing.  Keys are column indices (>= 0), the
+// empty slot is -1.  `mask` = H-1 for a power-of-two table of H slots.  A probe
+// sequence is bounded by H: a full table raises kErrHashFull instead of
+// spinning forever.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned hslot(int key, int shift)
+{
+    return ((unsigned)key * 0x9E3779B1u) >> shift;
+}
+
+// returns slot index, or -1 when the table is full; *fresh = 1 for a new key.
+// One compare-and-swap per probe: an empty slot is claimed, a slot already
+// holding the key is found by the failed swap's return value.
+__device__ __forceinline__ int hash_insert(int *keys, unsigned mask, int shift, int key,
+                                           int *fresh)
+{
+    unsigned s = hslot(key, shift) & mask;
+    for (unsigned probe = 0; probe <= mask; probe++) {
+        const int old = atomicCAS(keys + s, kEmpty, key);
+        if (old == kEmpty) { *fresh = 1; return (int)s; }
+        if (old == key) { *fresh = 0; return (int)s; }
+        s = (s + 1) & mask;
+    }
+    *fresh = 0;
+    return -1;
+}
+
+// log2 of a power of two
+__host__ __device__ __forceinline__ int ilog2(unsigned v)
+{
+    int r = 0;
+    while ((1u << r) < v) r++;
+    return r;
+}
+
+// In-place compaction of the occupied slots of a table of H slots (H a multiple
+// of G) to its front, preserving slot order.  Values (if any) move with keys.
+// Returns the number of occupied slots.  `red` holds kWaves+1 ints.
+template <int G, typename V>
+__device__ int compact_table(int *keys, V *vals, int H, int *red)
+{
+    const int t = Group<G>::tid();
+    int base = 0;
+    for (int s0 = 0; s0 < H; s0 += G) {
+        const int s = s0 + t;
+        const int k = keys[s];
+        V v{};
+        if (vals) v = vals[s];
+        const bool occ = k != kEmpty;
+        const uint64_t m = ballot(occ);
+        int pos = lanes_below(m);
+        int tot = __popcll(m);
+        if constexpr (G > kWave) {
+            const int w = t / kWave;
+            __syncthreads();
+            if (lane() == 0) red[w] = tot;
+            __syncthreads();
+            int before = 0;
+            tot = 0;
+            for (int i = 0; i < Group<G>::kWaves; i++) {
+                before += (i < w) ? red[i] : 0;
+                tot += red[i];
+            }
+            pos += before;
+        } else {
+            wave_sync();
+        }
+        if (occ) {
+            keys[base + pos] = k;
+            if (vals) vals[base + pos] = v;
+        }
+        base += tot;
+        Group<G>::sync();
+    }
+    return base;
+}
+
+// Rank of every compacted key among the first n keys (keys are distinct).
+__device__ __forceinline__ int key_rank(const int *keys, int n, int k)
+{
+    int r = 0;
+    int j = 0;
+    for (; j + 4 <= n; j += 4) {
+        r += (keys[j] < k) + (keys[j + 1] < k) + (keys[j + 2] < k) + (keys[j + 3] < k);
+    }
+    for (; j < n; j++) r += keys[j] < k;
+    return r;
+}
+
+}  // namespace sp3mm

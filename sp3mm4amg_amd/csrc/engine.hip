@@ -1,0 +1,1342 @@
+// engine.hip -- device context, HBM pool and the product pipelines.
+//
+// A product runs as: upper bounds + binning -> per-bin symbolic kernels ->
+// scan + re-binning by exact sizes -> per-bin numeric kernels, all on one HIP
+// stream.  The host reads small counters back twice per product (bin
+// populations, then output size) to size grids and the output arrays.
+#include <cstdarg>
+#include <cstdlib>
+#include <cstdio>
+#include <cstring>
+
+#include "device_common.hpp"
+#include "engine.hpp"
+
+namespace sp3mm {
+
+static thread_local char g_err[1024];
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+void clear_error() { g_err[0] = 0; }
+const char *last_error_text() { return g_err; }
+
+// --------------------------------------------------------------------------
+// Pool
+// --------------------------------------------------------------------------
+size_t Pool::size_class(size_t b)
+{
+    if (b <= 4096) return 4096;
+    const int lg = 63 - __builtin_clzll((unsigned long long)b);
+    const size_t g = (size_t)1 << (lg - 3);
+    return (b + g - 1) / g * g;
+}
+
+void *Pool::alloc(size_t bytes)
+{
+    const size_t c = size_class(bytes ? bytes : 1);
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = free_.find(c);
+    if (it != free_.end()) {
+        void *p = it->second;
+        free_.erase(it);
+        cached_ -= c;
+        live_[p] = c;
+        return p;
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, c);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        for (auto &kv : free_) (void)hipFree(kv.second);
+        free_.clear();
+        cached_ = 0;
+        e = hipMalloc(&p, c);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            size_t live = 0;
+            for (auto &kv : live_) live += kv.second;
+            set_error("hipMalloc(%zu) failed: %s (library holds %zu bytes in %zu live blocks)", c,
+                      hipGetErrorString(e), live, live_.size());
+            return nullptr;
+        }
+    }
+    live_[p] = c;
+    return p;
+}
+
+void Pool::release(void *p)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = live_.find(p);
+    if (it == live_.end()) return;
+    free_.emplace(it->second, p);
+    cached_ += it->second;
+    live_.erase(it);
+}
+
+void Pool::trim()
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    for (auto &kv : free_) (void)hipFree(kv.second);
+    free_.clear();
+    cached_ = 0;
+}
+
+// --------------------------------------------------------------------------
+// Context
+// --------------------------------------------------------------------------
+static Context *g_ctx = nullptr;
+static std::mutex g_ctx_mu;
+
+int init_device(int dev)
+{
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    if (g_ctx && g_ctx->device == dev) return 0;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        set_error("no HIP device available");
+        return -1;
+    }
+    if (dev < 0 || dev >= n) {
+        set_error("device %d out of range (%d devices)", dev, n);
+        return -1;
+    }
+    Context *c = new Context();
+    c->device = dev;
+    SP3MM_HIP(hipSetDevice(dev));
+    hipDeviceProp_t prop;
+    SP3MM_HIP(hipGetDeviceProperties(&prop, dev));
+    c->num_cu = prop.multiProcessorCount;
+    // dynamic LDS per workgroup the bin tables may use (SP3MM_LDS_CAP overrides)
+    c->lds_device = prop.sharedMemPerBlock;
+    c->max_lds = prop.sharedMemPerBlock;
+    if (const char *cap = getenv("SP3MM_LDS_CAP")) {
+        const size_t v = (size_t)strtoull(cap, nullptr, 10);
+        if (v >= 16384 && v < c->max_lds) c->max_lds = v;
+    }
+    SP3MM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    SP3MM_HIP(hipHostMalloc((void **)&c->host_small, 4096, hipHostMallocDefault));
+    g_ctx = c;
+    return 0;
+}
+
+Context *ctx()
+{
+    if (!g_ctx) {
+        if (init_device(0)) return nullptr;
+    }
+    return g_ctx;
+}
+
+Buf::Buf(size_t b) : bytes(b)
+{
+    Context *c = ctx();
+    p = c ? c->pool.alloc(b) : nullptr;
+}
+
+Buf::~Buf()
+{
+    if (p && g_ctx) g_ctx->pool.release(p);
+}
+
+Buf &Buf::operator=(Buf &&o) noexcept
+{
+    if (this != &o) {
+        if (p && g_ctx) g_ctx->pool.release(p);
+        p = o.p;
+        bytes = o.bytes;
+        o.p = nullptr;
+        o.bytes = 0;
+    }
+    return *this;
+}
+
+SpOperand operand(const sp3mm_dmat &m)
+{
+    SpOperand o;
+    o.rp = m.rp;
+    o.rp32 = m.rp32;
+    o.col = m.col;
+    o.val = m.val;
+    o.M = m.M;
+    o.N = m.N;
+    return o;
+}
+
+int dmat_alloc(sp3mm_dmat &m, int64_t M, int64_t N, int64_t nnz, int vtype)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    memset(&m, 0, sizeof m);
+    m.M = M;
+    m.N = N;
+    m.nnz = nnz;
+    m.vtype = vtype;
+    m.owned = 1;
+    m.rp = static_cast<int64_t *>(c->pool.alloc((size_t)(M + 1) * 8));
+    if (nnz < INT32_MAX) m.rp32 = static_cast<int32_t *>(c->pool.alloc((size_t)(M + 1) * 4));
+    m.col = static_cast<int32_t *>(c->pool.alloc((size_t)(nnz ? nnz : 1) * 4));
+    m.val = c->pool.alloc((size_t)(nnz ? nnz : 1) * (vtype == SP3MM_F32 ? 4 : 8));
+    if (!m.rp || !m.col || !m.val || (nnz < INT32_MAX && !m.rp32)) {
+        char why[600];
+        snprintf(why, sizeof why, "%s", last_error_text());
+        set_error("device allocation of a %lld x %lld matrix with %lld nnz failed: %s", (long long)M,
+                  (long long)N, (long long)nnz, why);
+        return -1;
+    }
+    return 0;
+}
+
+void dmat_release(sp3mm_dmat &m)
+{
+    if (m.owned && g_ctx) {
+        g_ctx->pool.release(m.rp);
+        g_ctx->pool.release(m.rp32);
+        g_ctx->pool.release(m.col);
+        g_ctx->pool.release(m.val);
+    }
+    memset(&m, 0, sizeof m);
+}
+
+// --------------------------------------------------------------------------
+// bin configurations
+// --------------------------------------------------------------------------
+struct BinCfg {
+    int G, H;
+};
+
+// largest power-of-two table (>= 64) whose LDS footprint fits the device
+static int largest_h(size_t max_lds, size_t (*lds)(int, int), int G)
+{
+    int H = 64;
+    while (lds(G, H * 2) <= max_lds && H < (1 << 20)) H *= 2;
+    return H;
+}
+static size_t dsymb_lds(int G, int H) { return direct_lds(G, H, H, false); }
+static size_t dnum_lds(int G, int H) { return direct_lds(G, H, H, true); }
+
+// pairwise symbolic: measure = row upper bound; a table of H >= ub keys cannot
+// fill.  Rows above the last bin are tried with the largest table that fits
+// and raise an error if it fills.
+static void symb_bins(size_t max_lds, BinThresholds &thr, std::vector<BinCfg> &cfg)
+{
+    const int hmax = largest_h(max_lds, symb_lds, 256);
+    thr.nb = 0;
+    cfg.clear();
+    for (long long v = 64; v <= hmax && thr.nb < kMaxBins - 1; v *= 2) {
+        thr.v[thr.nb++] = v;
+        cfg.push_back({v <= 1024 ? 64 : 256, (int)v});
+    }
+    cfg.push_back({256, hmax});
+}
+
+// pairwise numeric: measure = exact row length n, table of 2n slots
+static void num_bins(size_t max_lds, BinThresholds &thr, std::vector<BinCfg> &cfg)
+{
+    const int hmax = largest_h(max_lds, num_lds, 256);
+    thr.nb = 0;
+    cfg.clear();
+    for (long long v = 32; 2 * v <= hmax && thr.nb < kMaxBins - 1; v *= 2) {
+        thr.v[thr.nb++] = v;
+        const int H = (int)(2 * v);
+        cfg.push_back({H <= 512 ? 64 : 256, H});
+    }
+    cfg.push_back({0, 0});  // longer rows are not handled by the LDS kernels
+}
+
+// direct symbolic: measure = upper bound of the R*A row; both tables of H keys.
+// Rows whose second table fills are retried with the largest tables.
+static void dsymb_bins(size_t max_lds, BinThresholds &thr, std::vector<BinCfg> &cfg)
+{
+    const int hmax = largest_h(max_lds, dsymb_lds, 256);
+    thr.nb = 0;
+    cfg.clear();
+    for (long long v = 64; v <= hmax && thr.nb < kMaxBins - 1; v *= 2) {
+        thr.v[thr.nb++] = v;
+        cfg.push_back({v <= 512 ? 64 : 256, (int)v});
+    }
+    cfg.push_back({256, hmax});
+}
+
+// direct numeric: measure = max(nnz of the R*A row, nnz of the RAP row)
+static void dnum_bins(size_t max_lds, BinThresholds &thr, std::vector<BinCfg> &cfg)
+{
+    const int hmax = largest_h(max_lds, dnum_lds, 256);
+    thr.nb = 0;
+    cfg.clear();
+    for (long long v = 32; 2 * v <= hmax && thr.nb < kMaxBins - 1; v *= 2) {
+        thr.v[thr.nb++] = v;
+        const int H = (int)(2 * v);
+        cfg.push_back({H <= 256 ? 64 : 256, H});
+    }
+    cfg.push_back({0, 0});
+}
+
+// lanes per right-hand row: enough that 2^slog lanes x kChunk entries cover a
+// mean right row in one chunk (rounded up to a power of two, at most the group)
+static int sub_log(const sp3mm_dmat &B, int G)
+{
+    const double avg = B.M ? (double)B.nnz / (double)B.M : 1.0;
+    int l = 0;
+    while ((double)(1 << l) * 8.0 < avg && l < 6) l++;
+    const int gl = ilog2((unsigned)G);
+    return l > gl ? gl : l;
+}
+
+// control block layout (device, zeroed per product)
+struct Ctl {
+    int counts[16];
+    int counts2[16];
+    int nretry;
+    int err;
+    int pad[2];
+    unsigned long long totals[2];
+    int64_t nnz;
+    int64_t pad2;
+};
+
+static int max_blocks(const Context *c) { return c->num_cu * 16; }
+
+constexpr int kDenseThreads = 256;
+// persistent workgroups (each with its own dense workspace) for the long-row kernels
+static int dense_blocks(const Context *c, int rows)
+{
+    const int cap = c->num_cu * 2;
+    return rows < cap ? (rows > 0 ? rows : 1) : cap;
+}
+
+// dense workspaces of the direct long-row kernels: bitmap + list over A's
+// columns, bitmap over P's columns, and (numeric) the value arrays
+struct DenseDirectWs {
+    Buf bitsA, valsA, listc, listv, bitsP, valsP;
+    DenseWs ws{};
+    int alloc(int wb, int64_t ncolA, int64_t ncolP, bool num, hipStream_t s)
+    {
+        const int nwA = dense_words(ncolA), nwP = dense_words(ncolP);
+        const size_t slotsA = (size_t)wb * nwA * 32, slotsP = (size_t)wb * nwP * 32;
+        bitsA = Buf((size_t)wb * nwA * 4);
+        listc = Buf(slotsA * 4);
+        bitsP = Buf((size_t)wb * nwP * 4);
+        if (num) {
+            valsA = Buf(slotsA * 8);
+            listv = Buf(slotsA * 8);
+            valsP = Buf(slotsP * 8);
+        }
+        if (!bitsA.p || !listc.p || !bitsP.p || (num && (!valsA.p || !listv.p || !valsP.p))) {
+            set_error("dense workspace allocation failed");
+            return -1;
+        }
+        SP3MM_HIP(hipMemsetAsync(bitsA.p, 0, bitsA.bytes, s));
+        SP3MM_HIP(hipMemsetAsync(bitsP.p, 0, bitsP.bytes, s));
+        if (num) {
+            SP3MM_HIP(hipMemsetAsync(valsA.p, 0, valsA.bytes, s));
+            SP3MM_HIP(hipMemsetAsync(valsP.p, 0, valsP.bytes, s));
+        }
+        ws.bitsA = bitsA.as<unsigned>();
+        ws.valsA = valsA.as<double>();
+        ws.listc = listc.as<int32_t>();
+        ws.listv = listv.as<double>();
+        ws.bitsP = bitsP.as<unsigned>();
+        ws.valsP = valsP.as<double>();
+        ws.nwA = nwA;
+        ws.nwP = nwP;
+        return 0;
+    }
+};
+
+static const char *err_text(int e)
+{
+    if (e & kErrHashFull) return "an LDS hash table overflowed (row too long for the LDS kernels)";
+    if (e & kErrOverflow) return "numeric row length differs from the symbolic row length";
+    if (e & kErrBadInput) return "invalid input matrix";
+    return "device error";
+}
+
+struct Timer {
+    hipStream_t s;
+    bool on;
+    std::vector<hipEvent_t> ev;
+    explicit Timer(hipStream_t st, bool enabled) : s(st), on(enabled) {}
+    ~Timer()
+    {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+    int mark()
+    {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        (void)hipEventRecord(e, s);
+        ev.push_back(e);
+        return (int)ev.size() - 1;
+    }
+    double ms(int a, int b)
+    {
+        float t = 0.f;
+        if (a < 0 || b < 0) return 0.0;
+        (void)hipEventElapsedTime(&t, ev[a], ev[b]);
+        return (double)t;
+    }
+};
+
+static int read_ctl(Context *c, const Ctl *d, Ctl &h)
+{
+    SP3MM_HIP(hipGetLastError());  // a kernel launch that was rejected
+    SP3MM_HIP(hipMemcpyAsync(c->host_small, d, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+    SP3MM_HIP(hipStreamSynchronize(c->stream));
+    memcpy(&h, c->host_small, sizeof(Ctl));
+    return 0;
+}
+
+// --------------------------------------------------------------------------
+// C = A*B
+// --------------------------------------------------------------------------
+static int check_pair(const sp3mm_dmat &A, const sp3mm_dmat &B)
+{
+    if (A.N != B.M) {
+        set_error("inner dimensions differ: A is %lld x %lld, B is %lld x %lld", (long long)A.M,
+                  (long long)A.N, (long long)B.M, (long long)B.N);
+        return -1;
+    }
+    if (!B.rp32) {
+        set_error("right operand has >= 2^31 nonzeros");
+        return -1;
+    }
+    if (A.vtype != SP3MM_F64 || B.vtype != SP3MM_F64) {
+        set_error("FP64 product needs FP64 operands");
+        return -1;
+    }
+    return 0;
+}
+
+int spgemm_impl(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &C, sp3mm_stats *st, Timer *tm,
+                double *ms_num, bool symbolic_only = false)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    if (check_pair(A, B)) return -1;
+    hipStream_t s = c->stream;
+    const int64_t M = A.M;
+    const SpOperand a = operand(A), b = operand(B);
+    BinThresholds thr{}, thr2{};
+    std::vector<BinCfg> cfg, cfg2;
+    symb_bins(c->max_lds, thr, cfg);
+    num_bins(c->max_lds, thr2, cfg2);
+
+    const int nlists = (thr.nb > thr2.nb ? thr.nb : thr2.nb) + 1;
+    Buf ctl(sizeof(Ctl)), ub((size_t)(M + 1) * 8), rown((size_t)(M + 1) * 4),
+        lists((size_t)nlists * (M ? M : 1) * 4), tiles((size_t)scan_tiles(M) * 8);
+    if (!ctl.p || !ub.p || !rown.p || !lists.p || !tiles.p) {
+        set_error("scratch allocation failed");
+        return -1;
+    }
+    Ctl *d = ctl.as<Ctl>();
+    Ctl h;
+    SP3MM_HIP(hipMemsetAsync(d, 0, sizeof(Ctl), s));
+    launch_ub_bin(a, b, ub.as<int64_t>(), rown.as<int32_t>(), lists.as<int32_t>(), d->counts, thr,
+                  d->totals, s);
+    if (read_ctl(c, d, h)) return -1;
+    const long long mac = (long long)h.totals[0];
+
+    for (int i = 0; i < thr.nb; i++) {
+        if (h.counts[i] == 0) continue;
+        const int G = cfg[i].G, H = cfg[i].H;
+        launch_symb(G, H, sub_log(B, G), a, b, lists.as<int32_t>() + (int64_t)i * M, d->counts + i,
+                    h.counts[i], rown.as<int32_t>(), nullptr, nullptr, &d->err, max_blocks(c), s);
+    }
+    const int nwB = dense_words(B.N);
+    if (h.counts[thr.nb] > 0) {  // long rows: dense bitmap in HBM
+        const int wb = dense_blocks(c, h.counts[thr.nb]);
+        Buf bits((size_t)wb * nwB * 4);
+        if (!bits.p) {
+            set_error("dense workspace allocation failed");
+            return -1;
+        }
+        SP3MM_HIP(hipMemsetAsync(bits.p, 0, (size_t)wb * nwB * 4, s));
+        launch_dense_spgemm(false, wb, a, b, lists.as<int32_t>() + (int64_t)thr.nb * M, d->counts + thr.nb,
+                            bits.as<unsigned>(), nullptr, nwB, sub_log(B, kDenseThreads), rown.as<int32_t>(),
+                            nullptr, nullptr, nullptr, &d->err, s);
+    }
+    if (dmat_alloc(C, M, B.N, 0, SP3MM_F64)) return -1;
+    // C.rp32 was sized for nnz 0; it is re-checked once the total is known
+    launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d->nnz, s);
+    launch_rebin(rown.as<int32_t>(), nullptr, M, lists.as<int32_t>(), d->counts2, thr2, s);
+    if (read_ctl(c, d, h)) return -1;
+    if (h.err) {
+        set_error("symbolic phase: %s", err_text(h.err));
+        return -1;
+    }
+    const int64_t nnz = h.nnz;
+    if (symbolic_only) {
+        C.nnz = nnz;
+        if (st) st->mac1 += mac;
+        return 0;
+    }
+    C.nnz = nnz;
+    c->pool.release(C.col);
+    c->pool.release(C.val);
+    C.col = static_cast<int32_t *>(c->pool.alloc((size_t)(nnz ? nnz : 1) * 4));
+    C.val = c->pool.alloc((size_t)(nnz ? nnz : 1) * 8);
+    if (!C.col || !C.val) {
+        set_error("output allocation (%lld nnz) failed", (long long)nnz);
+        return -1;
+    }
+    if (nnz >= INT32_MAX) {
+        c->pool.release(C.rp32);
+        C.rp32 = nullptr;
+    }
+    const int t0 = tm ? tm->mark() : -1;
+    for (int i = 0; i < thr2.nb; i++) {
+        if (h.counts2[i] == 0) continue;
+        const int G = cfg2[i].G, H = cfg2[i].H;
+        launch_num(G, H, sub_log(B, G), a, b, lists.as<int32_t>() + (int64_t)i * M, d->counts2 + i,
+                   h.counts2[i], C.rp, C.col, static_cast<double *>(C.val), &d->err, max_blocks(c), s);
+    }
+    Buf dbits, dvals;
+    if (h.counts2[thr2.nb] > 0) {  // long rows: dense bitmap + values in HBM
+        const int wb = dense_blocks(c, h.counts2[thr2.nb]);
+        dbits = Buf((size_t)wb * nwB * 4);
+        dvals = Buf((size_t)wb * nwB * 32 * 8);
+        if (!dbits.p || !dvals.p) {
+            set_error("dense workspace allocation failed");
+            return -1;
+        }
+        SP3MM_HIP(hipMemsetAsync(dbits.p, 0, (size_t)wb * nwB * 4, s));
+        SP3MM_HIP(hipMemsetAsync(dvals.p, 0, (size_t)wb * nwB * 32 * 8, s));
+        launch_dense_spgemm(true, wb, a, b, lists.as<int32_t>() + (int64_t)thr2.nb * M, d->counts2 + thr2.nb,
+                            dbits.as<unsigned>(), dvals.as<double>(), nwB, sub_log(B, kDenseThreads), nullptr,
+                            C.rp, C.col, static_cast<double *>(C.val), &d->err, s);
+    }
+    const int t1 = tm ? tm->mark() : -1;
+    if (read_ctl(c, d, h)) return -1;
+    if (h.err) {
+        set_error("numeric phase: %s", err_text(h.err));
+        return -1;
+    }
+    if (tm && ms_num) *ms_num += tm->ms(t0, t1);
+    if (st) {
+        st->mac1 += mac;
+        st->nnz_out = nnz;
+    }
+    return 0;
+}
+
+int spgemm(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &C, sp3mm_stats *st, bool symbolic_only)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    Timer tm(c->stream, true);
+    const int e0 = tm.mark();
+    double ms_num = 0;
+    sp3mm_stats local{};
+    const int rc = spgemm_impl(A, B, C, &local, &tm, &ms_num, symbolic_only);
+    if (rc) {
+        dmat_release(C);
+        return rc;
+    }
+    const int e1 = tm.mark();
+    (void)hipEventSynchronize(tm.ev[e1]);
+    if (st) {
+        *st = local;
+        st->ms_total = tm.ms(e0, e1);
+        st->ms_numeric = ms_num;
+        st->ms_main = ms_num;
+    }
+    return 0;
+}
+
+// --------------------------------------------------------------------------
+// R*A*P, pair form (R*A)*P with the intermediate in HBM
+// --------------------------------------------------------------------------
+int rap_pair(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, sp3mm_dmat &C,
+             sp3mm_stats *st, bool timing)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    Timer tm(c->stream, true);
+    const int e0 = tm.mark();
+    sp3mm_stats s1{}, s2{};
+    double ms1 = 0, ms2 = 0;
+    sp3mm_dmat RA;
+    memset(&RA, 0, sizeof RA);
+    if (spgemm_impl(R, A, RA, &s1, &tm, &ms1)) {
+        dmat_release(RA);
+        return -1;
+    }
+    const int rc = spgemm_impl(RA, P, C, &s2, &tm, &ms2);
+    const int64_t mid = RA.nnz;
+    dmat_release(RA);
+    if (rc) {
+        dmat_release(C);
+        return rc;
+    }
+    const int e1 = tm.mark();
+    (void)hipEventSynchronize(tm.ev[e1]);
+    if (st) {
+        memset(st, 0, sizeof *st);
+        st->ms_total = tm.ms(e0, e1);
+        st->ms_numeric = ms1 + ms2;
+        st->ms_main = ms2;
+        st->mac1 = s1.mac1;
+        st->mac2 = s2.mac1;
+        st->nnz_mid = mid;
+        st->nnz_out = C.nnz;
+        st->main_kernel_launches = 1;
+    }
+    (void)timing;
+    return 0;
+}
+
+// --------------------------------------------------------------------------
+// R*A*P, direct form: rows of R*A never leave the CU
+// --------------------------------------------------------------------------
+static int rap_direct_twopass(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, sp3mm_dmat &C,
+                              sp3mm_stats *st, bool timing, bool symbolic_only)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (check_pair(R, A) || check_pair(A, P)) return -1;
+    if (R.vtype != SP3MM_F64) {
+        set_error("FP64 product needs FP64 operands");
+        return -1;
+    }
+    hipStream_t s = c->stream;
+    Timer tm(s, true);
+    const int e0 = tm.mark();
+    const int64_t M = R.M;
+    const SpOperand r = operand(R), a = operand(A), p = operand(P);
+    BinThresholds thr{}, thr2{};
+    std::vector<BinCfg> cfg, cfg2;
+    dsymb_bins(c->max_lds, thr, cfg);
+    dnum_bins(c->max_lds, thr2, cfg2);
+
+    const int nlists = (thr.nb > thr2.nb ? thr.nb : thr2.nb) + 1;
+    Buf ctl(sizeof(Ctl)), ub((size_t)(M + 1) * 8), rown((size_t)(M + 1) * 4),
+        n1((size_t)(M + 1) * 4), lists((size_t)nlists * (M ? M : 1) * 4),
+        retry((size_t)(M ? M : 1) * 4), tiles((size_t)scan_tiles(M) * 8);
+    if (!ctl.p || !ub.p || !rown.p || !n1.p || !lists.p || !retry.p || !tiles.p) {
+        set_error("scratch allocation failed");
+        return -1;
+    }
+    Ctl *d = ctl.as<Ctl>();
+    Ctl h;
+    SP3MM_HIP(hipMemsetAsync(d, 0, sizeof(Ctl), s));
+    launch_fill_i32(n1.as<int32_t>(), M, 0, s);
+    launch_ub_bin(r, a, ub.as<int64_t>(), rown.as<int32_t>(), lists.as<int32_t>(), d->counts, thr,
+                  d->totals, s);
+    const int eu = tm.mark();
+    if (read_ctl(c, d, h)) return -1;
+    const long long mac1 = (long long)h.totals[0];
+    const int slog1 = sub_log(A, 64), slog2 = sub_log(P, 64);
+    const int es0 = tm.mark();
+    for (int i = 0; i < thr.nb; i++) {
+        if (h.counts[i] == 0) continue;
+        const int G = cfg[i].G, H = cfg[i].H;
+        launch_direct(G, false, H, H, slog1 > ilog2(G) ? ilog2(G) : slog1, slog2, r, a, p,
+                      lists.as<int32_t>() + (int64_t)i * M, d->counts + i, h.counts[i], n1.as<int32_t>(),
+                      rown.as<int32_t>(), nullptr, nullptr, nullptr, retry.as<int32_t>(), &d->nretry,
+                      &d->err, max_blocks(c), s);
+    }
+    const int dlog1 = sub_log(A, kDenseThreads), dlog2 = sub_log(P, kDenseThreads);
+    if (h.counts[thr.nb] > 0) {  // long rows: dense accumulators in HBM
+        DenseDirectWs dw;
+        if (dw.alloc(dense_blocks(c, h.counts[thr.nb]), A.N, P.N, false, s)) return -1;
+        launch_dense_direct(false, dense_blocks(c, h.counts[thr.nb]), r, a, p,
+                            lists.as<int32_t>() + (int64_t)thr.nb * M, d->counts + thr.nb, dw.ws, dlog1, dlog2,
+                            n1.as<int32_t>(), rown.as<int32_t>(), nullptr, nullptr, nullptr, &d->err, s);
+    }
+    const int es1 = tm.mark();
+    if (dmat_alloc(C, M, P.N, 0, SP3MM_F64)) return -1;
+    launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d->nnz, s);
+    launch_rebin(n1.as<int32_t>(), rown.as<int32_t>(), M, lists.as<int32_t>(), d->counts2, thr2, s);
+    if (read_ctl(c, d, h)) {
+        dmat_release(C);
+        return -1;
+    }
+    if (h.nretry > 0 && !h.err) {
+        // rows whose RAP table filled: redo them densely, then rescan
+        DenseDirectWs dw;
+        if (dw.alloc(dense_blocks(c, h.nretry), A.N, P.N, false, s)) {
+            dmat_release(C);
+            return -1;
+        }
+        launch_dense_direct(false, dense_blocks(c, h.nretry), r, a, p, retry.as<int32_t>(), &d->nretry, dw.ws,
+                            dlog1, dlog2, n1.as<int32_t>(), rown.as<int32_t>(), nullptr, nullptr, nullptr,
+                            &d->err, s);
+        SP3MM_HIP(hipMemsetAsync(d->counts2, 0, sizeof(d->counts2), s));
+        launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d->nnz, s);
+        launch_rebin(n1.as<int32_t>(), rown.as<int32_t>(), M, lists.as<int32_t>(), d->counts2, thr2, s);
+        if (read_ctl(c, d, h)) {
+            dmat_release(C);
+            return -1;
+        }
+    }
+    const int es2 = tm.mark();
+    if (h.err) {
+        set_error("direct symbolic phase: %s", err_text(h.err));
+        dmat_release(C);
+        return -1;
+    }
+    const int64_t nnz = h.nnz;
+    C.nnz = nnz;
+    if (symbolic_only) return 0;
+    c->pool.release(C.col);
+    c->pool.release(C.val);
+    C.col = static_cast<int32_t *>(c->pool.alloc((size_t)(nnz ? nnz : 1) * 4));
+    C.val = c->pool.alloc((size_t)(nnz ? nnz : 1) * 8);
+    if (!C.col || !C.val) {
+        set_error("output allocation (%lld nnz) failed", (long long)nnz);
+        dmat_release(C);
+        return -1;
+    }
+    if (nnz >= INT32_MAX) {
+        c->pool.release(C.rp32);
+        C.rp32 = nullptr;
+    }
+    const int en0 = tm.mark();
+    int launches = 0;
+    for (int i = 0; i < thr2.nb; i++) {
+        if (h.counts2[i] == 0) continue;
+        const int G = cfg2[i].G, H = cfg2[i].H;
+        launch_direct(G, true, H, H, slog1 > ilog2(G) ? ilog2(G) : slog1, slog2, r, a, p,
+                      lists.as<int32_t>() + (int64_t)i * M, d->counts2 + i, h.counts2[i], nullptr,
+                      nullptr, C.rp, C.col, static_cast<double *>(C.val), nullptr, nullptr, &d->err,
+                      max_blocks(c), s);
+        launches++;
+    }
+    DenseDirectWs dwn;
+    if (h.counts2[thr2.nb] > 0) {
+        const int wb = dense_blocks(c, h.counts2[thr2.nb]);
+        if (dwn.alloc(wb, A.N, P.N, true, s)) {
+            dmat_release(C);
+            return -1;
+        }
+        launch_dense_direct(true, wb, r, a, p, lists.as<int32_t>() + (int64_t)thr2.nb * M, d->counts2 + thr2.nb,
+                            dwn.ws, dlog1, dlog2, nullptr, nullptr, C.rp, C.col, static_cast<double *>(C.val),
+                            &d->err, s);
+        launches++;
+    }
+    const int en1 = tm.mark();
+    if (read_ctl(c, d, h)) {
+        dmat_release(C);
+        return -1;
+    }
+    if (h.err) {
+        set_error("direct numeric phase: %s", err_text(h.err));
+        dmat_release(C);
+        return -1;
+    }
+    const int e1 = tm.mark();
+    (void)hipEventSynchronize(tm.ev[e1]);
+    if (st) {
+        memset(st, 0, sizeof *st);
+        st->ms_total = tm.ms(e0, e1);
+        st->ms_ub = tm.ms(e0, eu);
+        st->ms_symbolic = tm.ms(es0, es1);
+        st->ms_scan = tm.ms(es1, es2);
+        st->ms_numeric = tm.ms(en0, en1);
+        st->ms_main = st->ms_numeric;
+        st->mac1 = mac1;
+        st->nnz_out = nnz;
+        st->main_kernel_launches = launches;
+    }
+    (void)timing;
+    return 0;
+}
+
+// --------------------------------------------------------------------------
+// R*A*P, direct form in one pass: rows computed once into scratch slots, then
+// scanned and gathered into the output
+// --------------------------------------------------------------------------
+struct FusedCfg {
+    int G, H1, H2;
+};
+
+// bins by the upper bound of the R*A row: table 1 of H1 >= ub slots, table 2
+// of H1/4 (>= 64) slots; rows above the largest H1 that fits go to the
+// long-row kernels
+static void fused_bins(size_t max_lds, bool mixed, BinThresholds &thr, std::vector<FusedCfg> &cfg)
+{
+    thr.nb = 0;
+    cfg.clear();
+    for (long long v = 64; thr.nb < kMaxBins - 1; v *= 2) {
+        const int H1 = (int)v;
+        const int H2 = H1 / 4 > 64 ? H1 / 4 : 64;
+        const int G = H1 <= 512 ? 64 : 256;
+        if ((mixed ? mixed_lds(G, H1, H2) : fused_lds(G, H1, H2)) > max_lds) break;
+        thr.v[thr.nb++] = v;
+        cfg.push_back({G, H1, H2});
+    }
+    cfg.push_back({0, 0, 0});
+}
+
+// FusedPlan: the one-pass direct pipeline.  build() runs it eagerly, reading
+// counters back to size bins, scratch slots and the output; record() enqueues
+// the same kernels for the sizes found by build() with no host
+// synchronisation, so it can be captured into a HIP graph and replayed.
+struct FusedPlan {
+    sp3mm_dmat R{}, A{}, P{};
+    bool mixed = false;
+    double guard = 1e-2;
+    double *rowsum = nullptr;
+    BinThresholds thr{};
+    std::vector<FusedCfg> cfg;
+    std::vector<int> cnt, fcnt;
+    std::vector<int64_t> soff, nbase;
+    int nlong = 0, nretry = 0;
+    int64_t nnz = 0, stot = 0;
+    long long mac1 = 0;
+    int slog1 = 0, slog2 = 0, dlog1 = 0, dlog2 = 0;
+    Buf ctl, ub, rown, lists, retry, slotn, fix, tiles, scol, sval;
+    DenseDirectWs dws, rws;
+    sp3mm_dmat C{};
+    double ms_ub = 0, ms_num = 0, ms_scan = 0;
+
+    Ctl *d() const { return ctl.as<Ctl>(); }
+    SpOperand r() const { return operand(R); }
+    SpOperand a() const { return operand(A); }
+    SpOperand p() const { return operand(P); }
+
+    void launch_main(Context *c, hipStream_t s, int i)
+    {
+        const FusedCfg &f = cfg[i];
+        const int64_t M = R.M;
+        if (mixed)
+            launch_direct_mixed(f.G, f.H1, f.H2, slog1, slog2, (float)guard, r(), a(), p(),
+                                lists.as<int32_t>() + (int64_t)i * M, d()->counts + i, cnt[i], rown.as<int32_t>(),
+                                slotn.as<int32_t>() + nbase[i], scol.as<int32_t>() + soff[i],
+                                sval.as<double>() + soff[i], rowsum, retry.as<int32_t>(), &d()->nretry,
+                                fix.as<int32_t>() + nbase[i], d()->counts2 + i, &d()->err, max_blocks(c), s);
+        else
+            launch_direct_fused(f.G, f.H1, f.H2, slog1, slog2, r(), a(), p(), lists.as<int32_t>() + (int64_t)i * M,
+                                d()->counts + i, cnt[i], rown.as<int32_t>(), slotn.as<int32_t>() + nbase[i],
+                                scol.as<int32_t>() + soff[i], sval.as<double>() + soff[i], retry.as<int32_t>(),
+                                &d()->nretry, &d()->err, max_blocks(c), s);
+    }
+
+    void launch_tail(Context *c, hipStream_t s)
+    {
+        const int64_t M = R.M;
+        for (int i = 0; i < thr.nb; i++) {
+            if (cnt[i] == 0) continue;
+            launch_gather_rows(lists.as<int32_t>() + (int64_t)i * M, d()->counts + i, cnt[i],
+                               slotn.as<int32_t>() + nbase[i], cfg[i].H2, scol.as<int32_t>() + soff[i],
+                               sval.as<double>() + soff[i], C.rp, C.col, static_cast<double *>(C.val), nnz,
+                               &d()->err, max_blocks(c), s);
+        }
+        if (nlong > 0)
+            launch_dense_direct(true, dense_blocks(c, nlong), r(), a(), p(), lists.as<int32_t>() + (int64_t)thr.nb * M,
+                                d()->counts + thr.nb, dws.ws, dlog1, dlog2, nullptr, nullptr, C.rp, C.col,
+                                static_cast<double *>(C.val), &d()->err, s, mixed, rowsum);
+        if (nretry > 0)
+            launch_dense_direct(true, dense_blocks(c, nretry), r(), a(), p(), retry.as<int32_t>(), &d()->nretry,
+                                rws.ws, dlog1, dlog2, nullptr, nullptr, C.rp, C.col, static_cast<double *>(C.val),
+                                &d()->err, s, mixed, rowsum);
+    }
+
+    void launch_fixups(Context *c, hipStream_t s, bool all_bins)
+    {
+        if (!mixed) return;
+        for (int i = 0; i < thr.nb; i++) {
+            if (cnt[i] == 0 || (!all_bins && fcnt[i] == 0)) continue;
+            const FusedCfg &f = cfg[i];
+            launch_direct(f.G, true, f.H1, f.H2, slog1 > ilog2(f.G) ? ilog2(f.G) : slog1, slog2, r(), a(), p(),
+                          fix.as<int32_t>() + nbase[i], d()->counts2 + i, all_bins ? cnt[i] : fcnt[i], nullptr,
+                          nullptr, C.rp, C.col, static_cast<double *>(C.val), nullptr, nullptr, &d()->err,
+                          max_blocks(c), s, true);
+        }
+    }
+
+    int build(Context *c, Timer &tm)
+    {
+        hipStream_t s = c->stream;
+        const int64_t M = R.M;
+        fused_bins(c->max_lds, mixed, thr, cfg);
+        ctl = Buf(sizeof(Ctl));
+        ub = Buf((size_t)(M + 1) * 8);
+        rown = Buf((size_t)(M + 1) * 4);
+        lists = Buf((size_t)(thr.nb + 1) * (M ? M : 1) * 4);
+        retry = Buf((size_t)(M ? M : 1) * 4);
+        slotn = Buf((size_t)(M ? M : 1) * 4);
+        fix = Buf((size_t)(M ? M : 1) * 4);
+        tiles = Buf((size_t)scan_tiles(M) * 8);
+        if (!ctl.p || !ub.p || !rown.p || !lists.p || !retry.p || !slotn.p || !fix.p || !tiles.p) {
+            set_error("scratch allocation failed");
+            return -1;
+        }
+        Ctl h;
+        const int e0 = tm.mark();
+        SP3MM_HIP(hipMemsetAsync(d(), 0, sizeof(Ctl), s));
+        launch_ub_bin(r(), a(), ub.as<int64_t>(), rown.as<int32_t>(), lists.as<int32_t>(), d()->counts, thr,
+                      d()->totals, s);
+        const int eu = tm.mark();
+        if (read_ctl(c, d(), h)) return -1;
+        mac1 = (long long)h.totals[0];
+        cnt.assign(h.counts, h.counts + thr.nb + 1);
+        nlong = cnt[thr.nb];
+        soff.assign(thr.nb + 1, 0);
+        nbase.assign(thr.nb + 1, 0);
+        int64_t ntot = 0;
+        stot = 0;
+        for (int i = 0; i < thr.nb; i++) {
+            soff[i] = stot;
+            nbase[i] = ntot;
+            stot += (int64_t)cnt[i] * cfg[i].H2;
+            ntot += cnt[i];
+        }
+        scol = Buf((size_t)(stot ? stot : 1) * 4);
+        sval = Buf((size_t)(stot ? stot : 1) * 8);
+        if (!scol.p || !sval.p) {
+            set_error("scratch allocation (%lld entries) failed", (long long)stot);
+            return -1;
+        }
+        slog1 = sub_log(A, 64);
+        slog2 = sub_log(P, 64);
+        dlog1 = sub_log(A, kDenseThreads);
+        dlog2 = sub_log(P, kDenseThreads);
+        const int en0 = tm.mark();
+        for (int i = 0; i < thr.nb; i++)
+            if (cnt[i]) launch_main(c, s, i);
+        const int en1 = tm.mark();
+        if (nlong > 0) {
+            if (dws.alloc(dense_blocks(c, nlong), A.N, P.N, true, s)) return -1;
+            launch_dense_direct(false, dense_blocks(c, nlong), r(), a(), p(), lists.as<int32_t>() + (int64_t)thr.nb * M,
+                                d()->counts + thr.nb, dws.ws, dlog1, dlog2, nullptr, rown.as<int32_t>(), nullptr,
+                                nullptr, nullptr, &d()->err, s, mixed);
+        }
+        if (dmat_alloc(C, M, P.N, 0, SP3MM_F64)) return -1;
+        launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d()->nnz, s);
+        if (read_ctl(c, d(), h)) return -1;
+        nretry = h.nretry;
+        if (nretry > 0 && !h.err) {
+            // rows whose tables filled: count them densely, then rescan
+            if (rws.alloc(dense_blocks(c, nretry), A.N, P.N, true, s)) return -1;
+            launch_dense_direct(false, dense_blocks(c, nretry), r(), a(), p(), retry.as<int32_t>(), &d()->nretry,
+                                rws.ws, dlog1, dlog2, nullptr, rown.as<int32_t>(), nullptr, nullptr, nullptr,
+                                &d()->err, s, mixed);
+            launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d()->nnz, s);
+            if (read_ctl(c, d(), h)) return -1;
+        }
+        const int es = tm.mark();
+        if (h.err) {
+            set_error("direct product: %s", err_text(h.err));
+            return -1;
+        }
+        fcnt.assign(h.counts2, h.counts2 + thr.nb + 1);
+        nnz = h.nnz;
+        C.nnz = nnz;
+        c->pool.release(C.col);
+        c->pool.release(C.val);
+        C.col = static_cast<int32_t *>(c->pool.alloc((size_t)(nnz ? nnz : 1) * 4));
+        C.val = c->pool.alloc((size_t)(nnz ? nnz : 1) * 8);
+        if (!C.col || !C.val) {
+            set_error("output allocation (%lld nnz) failed", (long long)nnz);
+            return -1;
+        }
+        if (nnz >= INT32_MAX) {
+            c->pool.release(C.rp32);
+            C.rp32 = nullptr;
+        }
+        launch_tail(c, s);
+        launch_fixups(c, s, false);
+        if (read_ctl(c, d(), h)) return -1;
+        if (h.err) {
+            set_error("direct product: %s", err_text(h.err));
+            return -1;
+        }
+        ms_ub = tm.ms(e0, eu);
+        ms_num = tm.ms(en0, en1);
+        ms_scan = tm.ms(en1, es);
+        return 0;
+    }
+
+    // the whole pipeline for the sizes found by build(), no host synchronisation
+    void record(Context *c, hipStream_t s)
+    {
+        const int64_t M = R.M;
+        launch_fill_i32(reinterpret_cast<int32_t *>(d()), (int64_t)(sizeof(Ctl) / 4), 0, s);
+        launch_ub_bin(r(), a(), ub.as<int64_t>(), rown.as<int32_t>(), lists.as<int32_t>(), d()->counts, thr,
+                      d()->totals, s);
+        for (int i = 0; i < thr.nb; i++)
+            if (cnt[i]) launch_main(c, s, i);
+        if (nlong > 0)
+            launch_dense_direct(false, dense_blocks(c, nlong), r(), a(), p(), lists.as<int32_t>() + (int64_t)thr.nb * M,
+                                d()->counts + thr.nb, dws.ws, dlog1, dlog2, nullptr, rown.as<int32_t>(), nullptr,
+                                nullptr, nullptr, &d()->err, s, mixed);
+        if (nretry > 0)
+            launch_dense_direct(false, dense_blocks(c, nretry), r(), a(), p(), retry.as<int32_t>(), &d()->nretry,
+                                rws.ws, dlog1, dlog2, nullptr, rown.as<int32_t>(), nullptr, nullptr, nullptr,
+                                &d()->err, s, mixed);
+        launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d()->nnz, s);
+        launch_tail(c, s);
+        launch_fixups(c, s, true);
+        launch_check_nnz(&d()->nnz, nnz, &d()->nretry, nretry, &d()->err, s);
+    }
+
+    // after a replay: did the device see the sizes this plan was built for?
+    int check(Context *c)
+    {
+        Ctl h;
+        if (read_ctl(c, d(), h)) return -1;
+        if (h.err) {
+            set_error("replayed product: %s", err_text(h.err));
+            return -1;
+        }
+        return 0;
+    }
+
+    void release_all()
+    {
+        dmat_release(C);
+    }
+};
+
+static int rap_direct_fused(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, sp3mm_dmat &C,
+                            sp3mm_stats *st, bool mixed, double guard, double *rowsum)
+{
+    Context *c = ctx();
+    Timer tm(c->stream, true);
+    const int e0 = tm.mark();
+    FusedPlan plan;
+    plan.R = R;
+    plan.A = A;
+    plan.P = P;
+    plan.mixed = mixed;
+    plan.guard = guard;
+    plan.rowsum = rowsum;
+    if (plan.build(c, tm)) {
+        plan.release_all();
+        return -1;
+    }
+    const int e1 = tm.mark();
+    (void)hipEventSynchronize(tm.ev[e1]);
+    C = plan.C;
+    memset(&plan.C, 0, sizeof plan.C);
+    if (st) {
+        memset(st, 0, sizeof *st);
+        st->ms_total = tm.ms(e0, e1);
+        st->ms_ub = plan.ms_ub;
+        st->ms_numeric = plan.ms_num;
+        st->ms_scan = plan.ms_scan;
+        st->ms_main = plan.ms_num;
+        st->mac1 = plan.mac1;
+        st->nnz_out = plan.nnz;
+        long long nf = 0;
+        int launches = 0;
+        for (int i = 0; i < plan.thr.nb; i++) {
+            nf += plan.fcnt[i];
+            launches += plan.cnt[i] ? 1 : 0;
+        }
+        st->main_kernel_launches = launches;
+        st->guarded_rows = nf;
+        st->guarded_entries = plan.nretry + plan.nlong;
+    }
+    return 0;
+}
+
+int rap_direct(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, sp3mm_dmat &C, sp3mm_stats *st,
+               bool timing, bool symbolic_only)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (check_pair(R, A) || check_pair(A, P)) return -1;
+    static const bool twopass = getenv("SP3MM_DIRECT_TWOPASS") != nullptr;
+    if (symbolic_only || twopass) return rap_direct_twopass(R, A, P, C, st, timing, symbolic_only);
+    return rap_direct_fused(R, A, P, C, st, false, 0.0, nullptr);
+}
+
+int rap_direct_mixed(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, sp3mm_dmat &C, sp3mm_stats *st,
+                     double guard, double *rowsum)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (R.N != A.M || A.N != P.M) {
+        set_error("inner dimensions differ");
+        return -1;
+    }
+    if (!A.rp32 || !P.rp32) {
+        set_error("right operand has >= 2^31 nonzeros");
+        return -1;
+    }
+    if (R.vtype != SP3MM_F32 || A.vtype != SP3MM_F32 || P.vtype != SP3MM_F32) {
+        set_error("the mixed-precision product needs FP32 operands (sp3mm_dmat_convert)");
+        return -1;
+    }
+    return rap_direct_fused(R, A, P, C, st, true, guard > 0 ? guard : 1e-2, rowsum);
+}
+
+// --------------------------------------------------------------------------
+// batches: many independent products as one block-diagonal product
+// --------------------------------------------------------------------------
+struct Batch {
+    int count = 0;
+    bool mixed = false;
+    std::vector<int64_t> roff, coff, M;
+    sp3mm_dmat Rb{}, Ab{}, Pb{};
+    FusedPlan plan;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double ms_pack = 0, ms_build = 0;
+};
+
+static int pack(Context *c, int count, const sp3mm_dmat *const *ms, int vtype, sp3mm_dmat &out,
+                const std::vector<int64_t> &coloff)
+{
+    int64_t M = 0, N = 0, nnz = 0;
+    std::vector<PackItem> items((size_t)count);
+    for (int i = 0; i < count; i++) {
+        const sp3mm_dmat &m = *ms[i];
+        items[i] = PackItem{m.rp, m.col, m.val, m.M, m.nnz, M, nnz, (int32_t)coloff[i], 0};
+        M += m.M;
+        N += m.N;
+        nnz += m.nnz;
+    }
+    if (N >= INT32_MAX || M >= INT32_MAX || nnz >= INT32_MAX) {
+        set_error("batch: packed operand too large (%lld x %lld, %lld nnz)", (long long)M, (long long)N,
+                  (long long)nnz);
+        return -1;
+    }
+    if (dmat_alloc(out, M, N, nnz, vtype)) return -1;
+    Buf it(items.size() * sizeof(PackItem));
+    if (!it.p) return -1;
+    SP3MM_HIP(hipMemcpyAsync(it.p, items.data(), items.size() * sizeof(PackItem), hipMemcpyHostToDevice, c->stream));
+    launch_pack(it.as<PackItem>(), count, vtype, out.rp, out.col, out.val, c->stream);
+    // last row offset, then the int32 copy of the offsets (right operands)
+    SP3MM_HIP(hipMemcpyAsync(out.rp + M, &nnz, 8, hipMemcpyHostToDevice, c->stream));
+    SP3MM_HIP(hipStreamSynchronize(c->stream));
+    launch_rp_convert(out.rp, M, nnz, 0, out.rp, out.rp32, nullptr, c->stream);
+    SP3MM_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+void *batch_create(int count, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P, bool mixed,
+                   double guard, bool graph)
+{
+    Context *c = ctx();
+    if (!c) return nullptr;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (count <= 0) {
+        set_error("batch: empty batch");
+        return nullptr;
+    }
+    const int vt = mixed ? SP3MM_F32 : SP3MM_F64;
+    for (int i = 0; i < count; i++) {
+        if (R[i].N != A[i].M || A[i].N != P[i].M) {
+            set_error("batch: problem %d has mismatched shapes", i);
+            return nullptr;
+        }
+        if (R[i].vtype != vt || A[i].vtype != vt || P[i].vtype != vt) {
+            set_error("batch: problem %d operands must be %s", i, mixed ? "FP32" : "FP64");
+            return nullptr;
+        }
+    }
+    Batch *b = new Batch();
+    b->count = count;
+    b->mixed = mixed;
+    std::vector<const sp3mm_dmat *> rs, as, ps;
+    std::vector<int64_t> offR, offA, offP;
+    int64_t kr = 0, ka = 0, kp = 0, mr = 0;
+    for (int i = 0; i < count; i++) {
+        rs.push_back(R + i);
+        as.push_back(A + i);
+        ps.push_back(P + i);
+        offR.push_back(kr);  // R's columns are A's rows
+        offA.push_back(ka);  // A's columns are P's rows
+        offP.push_back(kp);  // P's columns are the product's columns
+        b->roff.push_back(mr);
+        b->coff.push_back(kp);
+        b->M.push_back(R[i].M);
+        kr += R[i].N;
+        ka += A[i].N;
+        kp += P[i].N;
+        mr += R[i].M;
+    }
+    Timer tm(c->stream, true);
+    const int e0 = tm.mark();
+    if (pack(c, count, rs.data(), vt, b->Rb, offR) || pack(c, count, as.data(), vt, b->Ab, offA) ||
+        pack(c, count, ps.data(), vt, b->Pb, offP)) {
+        dmat_release(b->Rb);
+        dmat_release(b->Ab);
+        dmat_release(b->Pb);
+        delete b;
+        return nullptr;
+    }
+    const int e1 = tm.mark();
+    b->plan.R = b->Rb;
+    b->plan.A = b->Ab;
+    b->plan.P = b->Pb;
+    b->plan.mixed = mixed;
+    b->plan.guard = guard > 0 ? guard : 1e-2;
+    if (b->plan.build(c, tm)) {
+        b->plan.release_all();
+        dmat_release(b->Rb);
+        dmat_release(b->Ab);
+        dmat_release(b->Pb);
+        delete b;
+        return nullptr;
+    }
+    const int e2 = tm.mark();
+    (void)hipEventSynchronize(tm.ev[e2]);
+    b->ms_pack = tm.ms(e0, e1);
+    b->ms_build = tm.ms(e1, e2);
+    if (graph) {
+        hipStream_t s = c->stream;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            set_error("batch: stream capture failed to start");
+            return b;  // usable without a graph
+        }
+        b->plan.record(c, s);
+        if (hipStreamEndCapture(s, &b->graph) != hipSuccess ||
+            hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            if (b->graph) (void)hipGraphDestroy(b->graph);
+            b->graph = nullptr;
+            b->exec = nullptr;
+        }
+    }
+    return b;
+}
+
+int batch_run(void *h, bool use_graph, double *ms)
+{
+    Context *c = ctx();
+    if (!c || !h) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    Batch *b = static_cast<Batch *>(h);
+    if (use_graph && !b->exec) {
+        set_error("batch: no graph was instantiated");
+        return -1;
+    }
+    hipEvent_t e0, e1;
+    SP3MM_HIP(hipEventCreate(&e0));
+    SP3MM_HIP(hipEventCreate(&e1));
+    SP3MM_HIP(hipEventRecord(e0, c->stream));
+    if (use_graph) SP3MM_HIP(hipGraphLaunch(b->exec, c->stream));
+    else b->plan.record(c, c->stream);
+    SP3MM_HIP(hipEventRecord(e1, c->stream));
+    const int rc = b->plan.check(c);
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms) *ms = t;
+    return rc;
+}
+
+int batch_result(void *h, int i, sp3mm_dmat &out)
+{
+    Context *c = ctx();
+    if (!c || !h) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    Batch *b = static_cast<Batch *>(h);
+    if (i < 0 || i >= b->count) {
+        set_error("batch: index %d out of range", i);
+        return -1;
+    }
+    const sp3mm_dmat &C = b->plan.C;
+    int64_t lo = 0, hi = 0;
+    SP3MM_HIP(hipMemcpy(&lo, C.rp + b->roff[i], 8, hipMemcpyDeviceToHost));
+    SP3MM_HIP(hipMemcpy(&hi, C.rp + b->roff[i] + b->M[i], 8, hipMemcpyDeviceToHost));
+    const int64_t N = (i + 1 < b->count ? b->coff[i + 1] : b->plan.P.N) - b->coff[i];
+    if (dmat_alloc(out, b->M[i], N, hi - lo, SP3MM_F64)) return -1;
+    launch_extract(C.rp, C.col, static_cast<const double *>(C.val), b->roff[i], b->M[i], (int32_t)b->coff[i], out.rp,
+                   out.rp32, out.col, static_cast<double *>(out.val), c->stream);
+    SP3MM_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+void batch_info(void *h, double *ms_pack, double *ms_build, int64_t *nnz, int32_t *has_graph)
+{
+    Batch *b = static_cast<Batch *>(h);
+    if (!b) return;
+    if (ms_pack) *ms_pack = b->ms_pack;
+    if (ms_build) *ms_build = b->ms_build;
+    if (nnz) *nnz = b->plan.nnz;
+    if (has_graph) *has_graph = b->exec ? 1 : 0;
+}
+
+void batch_free(void *h)
+{
+    Context *c = ctx();
+    Batch *b = static_cast<Batch *>(h);
+    if (!b || !c) return;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    (void)hipStreamSynchronize(c->stream);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->plan.release_all();
+    dmat_release(b->Rb);
+    dmat_release(b->Ab);
+    dmat_release(b->Pb);
+    delete b;
+}
+
+// --------------------------------------------------------------------------
+// sizing helpers exposed through the ABI
+// --------------------------------------------------------------------------
+int row_ub(const sp3mm_dmat &A, const sp3mm_dmat &B, int64_t *host_ub)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (A.N != B.M || !B.rp32) {
+        set_error("upper bound: shapes differ or right operand too large");
+        return -1;
+    }
+    const int64_t M = A.M;
+    BinThresholds thr{};
+    thr.nb = 0;
+    Buf ctl(sizeof(Ctl)), ub((size_t)(M + 1) * 8), lists((size_t)(M ? M : 1) * 4);
+    if (!ctl.p || !ub.p || !lists.p) {
+        set_error("scratch allocation failed");
+        return -1;
+    }
+    Ctl *d = ctl.as<Ctl>();
+    Ctl h;
+    SP3MM_HIP(hipMemsetAsync(d, 0, sizeof(Ctl), c->stream));
+    launch_ub_bin(operand(A), operand(B), ub.as<int64_t>(), nullptr, lists.as<int32_t>(), d->counts, thr,
+                  d->totals, c->stream);
+    if (M) SP3MM_HIP(hipMemcpyAsync(host_ub, ub.p, (size_t)M * 8, hipMemcpyDeviceToHost, c->stream));
+    if (read_ctl(c, d, h)) return -1;
+    host_ub[M] = (int64_t)h.totals[0];
+    return 0;
+}
+
+int ub_colparts(const sp3mm_dmat &A, const int64_t *host_offsets, int64_t boff_len, int grid_cols,
+                int64_t *host_out)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    const int64_t M = A.M;
+    Buf ctl(sizeof(Ctl)), boff((size_t)boff_len * 8), out((size_t)(M * grid_cols + 1) * 8);
+    if (!ctl.p || !boff.p || !out.p) {
+        set_error("scratch allocation failed");
+        return -1;
+    }
+    Ctl *d = ctl.as<Ctl>();
+    Ctl h;
+    SP3MM_HIP(hipMemsetAsync(d, 0, sizeof(Ctl), c->stream));
+    SP3MM_HIP(hipMemcpyAsync(boff.p, host_offsets, (size_t)boff_len * 8, hipMemcpyHostToDevice, c->stream));
+    launch_ub_colparts(A.rp, A.col, M, boff.as<int64_t>(), grid_cols, out.as<int64_t>(), d->totals, c->stream);
+    if (M)
+        SP3MM_HIP(hipMemcpyAsync(host_out, out.p, (size_t)M * grid_cols * 8, hipMemcpyDeviceToHost,
+                                 c->stream));
+    if (read_ctl(c, d, h)) return -1;
+    host_out[M * grid_cols] = (int64_t)h.totals[0];
+    return 0;
+}
+
+}  // namespace sp3mm

@@ -1,0 +1,1192 @@
+// kernels.hip -- gfx950 kernels of the Galerkin triple product R*A*P on sorted CSR.
+//
+// Phases (one launch sequence per product):
+//   k_ub_bin      per-row upper bound ub[r] = sum_{k in A_r} nnz(B_k)
+//                 (code/src/include/SpMMUtilsGeneric.h:112-139), rows scattered
+//                 into per-bin lists by a size measure; empty rows finished here
+//   k_symb<G>     exact row lengths with an LDS key hash (replaces the bitmap /
+//                 RB-tree symbolic, code/src/Sp3MM_CSR_OMP_SymbStep_Generic.c:84-231)
+//   k_scan_*      exclusive scan of row lengths into row offsets (replaces the
+//                 serial IRP prefix sum, code/src/Sp3MM_CSR_OMP_Num_Generic.c:40-47)
+//   k_num<G>      Gustavson numeric: products accumulated in an LDS hash
+//                 (ds_add_f64), compacted and rank-sorted into the output row
+//                 (replaces the dense accumulator + qsort,
+//                 code/src/include/SpMMUtilsGeneric.h:39-103,230-241)
+//   k_direct<G,NUM>  row r of R*A built in one LDS hash and forwarded into P
+//                 through a second LDS hash without leaving the CU (replaces
+//                 code/src/Sp3MM_CSR_OMP_UB_Generic.c:533-550)
+//
+// G is the number of threads cooperating on one output row: 64 (a wavefront;
+// a 256-thread workgroup then works on 4 rows at once, each wavefront on its
+// own LDS tables) or 256 (the whole workgroup on one row).  Table sizes H are
+// runtime powers of two chosen by the host from the bin of the row.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace sp3mm {
+
+// ---------------------------------------------------------------------------
+// upper bounds + binning
+// ---------------------------------------------------------------------------
+// Binning.  A workgroup of 256 threads owns kBinRows consecutive rows (8 per
+// thread, strided so that neighbouring threads take neighbouring rows).  Bin
+// populations are first counted with LDS atomics, then each workgroup books
+// space in every bin it uses with ONE global atomic per bin, so no global
+// counter sees more than (rows / kBinRows) atomics.  Rows of one workgroup
+// stay together inside a bin (their relative order is not kept).
+constexpr int kBinThreads = 256;
+constexpr int kBinPer = 8;
+constexpr int kBinRows = kBinThreads * kBinPer;
+
+__device__ __forceinline__ int bin_of(long long u, const BinThresholds &thr)
+{
+    if (u <= 0) return -1;
+    for (int i = 0; i < thr.nb; i++)
+        if (u <= thr.v[i]) return i;
+    return thr.nb;
+}
+
+// scatter the block's binned rows; `b` and `pos` hold each thread's kBinPer entries
+__device__ __forceinline__ void bin_scatter(const int *b, const int *pos, int64_t r0, int64_t M,
+                                            int32_t *lists, int *counts, int nbins, int *hist, int *base)
+{
+    __syncthreads();
+    if ((int)threadIdx.x < nbins) base[threadIdx.x] = hist[threadIdx.x] ? atomicAdd(counts + threadIdx.x, hist[threadIdx.x]) : 0;
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kBinPer; j++) {
+        const int64_t r = r0 + (int64_t)j * kBinThreads + threadIdx.x;
+        if (b[j] >= 0) lists[(int64_t)b[j] * M + base[b[j]] + pos[j]] = (int32_t)r;
+    }
+}
+
+// ub[r] = sum_{k in A_r} nnz(B_k) (code/src/include/SpMMUtilsGeneric.h:112-139);
+// rows binned by ub (rows with ub == 0 get rownnz = 0 and no bin).
+__global__ void __launch_bounds__(kBinThreads) k_ub_bin(const int64_t *__restrict__ arp, const int32_t *__restrict__ acol,
+                         const int32_t *__restrict__ brp, int64_t M, int64_t *__restrict__ ub,
+                         int32_t *__restrict__ rownnz, int32_t *__restrict__ lists,
+                         int *__restrict__ counts, BinThresholds thr,
+                         unsigned long long *__restrict__ totals)
+{
+    __shared__ int hist[kMaxBins + 1], base[kMaxBins + 1];
+    __shared__ unsigned long long red[2][kBinThreads / 64];
+    if ((int)threadIdx.x <= thr.nb) hist[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t r0 = (int64_t)blockIdx.x * kBinRows;
+    int b[kBinPer], pos[kBinPer];
+    unsigned long long s = 0, mx = 0;
+#pragma unroll
+    for (int j = 0; j < kBinPer; j++) {
+        const int64_t r = r0 + (int64_t)j * kBinThreads + threadIdx.x;
+        long long u = 0;
+        b[j] = -1;
+        if (r < M) {
+            for (int64_t e = arp[r]; e < arp[r + 1]; e++) {
+                const int k = acol[e];
+                u += brp[k + 1] - brp[k];
+            }
+            ub[r] = u;
+            if (u == 0 && rownnz) rownnz[r] = 0;
+            b[j] = bin_of(u, thr);
+            if (b[j] >= 0) pos[j] = atomicAdd(hist + b[j], 1);
+            s += (unsigned long long)u;
+            mx = (unsigned long long)u > mx ? (unsigned long long)u : mx;
+        }
+    }
+    bin_scatter(b, pos, r0, M, lists, counts, thr.nb + 1, hist, base);
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o);
+        const unsigned long long t = __shfl_xor(mx, o);
+        mx = t > mx ? t : mx;
+    }
+    if (lane() == 0) {
+        red[0][threadIdx.x / 64] = s;
+        red[1][threadIdx.x / 64] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long ts = 0, tm = 0;
+        for (int i = 0; i < kBinThreads / 64; i++) {
+            ts += red[0][i];
+            tm = red[1][i] > tm ? red[1][i] : tm;
+        }
+        if (ts) atomicAdd(totals, ts);
+        atomicMax(totals + 1, tm);
+    }
+}
+
+// Re-bin rows by an exact measure (row length, or max of two lengths); the
+// largest meas2 of every bin is recorded in maxes[] (when given).
+__global__ void __launch_bounds__(kBinThreads) k_rebin(const int32_t *__restrict__ meas1, const int32_t *__restrict__ meas2,
+                        int64_t M, int32_t *__restrict__ lists, int *__restrict__ counts,
+                        BinThresholds thr)
+{
+    __shared__ int hist[kMaxBins + 1], base[kMaxBins + 1];
+    if ((int)threadIdx.x <= thr.nb) hist[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t r0 = (int64_t)blockIdx.x * kBinRows;
+    int b[kBinPer], pos[kBinPer];
+#pragma unroll
+    for (int j = 0; j < kBinPer; j++) {
+        const int64_t r = r0 + (int64_t)j * kBinThreads + threadIdx.x;
+        b[j] = -1;
+        if (r < M) {
+            int u = meas1[r];
+            if (meas2) u = max(u, meas2[r]);
+            b[j] = bin_of(u, thr);
+            if (b[j] >= 0) pos[j] = atomicAdd(hist + b[j], 1);
+        }
+    }
+    bin_scatter(b, pos, r0, M, lists, counts, thr.nb + 1, hist, base);
+}
+
+// Upper bound split over column ranges of B: boff[k*G+g] .. boff[k*G+g+1] are
+// the entries of B's row k in range g (code/src/include/SpMMUtilsGeneric.h:146-174).
+__global__ void k_ub_colparts(const int64_t *__restrict__ arp, const int32_t *__restrict__ acol, int64_t M,
+                              const int64_t *__restrict__ boff, int G, int64_t *__restrict__ out,
+                              unsigned long long *__restrict__ total)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long s = 0;
+    if (r < M) {
+        for (int g = 0; g < G; g++) out[r * G + g] = 0;
+        for (int64_t e = arp[r]; e < arp[r + 1]; e++) {
+            const int64_t k = acol[e];
+            for (int g = 0; g < G; g++) {
+                const int64_t len = boff[k * G + g + 1] - boff[k * G + g];
+                out[r * G + g] += len;
+                s += (unsigned long long)len;
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane() == 0 && s) atomicAdd(total, s);
+}
+
+// ---------------------------------------------------------------------------
+// product expansion helpers
+// ---------------------------------------------------------------------------
+// Expand one left row (entries [e0,e1) of lcol/lval) against right operand B:
+// every product (b_col, lval*b_val) goes to the LDS table.  Lanes are split
+// into sub-groups of 2^slog lanes, one sub-group per left entry.  Each lane
+// first loads its left entry and the bounds of the right row, then walks the
+// right row in chunks of kChunk entries that are all loaded before any of
+// them is inserted, so a row costs three dependent global round trips (left
+// entries, right row bounds, right entries) however many entries it has.
+constexpr int kChunk = 8;
+
+template <int G, bool NUM, typename LV, typename BV = double>
+__device__ __forceinline__ void expand_into(const int32_t *lcol, const LV *lval,
+                                            int64_t e0, int64_t e1, const int32_t *__restrict__ brp,
+                                            const int32_t *__restrict__ bcol,
+                                            const BV *__restrict__ bval, int slog, int *keys,
+                                            double *vals, unsigned mask, int shift, int &fresh_cnt,
+                                            bool &full)
+{
+    const int t = Group<G>::tid();
+    const int sub = t & ((1 << slog) - 1);
+    const int step_e = G >> slog;
+    const int S = 1 << slog;
+    for (int64_t e = e0 + (t >> slog); e < e1; e += step_e) {
+        const int k = lcol[e];
+        double lv = 0.0;
+        if constexpr (NUM) lv = (double)lval[e];
+        const int b1 = brp[k + 1];
+        for (int q0 = brp[k] + sub; q0 < b1; q0 += S * kChunk) {
+            int c[kChunk];
+            double w[kChunk];
+#pragma unroll
+            for (int u = 0; u < kChunk; u++) {
+                const int q = q0 + u * S;
+                c[u] = q < b1 ? bcol[q] : kEmpty;
+                if constexpr (NUM) w[u] = q < b1 ? (double)bval[q] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kChunk; u++) {
+                if (c[u] == kEmpty) continue;
+                int fresh;
+                const int s = hash_insert(keys, mask, shift, c[u], &fresh);
+                if (s < 0) { full = true; continue; }
+                fresh_cnt += fresh;
+                if constexpr (NUM) atomicAdd(vals + s, lv * w[u]);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pairwise SpMM C = A*B: symbolic
+// ---------------------------------------------------------------------------
+template <int G>
+__global__ void __launch_bounds__(256) k_symb(SpOperand A, SpOperand B, const int32_t *__restrict__ rows,
+                                               const int *__restrict__ count, int H, int slog,
+                                               int32_t *__restrict__ rownnz,
+                                               int32_t *__restrict__ retry, int *__restrict__ nretry,
+                                               int *__restrict__ err)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int GPB = 256 / G;  // groups per block
+    const int g = threadIdx.x / G;
+    const int t = Group<G>::tid();
+    int *keys = reinterpret_cast<int *>(smem) + g * H;
+    long long *red = reinterpret_cast<long long *>(smem + (size_t)GPB * H * 4) + g * Group<G>::kWaves;
+    const unsigned mask = (unsigned)H - 1;
+    const int shift = 32 - ilog2((unsigned)H);
+    const int n = *count;
+    for (int i = blockIdx.x * GPB + g; i < n; i += gridDim.x * GPB) {
+        const int r = rows[i];
+        for (int s = t; s < H; s += G) keys[s] = kEmpty;
+        Group<G>::sync();
+        int cnt = 0;
+        bool full = false;
+        expand_into<G, false, double, double>(A.col, (const double *)nullptr, A.rp[r], A.rp[r + 1], B.rp32,
+                                              B.col, (const double *)nullptr, slog, keys, nullptr, mask, shift,
+                                              cnt, full);
+        const long long tot = group_sum<G>(cnt, red);
+        const long long anyfull = group_max<G>(full ? 1 : 0, red);
+        if (t == 0) {
+            if (anyfull) {
+                rownnz[r] = -1;
+                if (retry) retry[atomicAdd(nretry, 1)] = r;
+                else atomicOr(err, kErrHashFull);
+            } else {
+                rownnz[r] = (int32_t)tot;
+            }
+        }
+        Group<G>::sync();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pairwise SpMM C = A*B: numeric
+// ---------------------------------------------------------------------------
+template <int G>
+__global__ void __launch_bounds__(256) k_num(SpOperand A, SpOperand B, const int32_t *__restrict__ rows,
+                                              const int *__restrict__ count, int H, int slog,
+                                              const int64_t *__restrict__ crp,
+                                              int32_t *__restrict__ ccol, double *__restrict__ cval,
+                                              int *__restrict__ err)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int GPB = 256 / G;
+    const int g = threadIdx.x / G;
+    const int t = Group<G>::tid();
+    double *vals = reinterpret_cast<double *>(smem) + g * H;
+    int *keys = reinterpret_cast<int *>(smem + (size_t)GPB * H * 8) + g * H;
+    int *ired = reinterpret_cast<int *>(smem + (size_t)GPB * H * 12) + g * (Group<G>::kWaves + 1);
+    const unsigned mask = (unsigned)H - 1;
+    const int shift = 32 - ilog2((unsigned)H);
+    const int n = *count;
+    const double *aval = static_cast<const double *>(A.val);
+    const double *bval = static_cast<const double *>(B.val);
+    for (int i = blockIdx.x * GPB + g; i < n; i += gridDim.x * GPB) {
+        const int r = rows[i];
+        for (int s = t; s < H; s += G) {
+            keys[s] = kEmpty;
+            vals[s] = 0.0;
+        }
+        Group<G>::sync();
+        int cnt = 0;
+        bool full = false;
+        expand_into<G, true, double>(A.col, aval, A.rp[r], A.rp[r + 1], B.rp32, B.col, bval, slog,
+                                     keys, vals, mask, shift, cnt, full);
+        if (full) atomicOr(err, kErrHashFull);
+        Group<G>::sync();
+        const int nn = compact_table<G, double>(keys, vals, H, ired);
+        const int64_t off = crp[r];
+        const int64_t len = crp[r + 1] - off;
+        if (nn != len) {
+            if (t == 0) atomicOr(err, kErrOverflow);
+        } else {
+            for (int j = t; j < nn; j += G) {
+                const int k = keys[j];
+                const int rk = key_rank(keys, nn, k);
+                ccol[off + rk] = k;
+                cval[off + rk] = vals[j];
+            }
+        }
+        Group<G>::sync();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// direct triple product: row r of R*A in table 1, forwarded into P in table 2
+// ---------------------------------------------------------------------------
+template <int G, bool NUM, typename VT = double>
+__global__ void __launch_bounds__(256) k_direct(SpOperand R, SpOperand A, SpOperand P,
+                                                 const int32_t *__restrict__ rows,
+                                                 const int *__restrict__ count, int H1, int H2,
+                                                 int slog1, int slog2,
+                                                 int32_t *__restrict__ n1out,
+                                                 int32_t *__restrict__ rownnz,
+                                                 const int64_t *__restrict__ crp,
+                                                 int32_t *__restrict__ ccol, double *__restrict__ cval,
+                                                 int32_t *__restrict__ retry, int *__restrict__ nretry,
+                                                 int *__restrict__ err)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int GPB = 256 / G;
+    const int g = threadIdx.x / G;
+    const int t = Group<G>::tid();
+    unsigned char *p = smem;
+    double *v1 = nullptr, *v2 = nullptr;
+    if constexpr (NUM) {
+        v1 = reinterpret_cast<double *>(p) + g * H1;
+        p += (size_t)GPB * H1 * 8;
+        v2 = reinterpret_cast<double *>(p) + g * H2;
+        p += (size_t)GPB * H2 * 8;
+    }
+    int *k1 = reinterpret_cast<int *>(p) + g * H1;
+    p += (size_t)GPB * H1 * 4;
+    int *k2 = reinterpret_cast<int *>(p) + g * H2;
+    p += (size_t)GPB * H2 * 4;
+    long long *red = reinterpret_cast<long long *>(p) + g * Group<G>::kWaves;
+    p += (size_t)GPB * Group<G>::kWaves * 8;
+    int *ired = reinterpret_cast<int *>(p) + g * (Group<G>::kWaves + 1);
+
+    const unsigned m1 = (unsigned)H1 - 1, m2 = (unsigned)H2 - 1;
+    const int sh1 = 32 - ilog2((unsigned)H1), sh2 = 32 - ilog2((unsigned)H2);
+    const int n = *count;
+    const VT *rval = static_cast<const VT *>(R.val);
+    const VT *aval = static_cast<const VT *>(A.val);
+    const VT *pval = static_cast<const VT *>(P.val);
+    for (int i = blockIdx.x * GPB + g; i < n; i += gridDim.x * GPB) {
+        const int r = rows[i];
+        for (int s = t; s < H1; s += G) {
+            k1[s] = kEmpty;
+            if constexpr (NUM) v1[s] = 0.0;
+        }
+        for (int s = t; s < H2; s += G) {
+            k2[s] = kEmpty;
+            if constexpr (NUM) v2[s] = 0.0;
+        }
+        Group<G>::sync();
+        int c1 = 0;
+        bool full = false;
+        expand_into<G, NUM, VT, VT>(R.col, rval, R.rp[r], R.rp[r + 1], A.rp32, A.col, aval, slog1,
+                                    k1, v1, m1, sh1, c1, full);
+        Group<G>::sync();
+        const int nn1 = compact_table<G, double>(k1, v1, H1, ired);
+        int c2 = 0;
+        expand_into<G, NUM, double, VT>(k1, v1, 0, nn1, P.rp32, P.col, pval, slog2, k2, v2, m2, sh2, c2,
+                                    full);
+        if constexpr (!NUM) {
+            const long long tot = group_sum<G>(c2, red);
+            const long long anyfull = group_max<G>(full ? 1 : 0, red);
+            if (t == 0) {
+                if (n1out) n1out[r] = nn1;
+                if (anyfull) {
+                    rownnz[r] = -1;
+                    if (retry) retry[atomicAdd(nretry, 1)] = r;
+                    else atomicOr(err, kErrHashFull);
+                } else {
+                    rownnz[r] = (int32_t)tot;
+                }
+            }
+        } else {
+            if (full) atomicOr(err, kErrHashFull);
+            Group<G>::sync();
+            const int nn2 = compact_table<G, double>(k2, v2, H2, ired);
+            const int64_t off = crp[r];
+            const int64_t len = crp[r + 1] - off;
+            if (nn2 != len) {
+                if (t == 0) atomicOr(err, kErrOverflow);
+            } else {
+                for (int j = t; j < nn2; j += G) {
+                    const int k = k2[j];
+                    const int rk = key_rank(k2, nn2, k);
+                    ccol[off + rk] = k;
+                    cval[off + rk] = v2[j];
+                }
+            }
+        }
+        Group<G>::sync();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// direct triple product in ONE pass over the operands
+// ---------------------------------------------------------------------------
+// Row r of R*A is accumulated in table 1 (H1 >= the row's upper bound, so it
+// cannot fill), compacted, and forwarded into P through table 2 of H2 slots.
+// The finished row is rank-sorted and written to its slot of `cap` = H2
+// entries in a scratch area (slot i of the bin list -> scratch + i*cap); its
+// length goes to rownnz[r] and slotn[i].  A row whose table 2 fills is marked
+// slotn[i] = -1 and queued on `retry` for the long-row kernels.  After the
+// scan of rownnz, k_gather_rows moves every slot to its final place.
+template <int G>
+__global__ void __launch_bounds__(256) k_direct_fused(SpOperand R, SpOperand A, SpOperand P,
+                                                       const int32_t *__restrict__ rows,
+                                                       const int *__restrict__ count, int H1, int H2,
+                                                       int slog1, int slog2, int32_t *__restrict__ rownnz,
+                                                       int32_t *__restrict__ slotn,
+                                                       int32_t *__restrict__ scol, double *__restrict__ sval,
+                                                       int32_t *__restrict__ retry, int *__restrict__ nretry,
+                                                       int *__restrict__ err)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int GPB = 256 / G;
+    const int g = threadIdx.x / G;
+    const int t = Group<G>::tid();
+    unsigned char *p = smem;
+    double *v1 = reinterpret_cast<double *>(p) + g * H1;
+    p += (size_t)GPB * H1 * 8;
+    double *v2 = reinterpret_cast<double *>(p) + g * H2;
+    p += (size_t)GPB * H2 * 8;
+    int *k1 = reinterpret_cast<int *>(p) + g * H1;
+    p += (size_t)GPB * H1 * 4;
+    int *k2 = reinterpret_cast<int *>(p) + g * H2;
+    p += (size_t)GPB * H2 * 4;
+    long long *red = reinterpret_cast<long long *>(p) + g * Group<G>::kWaves;
+    p += (size_t)GPB * Group<G>::kWaves * 8;
+    int *ired = reinterpret_cast<int *>(p) + g * (Group<G>::kWaves + 1);
+
+    const unsigned m1 = (unsigned)H1 - 1, m2 = (unsigned)H2 - 1;
+    const int sh1 = 32 - ilog2((unsigned)H1), sh2 = 32 - ilog2((unsigned)H2);
+    const int n = *count;
+    const double *rval = static_cast<const double *>(R.val);
+    const double *aval = static_cast<const double *>(A.val);
+    const double *pval = static_cast<const double *>(P.val);
+    for (int i = blockIdx.x * GPB + g; i < n; i += gridDim.x * GPB) {
+        const int r = rows[i];
+        for (int s = t; s < H1; s += G) {
+            k1[s] = kEmpty;
+            v1[s] = 0.0;
+        }
+        for (int s = t; s < H2; s += G) {
+            k2[s] = kEmpty;
+            v2[s] = 0.0;
+        }
+        Group<G>::sync();
+        int c = 0;
+        bool full1 = false, full2 = false;
+        expand_into<G, true, double>(R.col, rval, R.rp[r], R.rp[r + 1], A.rp32, A.col, aval, slog1, k1, v1, m1,
+                                     sh1, c, full1);
+        Group<G>::sync();
+        const int nn1 = compact_table<G, double>(k1, v1, H1, ired);
+        if (!full1)
+            expand_into<G, true, double>(k1, v1, 0, nn1, P.rp32, P.col, pval, slog2, k2, v2, m2, sh2, c, full2);
+        const long long anyfull = group_max<G>((full1 || full2) ? 1 : 0, red);
+        if (anyfull) {
+            if (t == 0) {
+                slotn[i] = -1;
+                rownnz[r] = 0;
+                retry[atomicAdd(nretry, 1)] = r;
+            }
+        } else {
+            Group<G>::sync();
+            const int nn2 = compact_table<G, double>(k2, v2, H2, ired);
+            const int64_t off = (int64_t)i * H2;
+            for (int j = t; j < nn2; j += G) {
+                const int k = k2[j];
+                const int rk = key_rank(k2, nn2, k);
+                scol[off + rk] = k;
+                sval[off + rk] = v2[j];
+            }
+            if (t == 0) {
+                slotn[i] = nn2;
+                rownnz[r] = nn2;
+            }
+        }
+        Group<G>::sync();
+    }
+}
+
+// Move the rows of one bin from their scratch slots to the output (one
+// wavefront per row; slots marked -1 belong to the long-row path).
+__global__ void __launch_bounds__(256) k_gather_rows(const int32_t *__restrict__ rows, const int *__restrict__ count,
+                                                      const int32_t *__restrict__ slotn, int cap,
+                                                      const int32_t *__restrict__ scol,
+                                                      const double *__restrict__ sval,
+                                                      const int64_t *__restrict__ crp, int32_t *__restrict__ ccol,
+                                                      double *__restrict__ cval, int64_t out_cap,
+                                                      int *__restrict__ err)
+{
+    const int wpb = blockDim.x / 64;
+    const int n = *count;
+    for (int i = blockIdx.x * wpb + threadIdx.x / 64; i < n; i += gridDim.x * wpb) {
+        const int len = slotn[i];
+        if (len <= 0) continue;
+        const int64_t src = (int64_t)i * cap, dst = crp[rows[i]];
+        if (dst + len > out_cap) {  // output smaller than this replay needs
+            if (lane() == 0) atomicOr(err, kErrOverflow);
+            continue;
+        }
+        for (int j = lane(); j < len; j += 64) {
+            ccol[dst + j] = scol[src + j];
+            cval[dst + j] = sval[src + j];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan of int32 row lengths into int64 offsets (3 phases)
+// ---------------------------------------------------------------------------
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+__device__ __forceinline__ long long block_exclusive_scan(long long v, long long *sh, long long &total)
+{
+    // wave inclusive scan
+    const int l = lane();
+    long long x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        long long y = __shfl_up(x, o);
+        if (l >= o) x += y;
+    }
+    const int w = threadIdx.x / 64;
+    if (l == 63) sh[w] = x;
+    __syncthreads();
+    long long before = 0;
+    total = 0;
+    for (int i = 0; i < (int)(blockDim.x / 64); i++) {
+        before += (i < w) ? sh[i] : 0;
+        total += sh[i];
+    }
+    __syncthreads();
+    return before + x - v;
+}
+
+__global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const int32_t *__restrict__ in, int64_t n,
+                                                              int64_t *__restrict__ out,
+                                                              long long *__restrict__ tile_sums)
+{
+    __shared__ long long sh[kScanThreads / 64];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    long long v[kScanItems];
+    long long s = 0;
+    for (int j = 0; j < kScanItems; j++) {
+        const int64_t idx = base + j;
+        v[j] = idx < n ? (long long)in[idx] : 0;
+        s += v[j];
+    }
+    long long total;
+    long long ex = block_exclusive_scan(s, sh, total);
+    for (int j = 0; j < kScanItems; j++) {
+        const int64_t idx = base + j;
+        if (idx < n) out[idx] = ex;
+        ex += v[j];
+    }
+    if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// one workgroup scans the tile sums (up to kScanTile tiles per pass, looped)
+__global__ void __launch_bounds__(kScanThreads) k_scan_top(long long *__restrict__ tile_sums, int64_t ntiles,
+                                                            int64_t *__restrict__ total_out)
+{
+    __shared__ long long sh[kScanThreads / 64];
+    long long carry = 0;
+    for (int64_t t0 = 0; t0 < ntiles; t0 += kScanTile) {
+        const int64_t base = t0 + (int64_t)threadIdx.x * kScanItems;
+        long long v[kScanItems];
+        long long s = 0;
+        for (int j = 0; j < kScanItems; j++) {
+            const int64_t idx = base + j;
+            v[j] = idx < ntiles ? tile_sums[idx] : 0;
+            s += v[j];
+        }
+        long long total;
+        long long ex = block_exclusive_scan(s, sh, total) + carry;
+        for (int j = 0; j < kScanItems; j++) {
+            const int64_t idx = base + j;
+            if (idx < ntiles) tile_sums[idx] = ex;
+            ex += v[j];
+        }
+        carry += total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *total_out = carry;
+}
+
+__global__ void k_scan_add(int64_t *__restrict__ out, int64_t n, const long long *__restrict__ tile_off,
+                           const int64_t *__restrict__ total, int32_t *__restrict__ out32)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long tot = *total;
+    if (i < n) {
+        const long long v = out[i] + tile_off[i / kScanTile];
+        out[i] = v;
+        if (out32 && tot < 2147483647LL) out32[i] = (int32_t)v;
+    }
+    if (i == n) {
+        out[n] = tot;
+        if (out32 && tot < 2147483647LL) out32[n] = (int32_t)tot;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// long rows: one workgroup per row on a dense accumulator in HBM
+// ---------------------------------------------------------------------------
+// Each persistent workgroup owns one workspace: a presence bitmap of 32-bit
+// words and (numeric) a dense FP64 value array over the output columns.
+// Products set bits with global atomicOr and add values with global FP64
+// atomics; sorted emission walks the bitmap word by word (256 words per step,
+// one per thread, a workgroup scan giving output positions) and clears every
+// bit and value it emits, so the workspace is clean for the next row.
+constexpr int kDense = 256;
+
+__device__ __forceinline__ unsigned ld_word(const unsigned *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double ld_val(const double *p)
+{
+    const unsigned long long u = __hip_atomic_load(reinterpret_cast<const unsigned long long *>(p),
+                                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double((long long)u);
+}
+
+template <bool NUM, typename LV, typename BV = double>
+__device__ __forceinline__ void dense_expand(const int32_t *lcol, const LV *lval, int64_t e0, int64_t e1,
+                                             const int32_t *__restrict__ brp,
+                                             const int32_t *__restrict__ bcol,
+                                             const BV *__restrict__ bval, int slog, unsigned *bits,
+                                             double *vals, long long &fresh, double *psum = nullptr)
+{
+    const int t = threadIdx.x;
+    const int sub = t & ((1 << slog) - 1);
+    const int step_e = kDense >> slog;
+    const int step_b = 1 << slog;
+    for (int64_t e = e0 + (t >> slog); e < e1; e += step_e) {
+        const int k = lcol[e];
+        double lv = 0.0;
+        if constexpr (NUM) lv = (double)lval[e];
+        const int b0 = brp[k], b1 = brp[k + 1];
+        for (int q = b0 + sub; q < b1; q += step_b) {
+            const int c = bcol[q];
+            const unsigned bit = 1u << (c & 31);
+            const unsigned old = atomicOr(bits + (c >> 5), bit);
+            fresh += (old & bit) ? 0 : 1;
+            if constexpr (NUM) {
+                const double pr = lv * (double)bval[q];
+                unsafeAtomicAdd(vals + c, pr);
+                if (psum) *psum += pr;
+            }
+        }
+    }
+}
+
+// sorted emission of the set bits (and values) of a workspace; returns the
+// count.  Entries beyond `cap` are counted but not written.
+__device__ int dense_emit(unsigned *bits, double *vals, int nw, int32_t *outc, double *outv, long long cap,
+                          long long *sh)
+{
+    const int t = threadIdx.x;
+    long long base = 0;
+    for (int w0 = 0; w0 < nw; w0 += kDense) {
+        const int w = w0 + t;
+        unsigned x = w < nw ? ld_word(bits + w) : 0u;
+        long long total;
+        long long pos = base + block_exclusive_scan(__popc(x), sh, total);
+        if (x) bits[w] = 0u;
+        while (x) {
+            const int b = __ffs(x) - 1;
+            x &= x - 1;
+            const int c = w * 32 + b;
+            const bool in = pos < cap;
+            if (outc && in) outc[pos] = c;
+            if (vals) {
+                const double v = ld_val(vals + c);
+                vals[c] = 0.0;
+                if (outv && in) outv[pos] = v;
+            }
+            pos++;
+        }
+        base += total;
+    }
+    __threadfence();
+    __syncthreads();
+    return (int)base;
+}
+
+__device__ __forceinline__ long long block_sum_dense(long long v, long long *sh)
+{
+    long long total;
+    (void)block_exclusive_scan(v, sh, total);
+    return total;
+}
+
+// pairwise SpMM rows that are too long for the LDS tables
+template <bool NUM>
+__global__ void __launch_bounds__(kDense) k_dense_spgemm(SpOperand A, SpOperand B, const int32_t *__restrict__ rows,
+                                                          const int *__restrict__ count, unsigned *ws_bits,
+                                                          double *ws_vals, int nw, int slog,
+                                                          int32_t *__restrict__ rownnz,
+                                                          const int64_t *__restrict__ crp,
+                                                          int32_t *__restrict__ ccol, double *__restrict__ cval,
+                                                          int *__restrict__ err)
+{
+    __shared__ long long sh[kDense / 64];
+    unsigned *bits = ws_bits + (size_t)blockIdx.x * nw;
+    double *vals = NUM ? ws_vals + (size_t)blockIdx.x * ((size_t)nw * 32) : nullptr;
+    const double *aval = static_cast<const double *>(A.val);
+    const double *bval = static_cast<const double *>(B.val);
+    const int n = *count;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int r = rows[i];
+        long long fresh = 0;
+        dense_expand<NUM, double>(A.col, aval, A.rp[r], A.rp[r + 1], B.rp32, B.col, bval, slog, bits, vals,
+                                  fresh);
+        __threadfence();
+        __syncthreads();
+        if constexpr (!NUM) {
+            const long long tot = block_sum_dense(fresh, sh);
+            for (int w = threadIdx.x; w < nw; w += kDense) bits[w] = 0u;
+            if (threadIdx.x == 0) rownnz[r] = (int32_t)tot;
+            __threadfence();
+            __syncthreads();
+        } else {
+            const int64_t off = crp[r];
+            const int got = dense_emit(bits, vals, nw, ccol + off, cval + off, crp[r + 1] - off, sh);
+            if (threadIdx.x == 0 && got != crp[r + 1] - off) atomicOr(err, kErrOverflow);
+        }
+    }
+}
+
+// direct triple product rows that are too long for the LDS tables:
+// row of R*A densely over A's columns, compacted into a sorted list, then
+// forwarded densely over P's columns.
+template <bool NUM, typename VT = double>
+__global__ void __launch_bounds__(kDense) k_dense_direct(SpOperand R, SpOperand A, SpOperand P,
+                                                          const int32_t *__restrict__ rows,
+                                                          const int *__restrict__ count, DenseWs ws,
+                                                          int slog1, int slog2, int32_t *__restrict__ n1out,
+                                                          int32_t *__restrict__ rownnz,
+                                                          const int64_t *__restrict__ crp,
+                                                          int32_t *__restrict__ ccol, double *__restrict__ cval,
+                                                          double *__restrict__ rowsum, int *__restrict__ err)
+{
+    __shared__ long long sh[kDense / 64];
+    __shared__ double dsh[kDense / 64];
+    const size_t b = blockIdx.x;
+    unsigned *bitsA = ws.bitsA + b * ws.nwA;
+    double *valsA = NUM ? ws.valsA + b * ((size_t)ws.nwA * 32) : nullptr;
+    int32_t *listc = ws.listc + b * ((size_t)ws.nwA * 32);
+    double *listv = NUM ? ws.listv + b * ((size_t)ws.nwA * 32) : nullptr;
+    unsigned *bitsP = ws.bitsP + b * ws.nwP;
+    double *valsP = NUM ? ws.valsP + b * ((size_t)ws.nwP * 32) : nullptr;
+    const VT *rval = static_cast<const VT *>(R.val);
+    const VT *aval = static_cast<const VT *>(A.val);
+    const VT *pval = static_cast<const VT *>(P.val);
+    const int n = *count;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int r = rows[i];
+        long long f1 = 0, f2 = 0;
+        dense_expand<NUM, VT, VT>(R.col, rval, R.rp[r], R.rp[r + 1], A.rp32, A.col, aval, slog1, bitsA, valsA,
+                                  f1);
+        __threadfence();
+        __syncthreads();
+        const int n1 = dense_emit(bitsA, valsA, ws.nwA, listc, listv, (long long)ws.nwA * 32, sh);
+        double ps = 0.0;
+        dense_expand<NUM, double, VT>(listc, listv, 0, n1, P.rp32, P.col, pval, slog2, bitsP, valsP, f2, &ps);
+        __threadfence();
+        __syncthreads();
+        if (NUM && rowsum) {
+            for (int o = 32; o > 0; o >>= 1) ps += __shfl_xor(ps, o);
+            if (lane() == 0) dsh[threadIdx.x / 64] = ps;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double tot = 0.0;
+                for (int w = 0; w < kDense / 64; w++) tot += dsh[w];
+                rowsum[r] = tot;
+            }
+        }
+        if constexpr (!NUM) {
+            const long long tot = block_sum_dense(f2, sh);
+            for (int w = threadIdx.x; w < ws.nwP; w += kDense) bitsP[w] = 0u;
+            if (threadIdx.x == 0) {
+                rownnz[r] = (int32_t)tot;
+                if (n1out) n1out[r] = n1;
+            }
+            __threadfence();
+            __syncthreads();
+        } else {
+            const int64_t off = crp[r];
+            const int got = dense_emit(bitsP, valsP, ws.nwP, ccol + off, cval + off, crp[r + 1] - off, sh);
+            if (threadIdx.x == 0 && got != crp[r + 1] - off) atomicOr(err, kErrOverflow);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// upload / download conversions and validation
+// ---------------------------------------------------------------------------
+// rp64 (host offsets, index base `base`) -> 0-based rp (+ rp32); checks monotone
+__global__ void k_rp_convert(const int64_t *__restrict__ in, int64_t M, int64_t nnz, int base,
+                             int64_t *__restrict__ rp, int32_t *__restrict__ rp32,
+                             int *__restrict__ err)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > M) return;
+    const int64_t v = in[i] - base;
+    rp[i] = v;
+    if (rp32) rp32[i] = (int32_t)v;
+    bool bad = v < 0 || v > nnz;
+    if (i == 0) bad |= v != 0;
+    if (i == M) bad |= v != nnz;
+    if (i < M && in[i + 1] < in[i]) bad = true;
+    if (bad && err) atomicOr(err, kErrBadInput);
+}
+
+// int64 columns (base) -> int32 0-based; checks range and strict increase per row
+__global__ void k_col_convert(const int64_t *__restrict__ in, const int64_t *__restrict__ rp, int64_t M,
+                              int64_t N, int base, int32_t *__restrict__ col, int *__restrict__ err)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= M) return;
+    const int64_t nnz = rp[M];
+    int64_t e0 = rp[r], e1 = rp[r + 1];
+    // offsets are validated separately; never walk outside [0, nnz) whatever they hold
+    if (e0 < 0 || e1 > nnz || e0 > e1 || nnz < 0) {
+        atomicOr(err, kErrBadInput);
+        return;
+    }
+    int64_t prev = -1;
+    bool bad = false;
+    for (int64_t e = e0; e < e1; e++) {
+        const int64_t c = in[e] - base;
+        bad |= c < 0 || c >= N || c <= prev;
+        prev = c;
+        col[e] = (int32_t)c;
+    }
+    if (bad) atomicOr(err, kErrBadInput);
+}
+
+__global__ void k_col_widen(const int32_t *__restrict__ col, int64_t nnz, int base, int64_t *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nnz) out[i] = (int64_t)col[i] + base;
+}
+
+__global__ void k_rp_shift(const int64_t *__restrict__ rp, int64_t n, int base, int64_t *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = rp[i] + base;
+}
+
+__global__ void k_f64_to_f32(const double *__restrict__ in, int64_t n, float *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (float)in[i];
+}
+
+__global__ void k_f32_to_f64(const float *__restrict__ in, int64_t n, double *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (double)in[i];
+}
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_copy_probe(const double4 *__restrict__ in, double4 *__restrict__ out, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = in[i];
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+static inline unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
+
+void launch_ub_bin(const SpOperand &A, const SpOperand &B, int64_t *ub, int32_t *rownnz,
+                   int32_t *lists, int *counts, const BinThresholds &thr,
+                   unsigned long long *totals, hipStream_t st)
+{
+    if (A.M == 0) return;
+    hipLaunchKernelGGL(k_ub_bin, dim3(blocks_for(A.M, kBinRows)), dim3(kBinThreads), 0, st, A.rp, A.col, B.rp32,
+                       A.M, ub, rownnz, lists, counts, thr, totals);
+}
+
+void launch_ub_colparts(const int64_t *arp, const int32_t *acol, int64_t M, const int64_t *boff, int G,
+                        int64_t *out, unsigned long long *total, hipStream_t st)
+{
+    if (M == 0) return;
+    hipLaunchKernelGGL(k_ub_colparts, dim3(blocks_for(M, 256)), dim3(256), 0, st, arp, acol, M, boff, G, out,
+                       total);
+}
+
+void launch_rebin(const int32_t *m1, const int32_t *m2, int64_t M, int32_t *lists, int *counts,
+                  const BinThresholds &thr, hipStream_t st)
+{
+    if (M == 0) return;
+    hipLaunchKernelGGL(k_rebin, dim3(blocks_for(M, kBinRows)), dim3(kBinThreads), 0, st, m1, m2, M, lists, counts,
+                       thr);
+}
+
+static unsigned grid_for(int rows, int G, int max_blocks)
+{
+    const int gpb = 256 / G;
+    int64_t b = (rows + gpb - 1) / gpb;
+    if (b > max_blocks) b = max_blocks;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+size_t symb_lds(int G, int H) { return (size_t)(256 / G) * H * 4 + (size_t)(256 / G) * (G / 64) * 8; }
+size_t num_lds(int G, int H) { return (size_t)(256 / G) * H * 12 + (size_t)(256 / G) * (G / 64 + 1) * 4; }
+size_t direct_lds(int G, int H1, int H2, bool num)
+{
+    const size_t gpb = 256 / G;
+    size_t b = gpb * ((size_t)H1 + H2) * (num ? 12 : 4);
+    b += gpb * (G / 64) * 8 + gpb * (G / 64 + 1) * 4;
+    return b;
+}
+
+void launch_symb(int G, int H, int slog, const SpOperand &A, const SpOperand &B, const int32_t *rows,
+                 const int *count, int rows_host, int32_t *rownnz, int32_t *retry, int *nretry,
+                 int *err, int max_blocks, hipStream_t st)
+{
+    const unsigned grid = grid_for(rows_host, G, max_blocks);
+    const size_t lds = symb_lds(G, H);
+    if (G == 64)
+        hipLaunchKernelGGL(k_symb<64>, dim3(grid), dim3(256), lds, st, A, B, rows, count, H, slog,
+                           rownnz, retry, nretry, err);
+    else
+        hipLaunchKernelGGL(k_symb<256>, dim3(grid), dim3(256), lds, st, A, B, rows, count, H, slog,
+                           rownnz, retry, nretry, err);
+}
+
+void launch_num(int G, int H, int slog, const SpOperand &A, const SpOperand &B, const int32_t *rows,
+                const int *count, int rows_host, const int64_t *crp, int32_t *ccol, double *cval,
+                int *err, int max_blocks, hipStream_t st)
+{
+    const unsigned grid = grid_for(rows_host, G, max_blocks);
+    const size_t lds = num_lds(G, H);
+    if (G == 64)
+        hipLaunchKernelGGL(k_num<64>, dim3(grid), dim3(256), lds, st, A, B, rows, count, H, slog, crp,
+                           ccol, cval, err);
+    else
+        hipLaunchKernelGGL(k_num<256>, dim3(grid), dim3(256), lds, st, A, B, rows, count, H, slog, crp,
+                           ccol, cval, err);
+}
+
+void launch_direct(int G, bool num, int H1, int H2, int slog1, int slog2, const SpOperand &R,
+                   const SpOperand &A, const SpOperand &P, const int32_t *rows, const int *count,
+                   int rows_host, int32_t *n1out, int32_t *rownnz, const int64_t *crp, int32_t *ccol,
+                   double *cval, int32_t *retry, int *nretry, int *err, int max_blocks,
+                   hipStream_t st, bool f32in)
+{
+    const unsigned grid = grid_for(rows_host, G, max_blocks);
+    const size_t lds = direct_lds(G, H1, H2, num);
+#define SP3MM_DIRECT_ARGS R, A, P, rows, count, H1, H2, slog1, slog2, n1out, rownnz, crp, ccol, cval, \
+                          retry, nretry, err
+    if (f32in) {
+        if (!num) return;  // the mixed path only recomputes numerics from FP32 inputs
+        if (G == 64) hipLaunchKernelGGL((k_direct<64, true, float>), dim3(grid), dim3(256), lds, st, SP3MM_DIRECT_ARGS);
+        else hipLaunchKernelGGL((k_direct<256, true, float>), dim3(grid), dim3(256), lds, st, SP3MM_DIRECT_ARGS);
+    } else if (G == 64) {
+        if (num) hipLaunchKernelGGL((k_direct<64, true>), dim3(grid), dim3(256), lds, st, SP3MM_DIRECT_ARGS);
+        else hipLaunchKernelGGL((k_direct<64, false>), dim3(grid), dim3(256), lds, st, SP3MM_DIRECT_ARGS);
+    } else {
+        if (num) hipLaunchKernelGGL((k_direct<256, true>), dim3(grid), dim3(256), lds, st, SP3MM_DIRECT_ARGS);
+        else hipLaunchKernelGGL((k_direct<256, false>), dim3(grid), dim3(256), lds, st, SP3MM_DIRECT_ARGS);
+    }
+#undef SP3MM_DIRECT_ARGS
+}
+
+int dense_words(int64_t ncols) { return (int)((ncols + 31) / 32) + 1; }
+
+void launch_dense_spgemm(bool num, int blocks, const SpOperand &A, const SpOperand &B, const int32_t *rows,
+                         const int *count, unsigned *bits, double *vals, int nw, int slog, int32_t *rownnz,
+                         const int64_t *crp, int32_t *ccol, double *cval, int *err, hipStream_t st)
+{
+    if (num)
+        hipLaunchKernelGGL(k_dense_spgemm<true>, dim3(blocks), dim3(kDense), 0, st, A, B, rows, count, bits, vals,
+                           nw, slog, rownnz, crp, ccol, cval, err);
+    else
+        hipLaunchKernelGGL(k_dense_spgemm<false>, dim3(blocks), dim3(kDense), 0, st, A, B, rows, count, bits,
+                           vals, nw, slog, rownnz, crp, ccol, cval, err);
+}
+
+void launch_dense_direct(bool num, int blocks, const SpOperand &R, const SpOperand &A, const SpOperand &P,
+                         const int32_t *rows, const int *count, const DenseWs &ws, int slog1, int slog2,
+                         int32_t *n1out, int32_t *rownnz, const int64_t *crp, int32_t *ccol, double *cval,
+                         int *err, hipStream_t st, bool f32in, double *rowsum)
+{
+    if (f32in) {
+        if (num)
+            hipLaunchKernelGGL((k_dense_direct<true, float>), dim3(blocks), dim3(kDense), 0, st, R, A, P, rows, count,
+                               ws, slog1, slog2, n1out, rownnz, crp, ccol, cval, rowsum, err);
+        else
+            hipLaunchKernelGGL((k_dense_direct<false, float>), dim3(blocks), dim3(kDense), 0, st, R, A, P, rows,
+                               count, ws, slog1, slog2, n1out, rownnz, crp, ccol, cval, rowsum, err);
+        return;
+    }
+    if (num)
+        hipLaunchKernelGGL(k_dense_direct<true>, dim3(blocks), dim3(kDense), 0, st, R, A, P, rows, count, ws,
+                           slog1, slog2, n1out, rownnz, crp, ccol, cval, rowsum, err);
+    else
+        hipLaunchKernelGGL(k_dense_direct<false>, dim3(blocks), dim3(kDense), 0, st, R, A, P, rows, count, ws,
+                           slog1, slog2, n1out, rownnz, crp, ccol, cval, rowsum, err);
+}
+
+size_t fused_lds(int G, int H1, int H2)
+{
+    const size_t gpb = 256 / G;
+    return gpb * ((size_t)H1 + H2) * 12 + gpb * (G / 64) * 8 + gpb * (G / 64 + 1) * 4;
+}
+
+void launch_direct_fused(int G, int H1, int H2, int slog1, int slog2, const SpOperand &R, const SpOperand &A,
+                         const SpOperand &P, const int32_t *rows, const int *count, int rows_host,
+                         int32_t *rownnz, int32_t *slotn, int32_t *scol, double *sval, int32_t *retry,
+                         int *nretry, int *err, int max_blocks, hipStream_t st)
+{
+    const unsigned grid = grid_for(rows_host, G, max_blocks);
+    const size_t lds = fused_lds(G, H1, H2);
+    if (G == 64)
+        hipLaunchKernelGGL(k_direct_fused<64>, dim3(grid), dim3(256), lds, st, R, A, P, rows, count, H1, H2, slog1,
+                           slog2, rownnz, slotn, scol, sval, retry, nretry, err);
+    else
+        hipLaunchKernelGGL(k_direct_fused<256>, dim3(grid), dim3(256), lds, st, R, A, P, rows, count, H1, H2,
+                           slog1, slog2, rownnz, slotn, scol, sval, retry, nretry, err);
+}
+
+void launch_gather_rows(const int32_t *rows, const int *count, int rows_host, const int32_t *slotn, int cap,
+                        const int32_t *scol, const double *sval, const int64_t *crp, int32_t *ccol, double *cval,
+                        int64_t out_cap, int *err, int max_blocks, hipStream_t st)
+{
+    const unsigned grid = grid_for(rows_host, 64, max_blocks);
+    hipLaunchKernelGGL(k_gather_rows, dim3(grid), dim3(256), 0, st, rows, count, slotn, cap, scol, sval, crp, ccol,
+                       cval, out_cap, err);
+}
+
+__global__ void k_check_nnz(const int64_t *nnz, int64_t want, const int *nretry, int want_retry, int *err)
+{
+    if (threadIdx.x == 0 && (*nnz != want || *nretry != want_retry)) atomicOr(err, kErrOverflow);
+}
+
+void launch_check_nnz(const int64_t *nnz, int64_t want, const int *nretry, int want_retry, int *err, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_check_nnz, dim3(1), dim3(64), 0, st, nnz, want, nretry, want_retry, err);
+}
+
+// ---------------------------------------------------------------------------
+// batches: block-diagonal packing of many operands, and extraction of one block
+// ---------------------------------------------------------------------------
+// blockIdx.y = problem; its rows get offset roff, its entries nzoff, its
+// columns coloff.  The packed row offsets are written for rows [roff, roff+M).
+template <typename VT>
+__global__ void k_pack(const PackItem *__restrict__ items, int64_t *__restrict__ rp, int32_t *__restrict__ col,
+                       VT *__restrict__ val)
+{
+    const PackItem it = items[blockIdx.y];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int64_t r = t0; r < it.M; r += stride) rp[it.roff + r] = it.rp[r] + it.nzoff;
+    const VT *v = static_cast<const VT *>(it.val);
+    for (int64_t e = t0; e < it.nnz; e += stride) {
+        col[it.nzoff + e] = it.col[e] + it.coloff;
+        val[it.nzoff + e] = v[e];
+    }
+}
+
+void launch_pack(const PackItem *items, int count, int vtype, int64_t *rp, int32_t *col, void *val, hipStream_t st)
+{
+    if (count <= 0) return;
+    if (vtype == 1)
+        hipLaunchKernelGGL(k_pack<float>, dim3(64, count), dim3(256), 0, st, items, rp, col, static_cast<float *>(val));
+    else
+        hipLaunchKernelGGL(k_pack<double>, dim3(64, count), dim3(256), 0, st, items, rp, col,
+                           static_cast<double *>(val));
+}
+
+// rows [r0, r0+M) of a packed product, rebased to 0 and shifted back by coloff
+__global__ void k_extract(const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
+                          const double *__restrict__ val, int64_t r0, int64_t M, int32_t coloff,
+                          int64_t *__restrict__ orp, int32_t *__restrict__ orp32, int32_t *__restrict__ ocol,
+                          double *__restrict__ oval)
+{
+    const int64_t base = rp[r0];
+    const int64_t nnz = rp[r0 + M] - base;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int64_t r = t0; r <= M; r += stride) {
+        const int64_t v = rp[r0 + r] - base;
+        orp[r] = v;
+        if (orp32) orp32[r] = (int32_t)v;
+    }
+    for (int64_t e = t0; e < nnz; e += stride) {
+        ocol[e] = col[base + e] - coloff;
+        oval[e] = val[base + e];
+    }
+}
+
+void launch_extract(const int64_t *rp, const int32_t *col, const double *val, int64_t r0, int64_t M, int32_t coloff,
+                    int64_t *orp, int32_t *orp32, int32_t *ocol, double *oval, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_extract, dim3(256), dim3(256), 0, st, rp, col, val, r0, M, coloff, orp, orp32, ocol, oval);
+}
+
+void launch_scan(const int32_t *in, int64_t n, int64_t *out, int32_t *out32, long long *tile_sums,
+                 int64_t *total, hipStream_t st)
+{
+    const int64_t ntiles = (n + kScanTile - 1) / kScanTile;
+    if (n > 0)
+        hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(kScanThreads), 0, st, in, n, out,
+                           tile_sums);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, st, tile_sums, ntiles, total);
+    hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(n + 1, 256)), dim3(256), 0, st, out, n, tile_sums, total,
+                       out32);
+}
+
+int64_t scan_tiles(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+void launch_rp_convert(const int64_t *in, int64_t M, int64_t nnz, int base, int64_t *rp, int32_t *rp32,
+                       int *err, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_rp_convert, dim3(blocks_for(M + 1, 256)), dim3(256), 0, st, in, M, nnz, base, rp,
+                       rp32, err);
+}
+
+void launch_col_convert(const int64_t *in, const int64_t *rp, int64_t M, int64_t N, int base, int32_t *col,
+                        int *err, hipStream_t st)
+{
+    if (M == 0) return;
+    hipLaunchKernelGGL(k_col_convert, dim3(blocks_for(M, 256)), dim3(256), 0, st, in, rp, M, N, base, col,
+                       err);
+}
+
+void launch_col_widen(const int32_t *col, int64_t nnz, int base, int64_t *out, hipStream_t st)
+{
+    if (nnz == 0) return;
+    hipLaunchKernelGGL(k_col_widen, dim3(blocks_for(nnz, 256)), dim3(256), 0, st, col, nnz, base, out);
+}
+
+void launch_rp_shift(const int64_t *rp, int64_t n, int base, int64_t *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_rp_shift, dim3(blocks_for(n, 256)), dim3(256), 0, st, rp, n, base, out);
+}
+
+void launch_f64_to_f32(const double *in, int64_t n, float *out, hipStream_t st)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(blocks_for(n, 256)), dim3(256), 0, st, in, n, out);
+}
+
+void launch_f32_to_f64(const float *in, int64_t n, double *out, hipStream_t st)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(blocks_for(n, 256)), dim3(256), 0, st, in, n, out);
+}
+
+void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, n, v);
+}
+
+void launch_copy_probe(const void *in, void *out, int64_t bytes, hipStream_t st)
+{
+    const int64_t n = bytes / 32;
+    hipLaunchKernelGGL(k_copy_probe, dim3(4096), dim3(256), 0, st, (const double4 *)in, (double4 *)out, n);
+}
+
+}  // namespace sp3mm

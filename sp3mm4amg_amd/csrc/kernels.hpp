@@ -1,0 +1,116 @@
+// kernels.hpp -- host-side declarations of the gfx950 kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace sp3mm {
+
+// Device view of one CSR operand.  Left operands are read through rp (int64),
+// right operands through rp32 (their nnz must be < 2^31).
+struct SpOperand {
+    const int64_t *rp;
+    const int32_t *rp32;
+    const int32_t *col;
+    const void *val;
+    int64_t M, N;
+};
+
+constexpr int kMaxBins = 14;
+
+// Row goes to the first bin b with measure <= v[b]; above v[nb-1] -> bin nb.
+struct BinThresholds {
+    int nb;
+    long long v[kMaxBins];
+};
+
+void launch_ub_bin(const SpOperand &A, const SpOperand &B, int64_t *ub, int32_t *rownnz,
+                   int32_t *lists, int *counts, const BinThresholds &thr,
+                   unsigned long long *totals, hipStream_t st);
+void launch_ub_colparts(const int64_t *arp, const int32_t *acol, int64_t M, const int64_t *boff, int G,
+                        int64_t *out, unsigned long long *total, hipStream_t st);
+void launch_rebin(const int32_t *m1, const int32_t *m2, int64_t M, int32_t *lists, int *counts,
+                  const BinThresholds &thr, hipStream_t st);
+
+size_t symb_lds(int G, int H);
+size_t num_lds(int G, int H);
+size_t direct_lds(int G, int H1, int H2, bool num);
+
+void launch_symb(int G, int H, int slog, const SpOperand &A, const SpOperand &B, const int32_t *rows,
+                 const int *count, int rows_host, int32_t *rownnz, int32_t *retry, int *nretry,
+                 int *err, int max_blocks, hipStream_t st);
+void launch_num(int G, int H, int slog, const SpOperand &A, const SpOperand &B, const int32_t *rows,
+                const int *count, int rows_host, const int64_t *crp, int32_t *ccol, double *cval,
+                int *err, int max_blocks, hipStream_t st);
+void launch_direct(int G, bool num, int H1, int H2, int slog1, int slog2, const SpOperand &R,
+                   const SpOperand &A, const SpOperand &P, const int32_t *rows, const int *count,
+                   int rows_host, int32_t *n1out, int32_t *rownnz, const int64_t *crp, int32_t *ccol,
+                   double *cval, int32_t *retry, int *nretry, int *err, int max_blocks,
+                   hipStream_t st, bool f32in = false);
+
+// per-workgroup dense workspaces of the long-row kernels (nw* = words per bitmap,
+// arrays of values/list entries hold nw*32 slots per workgroup)
+struct DenseWs {
+    unsigned *bitsA;
+    double *valsA;
+    int32_t *listc;
+    double *listv;
+    unsigned *bitsP;
+    double *valsP;
+    int nwA, nwP;
+};
+
+int dense_words(int64_t ncols);
+void launch_dense_spgemm(bool num, int blocks, const SpOperand &A, const SpOperand &B, const int32_t *rows,
+                         const int *count, unsigned *bits, double *vals, int nw, int slog, int32_t *rownnz,
+                         const int64_t *crp, int32_t *ccol, double *cval, int *err, hipStream_t st);
+void launch_dense_direct(bool num, int blocks, const SpOperand &R, const SpOperand &A, const SpOperand &P,
+                         const int32_t *rows, const int *count, const DenseWs &ws, int slog1, int slog2,
+                         int32_t *n1out, int32_t *rownnz, const int64_t *crp, int32_t *ccol, double *cval,
+                         int *err, hipStream_t st, bool f32in = false, double *rowsum = nullptr);
+
+size_t fused_lds(int G, int H1, int H2);
+void launch_direct_fused(int G, int H1, int H2, int slog1, int slog2, const SpOperand &R, const SpOperand &A,
+                         const SpOperand &P, const int32_t *rows, const int *count, int rows_host,
+                         int32_t *rownnz, int32_t *slotn, int32_t *scol, double *sval, int32_t *retry,
+                         int *nretry, int *err, int max_blocks, hipStream_t st);
+void launch_gather_rows(const int32_t *rows, const int *count, int rows_host, const int32_t *slotn, int cap,
+                        const int32_t *scol, const double *sval, const int64_t *crp, int32_t *ccol, double *cval,
+                        int64_t out_cap, int *err, int max_blocks, hipStream_t st);
+void launch_check_nnz(const int64_t *nnz, int64_t want, const int *nretry, int want_retry, int *err, hipStream_t st);
+
+size_t mixed_lds(int G, int H1, int H2);
+void launch_direct_mixed(int G, int H1, int H2, int slog1, int slog2, float guard, const SpOperand &R,
+                         const SpOperand &A, const SpOperand &P, const int32_t *rows, const int *count,
+                         int rows_host, int32_t *rownnz, int32_t *slotn, int32_t *scol, double *sval,
+                         double *rowsum, int32_t *retry, int *nretry, int32_t *fix, int *nfix, int *err,
+                         int max_blocks, hipStream_t st);
+
+struct PackItem {
+    const int64_t *rp;
+    const int32_t *col;
+    const void *val;
+    int64_t M, nnz, roff, nzoff;
+    int32_t coloff;
+    int32_t pad;
+};
+void launch_pack(const PackItem *items, int count, int vtype, int64_t *rp, int32_t *col, void *val, hipStream_t st);
+void launch_extract(const int64_t *rp, const int32_t *col, const double *val, int64_t r0, int64_t M, int32_t coloff,
+                    int64_t *orp, int32_t *orp32, int32_t *ocol, double *oval, hipStream_t st);
+
+int64_t scan_tiles(int64_t n);
+void launch_scan(const int32_t *in, int64_t n, int64_t *out, int32_t *out32, long long *tile_sums,
+                 int64_t *total, hipStream_t st);
+
+void launch_rp_convert(const int64_t *in, int64_t M, int64_t nnz, int base, int64_t *rp, int32_t *rp32,
+                       int *err, hipStream_t st);
+void launch_col_convert(const int64_t *in, const int64_t *rp, int64_t M, int64_t N, int base, int32_t *col,
+                        int *err, hipStream_t st);
+void launch_col_widen(const int32_t *col, int64_t nnz, int base, int64_t *out, hipStream_t st);
+void launch_rp_shift(const int64_t *rp, int64_t n, int base, int64_t *out, hipStream_t st);
+void launch_f64_to_f32(const double *in, int64_t n, float *out, hipStream_t st);
+void launch_f32_to_f64(const float *in, int64_t n, double *out, hipStream_t st);
+void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
+void launch_copy_probe(const void *in, void *out, int64_t bytes, hipStream_t st);
+
+}  // namespace sp3mm

@@ -1,0 +1,457 @@
+"""Python mirror of the Sp3MM interface over the C ABI of libsp3mm_amd.so.
+
+Two layers, as in include/sp3mm_amd.h:
+
+* the reference-compatible entry points (``sp3mmRowByRowPair_0`` ...), called
+  with host ``spmat`` structs built from ``CSR`` arrays -- the same calls a C or
+  Fortran caller of Sp3MM4AMG makes (code/test/Sp3MM_test.c:121,254);
+* the device-resident interface (``DeviceCSR``, ``rap``, ``spgemm``) used by
+  the benchmark and by callers that keep matrices in HBM.
+
+There is no fallback: every function raises ``RuntimeError`` when the HIP
+library is missing or the GPU call fails.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .csr import CSR
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsp3mm_amd.so")
+_LIB = None
+
+F64, F32 = 0, 1
+PAIR, DIRECT = 0, 1
+FP64, MIXED = 0, 1
+RBTREE, IDXMAP = 0, 1
+
+
+class Spmat(C.Structure):
+    """code/src/include/sparseMatrix.h:41-53"""
+    _fields_ = [("NZ", C.c_ulong), ("M", C.c_ulong), ("N", C.c_ulong),
+                ("AS", C.POINTER(C.c_double)), ("JA", C.POINTER(C.c_ulong)),
+                ("IRP", C.POINTER(C.c_ulong)), ("MAX_ROW_NZ", C.c_ulong)]
+
+
+class Config(C.Structure):
+    """code/src/include/config.h:42-49"""
+    _fields_ = [("gridRows", C.c_ushort), ("gridCols", C.c_ushort),
+                ("symbMMRowImplID", C.c_int), ("threadNum", C.c_uint),
+                ("chunkDistrbFunc", C.c_void_p)]
+
+
+class DMat(C.Structure):
+    _fields_ = [("M", C.c_int64), ("N", C.c_int64), ("nnz", C.c_int64),
+                ("rp", C.c_void_p), ("rp32", C.c_void_p), ("col", C.c_void_p),
+                ("val", C.c_void_p), ("vtype", C.c_int32), ("owned", C.c_int32)]
+
+
+class Opts(C.Structure):
+    _fields_ = [("path", C.c_int32), ("precision", C.c_int32), ("guard", C.c_double),
+                ("timing", C.c_int32), ("reserved", C.c_int32), ("rowsum", C.c_void_p)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("ms_total", C.c_double), ("ms_ub", C.c_double), ("ms_symbolic", C.c_double),
+                ("ms_scan", C.c_double), ("ms_numeric", C.c_double), ("ms_main", C.c_double),
+                ("mac1", C.c_int64), ("mac2", C.c_int64), ("nnz_mid", C.c_int64),
+                ("nnz_out", C.c_int64), ("guarded_rows", C.c_int64),
+                ("guarded_entries", C.c_int64), ("main_kernel_launches", C.c_int32),
+                ("reserved", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+SPMM_NAMES = ["spmmSerial_", "spmmRowByRow_", "spmmRowByRow1DBlocks_", "spmmRowByRow2DBlocks_",
+              "spmmRowByRow2DBlocksAllocated_", "spmmRowByRow_SymbNum_",
+              "spmmRowByRow1DBlocks_SymbNum_", "spmmRowByRow2DBlocks_SymbNum_"]
+
+
+def lib():
+    """Load libsp3mm_amd.so (raises if it is not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run python -m sp3mm4amg_amd.build")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        for b in (0, 1):
+            for n in ("sp3mmRowByRowPair_", "sp3mmRowByRowMerged_"):
+                f = getattr(L, f"{n}{b}")
+                f.argtypes = [P(Spmat), P(Spmat), P(Spmat), P(Config), C.c_void_p]
+                f.restype = P(Spmat)
+            for n in SPMM_NAMES:
+                f = getattr(L, f"{n}{b}")
+                f.argtypes = [P(Spmat), P(Spmat), P(Config)]
+                f.restype = P(Spmat)
+            f = getattr(L, f"spMMSizeUpperbound_{b}")
+            f.argtypes = [P(Spmat), P(Spmat)]
+            f.restype = P(C.c_ulong)
+            f = getattr(L, f"spMMSizeUpperboundColParts_{b}")
+            f.argtypes = [P(Spmat), P(Spmat), C.c_ushort, P(C.c_ulong)]
+            f.restype = P(C.c_ulong)
+            f = getattr(L, f"SpMM_Symb___{b}")
+            f.argtypes = [C.c_int, P(Spmat), P(Spmat)]
+            f.restype = P(C.c_ulong)
+            f = getattr(L, f"SpMM_Symb_OutIdxs__{b}")
+            f.argtypes = [C.c_int, P(Spmat), P(Spmat), P(P(P(C.c_ulong)))]
+            f.restype = P(C.c_ulong)
+            f = getattr(L, f"Sp3MM_Symb_OutIdxs_{b}")
+            f.argtypes = [C.c_int, P(Spmat), P(Spmat), P(Spmat), P(P(P(C.c_ulong)))]
+            f.restype = P(C.c_ulong)
+        L.freeSpmat.argtypes = [P(Spmat)]
+        L.C_FortranShiftIdxs.argtypes = [P(Spmat)]
+        L.Fortran_C_ShiftIdxs.argtypes = [P(Spmat)]
+        L.allocSpMatrix.argtypes = [C.c_ulong, C.c_ulong]
+        L.allocSpMatrix.restype = P(Spmat)
+        L.sp3mm_init.argtypes = [C.c_int]
+        L.sp3mm_last_error.restype = C.c_char_p
+        L.sp3mm_dmat_upload.argtypes = [P(DMat), C.c_int64, C.c_int64, C.c_int64, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_int32, C.c_int32]
+        L.sp3mm_dmat_download.argtypes = [P(DMat), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
+        L.sp3mm_dmat_convert.argtypes = [P(DMat), C.c_int32]
+        L.sp3mm_dmat_free.argtypes = [P(DMat)]
+        L.sp3mm_dmat_free.restype = None
+        L.sp3mm_rap.argtypes = [P(DMat), P(DMat), P(DMat), P(DMat), P(Opts), P(Stats)]
+        L.sp3mm_spgemm.argtypes = [P(DMat), P(DMat), P(DMat), P(Stats)]
+        L.sp3mm_ub.argtypes = [P(DMat), P(DMat), C.c_void_p]
+        L.sp3mm_symbolic.argtypes = [P(DMat), P(DMat), P(DMat), C.c_void_p]
+        L.sp3mm_batch_create.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, P(Opts), C.c_int32,
+                                         P(C.c_void_p)]
+        L.sp3mm_batch_run.argtypes = [C.c_void_p, C.c_int32, P(C.c_double)]
+        L.sp3mm_batch_result.argtypes = [C.c_void_p, C.c_int32, P(DMat)]
+        L.sp3mm_batch_info.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_int64), P(C.c_int32)]
+        L.sp3mm_batch_free.argtypes = [C.c_void_p]
+        L.sp3mm_batch_free.restype = None
+        L.sp3mm_device_alloc.argtypes = [C.c_int64]
+        L.sp3mm_device_alloc.restype = C.c_void_p
+        L.sp3mm_device_free.argtypes = [C.c_void_p]
+        L.sp3mm_device_free.restype = None
+        L.sp3mm_copy_to_host.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+        L.sp3mm_sync.restype = C.c_int
+        L.sp3mm_copy_probe.argtypes = [C.c_int64, C.c_int32]
+        L.sp3mm_copy_probe.restype = C.c_double
+        _LIB = L
+    return _LIB
+
+
+def last_error() -> str:
+    return lib().sp3mm_last_error().decode(errors="replace")
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise RuntimeError(f"{what} failed: {last_error()}")
+
+
+def init(device: int = 0) -> None:
+    _check(lib().sp3mm_init(device), "sp3mm_init")
+
+
+# ---------------------------------------------------------------------------
+# device-resident interface
+# ---------------------------------------------------------------------------
+class DeviceCSR:
+    """A sorted CSR matrix resident in HBM (owned by the library's pool)."""
+
+    def __init__(self, d: DMat):
+        self.d = d
+
+    @classmethod
+    def upload(cls, m: CSR, vtype: int = F64, base: int = 0) -> "DeviceCSR":
+        irp = np.ascontiguousarray(m.irp + base, dtype=np.int64)
+        ja = np.ascontiguousarray(m.ja + base, dtype=np.int64)
+        vals = np.ascontiguousarray(m.vals, dtype=np.float32 if vtype == F32 else np.float64)
+        d = DMat()
+        _check(lib().sp3mm_dmat_upload(C.byref(d), m.M, m.N, m.nnz, irp.ctypes.data,
+                                       ja.ctypes.data, vals.ctypes.data, vtype, base),
+               "sp3mm_dmat_upload")
+        return cls(d)
+
+    @property
+    def shape(self):
+        return (self.d.M, self.d.N)
+
+    @property
+    def nnz(self) -> int:
+        return self.d.nnz
+
+    def download(self, base: int = 0) -> CSR:
+        M, nz = self.d.M, self.d.nnz
+        irp = np.empty(M + 1, dtype=np.int64)
+        ja = np.empty(max(nz, 1), dtype=np.int64)
+        vals = np.empty(max(nz, 1), dtype=np.float32 if self.d.vtype == F32 else np.float64)
+        _check(lib().sp3mm_dmat_download(C.byref(self.d), irp.ctypes.data, ja.ctypes.data,
+                                         vals.ctypes.data, base), "sp3mm_dmat_download")
+        return CSR(M, self.d.N, irp - base, ja[:nz] - base, vals[:nz])
+
+    def convert(self, vtype: int) -> None:
+        _check(lib().sp3mm_dmat_convert(C.byref(self.d), vtype), "sp3mm_dmat_convert")
+
+    def free(self) -> None:
+        if self.d.owned:
+            lib().sp3mm_dmat_free(C.byref(self.d))
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def rap(R: DeviceCSR, A: DeviceCSR, P: DeviceCSR, path: int = DIRECT, precision: int = FP64,
+        guard: float = 0.0, timing: bool = False, rowsum_dev: int | None = None):
+    """R*A*P on device-resident operands; returns (DeviceCSR, stats dict)."""
+    out = DMat()
+    st = Stats()
+    o = Opts(path, precision, guard, int(timing), 0, rowsum_dev)
+    _check(lib().sp3mm_rap(C.byref(R.d), C.byref(A.d), C.byref(P.d), C.byref(out), C.byref(o),
+                           C.byref(st)), "sp3mm_rap")
+    return DeviceCSR(out), st.as_dict()
+
+
+def spgemm(A: DeviceCSR, B: DeviceCSR):
+    out = DMat()
+    st = Stats()
+    _check(lib().sp3mm_spgemm(C.byref(A.d), C.byref(B.d), C.byref(out), C.byref(st)),
+           "sp3mm_spgemm")
+    return DeviceCSR(out), st.as_dict()
+
+
+def ub(A: DeviceCSR, B: DeviceCSR) -> np.ndarray:
+    out = np.zeros(A.d.M + 1, dtype=np.int64)
+    _check(lib().sp3mm_ub(C.byref(A.d), C.byref(B.d), out.ctypes.data), "sp3mm_ub")
+    return out
+
+
+def symbolic(A: DeviceCSR, B: DeviceCSR, Cm: DeviceCSR | None = None) -> np.ndarray:
+    out = np.zeros(A.d.M + 1, dtype=np.int64)
+    _check(lib().sp3mm_symbolic(C.byref(A.d), C.byref(B.d), C.byref(Cm.d) if Cm else None,
+                                out.ctypes.data), "sp3mm_symbolic")
+    return out
+
+
+class DeviceArray:
+    """A raw device buffer from the library's pool (e.g. the FP64 row sums)."""
+
+    def __init__(self, n: int, dtype=np.float64):
+        self.n, self.dtype = n, np.dtype(dtype)
+        self.ptr = lib().sp3mm_device_alloc(self.n * self.dtype.itemsize)
+        if not self.ptr:
+            raise RuntimeError("sp3mm_device_alloc failed")
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.n, dtype=self.dtype)
+        _check(lib().sp3mm_copy_to_host(out.ctypes.data, self.ptr, out.nbytes), "sp3mm_copy_to_host")
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            lib().sp3mm_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Many products R[i]*A[i]*P[i] packed into one block-diagonal product
+    (optionally captured as one HIP graph)."""
+
+    def __init__(self, Rs, As, Ps, mixed: bool = False, guard: float = 0.0, graph: bool = True):
+        n = len(Rs)
+        arr = DMat * n
+        self._R, self._A, self._P = arr(*[m.d for m in Rs]), arr(*[m.d for m in As]), arr(*[m.d for m in Ps])
+        o = Opts(DIRECT, MIXED if mixed else FP64, guard, 0, 0, None)
+        h = C.c_void_p()
+        _check(lib().sp3mm_batch_create(n, self._R, self._A, self._P, C.byref(o), int(graph), C.byref(h)),
+               "sp3mm_batch_create")
+        self.h = h
+        self.count = n
+
+    def run(self, graph: bool = True) -> float:
+        ms = C.c_double()
+        _check(lib().sp3mm_batch_run(self.h, int(graph), C.byref(ms)), "sp3mm_batch_run")
+        return ms.value
+
+    def result(self, i: int) -> DeviceCSR:
+        d = DMat()
+        _check(lib().sp3mm_batch_result(self.h, i, C.byref(d)), "sp3mm_batch_result")
+        return DeviceCSR(d)
+
+    def info(self):
+        a, b, n, g = C.c_double(), C.c_double(), C.c_int64(), C.c_int32()
+        lib().sp3mm_batch_info(self.h, C.byref(a), C.byref(b), C.byref(n), C.byref(g))
+        return {"ms_pack": a.value, "ms_build": b.value, "nnz": n.value, "graph": bool(g.value)}
+
+    def free(self) -> None:
+        if self.h:
+            lib().sp3mm_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def rap_mixed_host(R: CSR, A: CSR, P: CSR, guard: float = 0.0, rowsum: bool = False):
+    """Mixed-precision direct product from host matrices (values rounded to FP32
+    on upload); returns (CSR with FP64 values, stats, row sums or None)."""
+    dR, dA, dP = (DeviceCSR.upload(m, vtype=F32) for m in (R, A, P))
+    rs = DeviceArray(R.M) if rowsum else None
+    out, st = rap(dR, dA, dP, path=DIRECT, precision=MIXED, guard=guard,
+                  rowsum_dev=rs.ptr if rs else None)
+    res = out.download()
+    sums = rs.numpy() if rs else None
+    for m in (dR, dA, dP, out):
+        m.free()
+    if rs:
+        rs.free()
+    return res, st, sums
+
+
+def copy_probe(nbytes: int, reps: int = 20) -> float:
+    """Milliseconds per device-to-device copy of nbytes (HBM bandwidth probe)."""
+    return lib().sp3mm_copy_probe(int(nbytes), int(reps))
+
+
+def rap_host(R: CSR, A: CSR, P: CSR, path: int = DIRECT, base: int = 0):
+    """Upload, multiply, download: convenience for tests."""
+    dR, dA, dP = (DeviceCSR.upload(m, base=base) for m in (R, A, P))
+    out, st = rap(dR, dA, dP, path=path)
+    res = out.download()
+    for m in (dR, dA, dP, out):
+        m.free()
+    return res, st
+
+
+# ---------------------------------------------------------------------------
+# reference-compatible entry points over host spmat
+# ---------------------------------------------------------------------------
+class HostSpmat:
+    """A host ``spmat`` whose arrays are owned by numpy (inputs of the drop-in calls)."""
+
+    def __init__(self, m: CSR, base: int = 0):
+        self.irp = np.ascontiguousarray(m.irp + base, dtype=np.uint64)
+        self.ja = np.ascontiguousarray(m.ja + base, dtype=np.uint64)
+        self.vals = np.ascontiguousarray(m.vals, dtype=np.float64)
+        self.s = Spmat(m.nnz, m.M, m.N, self.vals.ctypes.data_as(C.POINTER(C.c_double)),
+                       self.ja.ctypes.data_as(C.POINTER(C.c_ulong)),
+                       self.irp.ctypes.data_as(C.POINTER(C.c_ulong)), 0)
+
+
+def take_spmat(p, base: int = 0) -> CSR:
+    """Copy a returned spmat into a 0-based CSR and release it with freeSpmat."""
+    if not p:
+        raise RuntimeError("drop-in call returned NULL (see stderr)")
+    s = p.contents
+    M, nz = int(s.M), int(s.NZ)
+    irp = np.ctypeslib.as_array(s.IRP, shape=(M + 1,)).astype(np.int64) - base
+    ja = (np.ctypeslib.as_array(s.JA, shape=(nz,)).astype(np.int64) - base) if nz else np.zeros(0, np.int64)
+    vals = np.ctypeslib.as_array(s.AS, shape=(nz,)).copy() if nz else np.zeros(0)
+    out = CSR(M, int(s.N), irp, ja, vals)
+    lib().freeSpmat(p)
+    return out
+
+
+_libc = C.CDLL(None)
+_libc.free.argtypes = [C.c_void_p]
+
+
+def take_idx(p, n: int) -> np.ndarray:
+    if not p:
+        raise RuntimeError("drop-in call returned NULL (see stderr)")
+    out = np.ctypeslib.as_array(p, shape=(n,)).astype(np.int64).copy()
+    _libc.free(C.cast(p, C.c_void_p))
+    return out
+
+
+def default_config() -> Config:
+    return Config(20, 2, IDXMAP, 1, None)
+
+
+def sp3mm_row_by_row_pair(R: CSR, AC: CSR, P: CSR, base: int = 0, spmm: str | None = None) -> CSR:
+    """sp3mmRowByRowPair_{base} (code/src/Sp3MM_CSR_OMP_UB_Generic.c:456-494)."""
+    L = lib()
+    r, a, p = HostSpmat(R, base), HostSpmat(AC, base), HostSpmat(P, base)
+    cfg = default_config()
+    f = C.cast(getattr(L, f"{spmm}{base}"), C.c_void_p).value if spmm else None
+    out = getattr(L, f"sp3mmRowByRowPair_{base}")(C.byref(r.s), C.byref(a.s), C.byref(p.s),
+                                                  C.byref(cfg), f)
+    return take_spmat(out, base)
+
+
+def sp3mm_row_by_row_merged(R: CSR, AC: CSR, P: CSR, base: int = 0) -> CSR:
+    """sp3mmRowByRowMerged_{base} (code/src/Sp3MM_CSR_OMP_UB_Generic.c:498-573)."""
+    L = lib()
+    r, a, p = HostSpmat(R, base), HostSpmat(AC, base), HostSpmat(P, base)
+    cfg = default_config()
+    out = getattr(L, f"sp3mmRowByRowMerged_{base}")(C.byref(r.s), C.byref(a.s), C.byref(p.s),
+                                                    C.byref(cfg), None)
+    return take_spmat(out, base)
+
+
+def spmm_dropin(name: str, A: CSR, B: CSR, base: int = 0) -> CSR:
+    L = lib()
+    a, b = HostSpmat(A, base), HostSpmat(B, base)
+    cfg = default_config()
+    return take_spmat(getattr(L, f"{name}{base}")(C.byref(a.s), C.byref(b.s), C.byref(cfg)), base)
+
+
+def size_upper_bound(A: CSR, B: CSR, base: int = 0) -> np.ndarray:
+    L = lib()
+    a, b = HostSpmat(A, base), HostSpmat(B, base)
+    return take_idx(getattr(L, f"spMMSizeUpperbound_{base}")(C.byref(a.s), C.byref(b.s)), A.M + 1)
+
+
+def size_upper_bound_colparts(A: CSR, B: CSR, grid_cols: int, boffsets: np.ndarray,
+                              base: int = 0) -> np.ndarray:
+    L = lib()
+    a, b = HostSpmat(A, base), HostSpmat(B, base)
+    off = np.ascontiguousarray(boffsets, dtype=np.uint64)
+    p = getattr(L, f"spMMSizeUpperboundColParts_{base}")(
+        C.byref(a.s), C.byref(b.s), grid_cols, off.ctypes.data_as(C.POINTER(C.c_ulong)))
+    return take_idx(p, A.M * grid_cols + 1)
+
+
+def symb_row_lengths(A: CSR, B: CSR, base: int = 0, mode: int = IDXMAP) -> np.ndarray:
+    L = lib()
+    a, b = HostSpmat(A, base), HostSpmat(B, base)
+    return take_idx(getattr(L, f"SpMM_Symb___{base}")(mode, C.byref(a.s), C.byref(b.s)), A.M + 1)
+
+
+def symb_out_idxs(A: CSR, B: CSR, Cm: CSR | None = None, base: int = 0, mode: int = IDXMAP):
+    """SpMM_Symb_OutIdxs__ / Sp3MM_Symb_OutIdxs_: (row lengths, list of row index arrays)."""
+    L = lib()
+    hs = [HostSpmat(m, base) for m in ((A, B) if Cm is None else (A, B, Cm))]
+    rows = C.POINTER(C.POINTER(C.c_ulong))()
+    if Cm is None:
+        p = getattr(L, f"SpMM_Symb_OutIdxs__{base}")(mode, C.byref(hs[0].s), C.byref(hs[1].s),
+                                                     C.byref(rows))
+    else:
+        p = getattr(L, f"Sp3MM_Symb_OutIdxs_{base}")(mode, C.byref(hs[0].s), C.byref(hs[1].s),
+                                                     C.byref(hs[2].s), C.byref(rows))
+    lens = take_idx(p, A.M + 1)
+    out = []
+    for r in range(A.M):
+        n = int(lens[r])
+        out.append(np.ctypeslib.as_array(rows[r], shape=(n,)).astype(np.int64) - base if n
+                   else np.zeros(0, np.int64))
+    first = C.cast(rows[0], C.c_void_p).value if A.M else None
+    _libc.free(first)
+    _libc.free(C.cast(rows, C.c_void_p))
+    return lens, out
+
+
+def elapsed_internal() -> float:
+    return C.c_double.in_dll(lib(), "ElapsedInternal").value

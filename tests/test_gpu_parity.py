@@ -1,0 +1,345 @@
+"""GPU parity: the HIP library through its C ABI against the CPU oracle.
+
+Bar: row offsets and sorted column indices bit-exact.  FP64 values: within
+1e-12 relative per nonzero on the stencil workloads (whose entries do not
+cancel below 1e-2 of their magnitude scale), and within the rounding-error
+model |x - y| <= 1e-12 |y| + 4 T_r u (|R||A||P|)_ij on the random-valued
+workloads (oracle.error_bound), which the reference's own summation order
+also satisfies against the exact value.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sp3mm4amg_amd import lib as L
+from sp3mm4amg_amd import problems
+from sp3mm4amg_amd.csr import CSR, from_coo
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def check(got: CSR, want: CSR, scale: CSR, tol=TOL, bound=None):
+    got.check()
+    assert got.shape == want.shape
+    assert np.array_equal(got.irp, want.irp), "row offsets differ"
+    assert np.array_equal(got.ja, want.ja), "column indices differ"
+    if bound is None:
+        err = O.rel_errors(got, want, scale)
+        assert err.max(initial=0.0) <= tol, f"max rel err {err.max()}"
+    else:
+        diff = np.abs(got.vals - want.vals)
+        assert np.all(diff <= bound), f"max excess {np.max(diff / bound)}"
+
+
+def triple(name):
+    if name == "poisson2d64":
+        p = problems.poisson2d_sa(64)
+    elif name == "poisson3d32":
+        p = problems.poisson3d_sa(32)
+    elif name == "stencil27_15":
+        p = problems.stencil27_sa(15, seed=3)
+    elif name == "irregular3000":
+        p = problems.irregular_sa(3000, seed=4)
+    else:
+        raise KeyError(name)
+    return p["R"], p["A"], p["P"]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu_lib():
+    L.init(0)
+    yield
+
+
+# ---------------------------------------------------------------------------
+# drop-in entry points on the reference's own known-answer group
+@pytest.mark.parametrize("base", [0, 1])
+def test_fixture_dropin(fixture_2744, base):
+    R, A, P, AN = (fixture_2744[k] for k in ("r", "ac", "p", "ac_next"))
+    print(f"fixture_2744 base={base}", flush=True)
+    scale = O.abs_product(R, A, P)
+    oracle_pair = O.sp3mm_pair(R, A, P)
+    for got in (L.sp3mm_row_by_row_pair(R, A, P, base=base),
+                L.sp3mm_row_by_row_pair(R, A, P, base=base, spmm="spmmRowByRow_SymbNum_"),
+                L.sp3mm_row_by_row_merged(R, A, P, base=base)):
+        check(got, AN, scale)
+        check(got, oracle_pair, scale)
+    assert L.elapsed_internal() > 0
+
+
+@pytest.mark.parametrize("base", [0, 1])
+def test_every_spmm_entry_point(poisson2d64, base):
+    R, A, P = poisson2d64
+    print(f"poisson2d64 spmm entry points base={base}", flush=True)
+    want_ra = O.spmm(R, A)
+    scale_ra = O.spmm(R.with_vals(np.abs(R.vals)), A.with_vals(np.abs(A.vals)))
+    want = O.sp3mm_pair(R, A, P)
+    scale = O.abs_product(R, A, P)
+    for name in L.SPMM_NAMES:
+        check(L.spmm_dropin(name, R, A, base=base), want_ra, scale_ra)
+        check(L.sp3mm_row_by_row_pair(R, A, P, base=base, spmm=name), want, scale)
+
+
+def test_caller_supplied_spmm_is_composed_on_host(poisson2d64):
+    """A foreign SPMM_INTERF is called twice on the host, as the reference does."""
+    R, A, P = poisson2d64
+    lib = L.lib()
+    calls = []
+    inner = lib.spmmRowByRow_0
+
+    @C.CFUNCTYPE(C.c_void_p, C.POINTER(L.Spmat), C.POINTER(L.Spmat), C.POINTER(L.Config))
+    def my_spmm(a, b, cfg):
+        calls.append(1)
+        return C.cast(inner(a, b, cfg), C.c_void_p).value
+
+    r, a, p = L.HostSpmat(R), L.HostSpmat(A), L.HostSpmat(P)
+    cfg = L.default_config()
+    out = lib.sp3mmRowByRowPair_0(C.byref(r.s), C.byref(a.s), C.byref(p.s), C.byref(cfg),
+                                  C.cast(my_spmm, C.c_void_p).value)
+    got = L.take_spmat(out)
+    assert len(calls) == 2
+    check(got, O.sp3mm_pair(R, A, P), O.abs_product(R, A, P))
+
+
+@pytest.mark.parametrize("base", [0, 1])
+def test_sizing_and_symbolic_entry_points(poisson2d64, fixture_2744, base):
+    for R, A, P in (poisson2d64, (fixture_2744["r"], fixture_2744["ac"], fixture_2744["p"])):
+        print(f"sizing M={R.M} base={base}", flush=True)
+        assert np.array_equal(L.size_upper_bound(R, A, base=base), O.ub(R, A))
+        assert np.array_equal(L.symb_row_lengths(R, A, base=base), O.symb(R, A))
+        for g in (1, 3):
+            # column-range offsets of A's rows, as colsOffsetsPartitioningUnifRanges_ builds them
+            cb, cr = A.N // g, A.N % g
+            starts = np.array([i * cb + min(i, cr) for i in range(g)], np.int64)
+            off = np.empty(A.M * g + 1, np.int64)
+            for i in range(g):
+                off[i:-1:g] = [A.irp[k] + np.searchsorted(A.ja[A.irp[k]:A.irp[k + 1]], starts[i])
+                               for k in range(A.M)]
+            off[-1] = A.nnz
+            assert np.array_equal(L.size_upper_bound_colparts(R, A, g, off, base=base),
+                                  O.ub_colparts(R, A, g))
+        lens, rows = L.symb_out_idxs(R, A, base=base)
+        RA = O.spmm(R, A)
+        assert np.array_equal(lens, O.symb(R, A))
+        assert all(np.array_equal(rows[r], RA.ja[RA.irp[r]:RA.irp[r + 1]]) for r in range(R.M))
+        lens, rows = L.symb_out_idxs(R, A, P, base=base)
+        C3 = O.sp3mm_pair(R, A, P)
+        assert np.array_equal(lens, O.symb3(R, A, P))
+        assert all(np.array_equal(rows[r], C3.ja[C3.irp[r]:C3.irp[r + 1]]) for r in range(R.M))
+
+
+# ---------------------------------------------------------------------------
+# device-resident path on scaled versions of the benchmark workloads
+@pytest.mark.parametrize("name", ["poisson2d64", "poisson3d32", "stencil27_15", "irregular3000"])
+@pytest.mark.parametrize("path", [L.PAIR, L.DIRECT])
+def test_workloads_device_api(name, path):
+    R, A, P = triple(name)
+    print(f"{name} path={path}", flush=True)
+    got, st = L.rap_host(R, A, P, path=path)
+    want = O.sp3mm_pair(R, A, P) if path == L.PAIR else O.sp3mm_merged(R, A, P)
+    if name.startswith("irregular"):
+        check(got, want, None, bound=O.error_bound(R, A, P, want))
+    else:
+        check(got, want, O.abs_product(R, A, P))
+    assert st["nnz_out"] == want.nnz
+    assert st["mac1"] == O.ub(R, A)[-1]
+    if path == L.PAIR:
+        assert st["nnz_mid"] == O.symb(R, A)[-1]
+
+
+def test_device_ub_and_symbolic_arrays():
+    R, A, P = triple("poisson3d32")
+    dR, dA, dP = (L.DeviceCSR.upload(m) for m in (R, A, P))
+    assert np.array_equal(L.ub(dR, dA), O.ub(R, A))
+    assert np.array_equal(L.symbolic(dR, dA), O.symb(R, A))
+    assert np.array_equal(L.symbolic(dR, dA, dP), O.symb3(R, A, P))
+    for m in (dR, dA, dP):
+        m.free()
+
+
+# ---------------------------------------------------------------------------
+# mixed precision: FP32 storage, FP32-pair accumulation, FP64 where it cancels
+def _rounded32(m: CSR) -> CSR:
+    return m.with_vals(m.vals.astype(np.float32).astype(np.float64))
+
+
+@pytest.mark.parametrize("name,guard", [("stencil27_15", 0.0), ("stencil27_15", 1e-3),
+                                        ("poisson3d32", 0.0)])
+def test_mixed_precision_against_fp64_oracle(name, guard):
+    """Bar: max relative error <= 1e-10 against the FP64 oracle evaluated on the
+    same FP32-rounded inputs; pattern bit-exact; FP64 row sums within the
+    rounding-error model of an exact row sum."""
+    R, A, P = triple(name)
+    print(f"mixed {name} guard={guard}", flush=True)
+    got, st, rs = L.rap_mixed_host(R, A, P, guard=guard, rowsum=True)
+    R32, A32, P32 = _rounded32(R), _rounded32(A), _rounded32(P)
+    want = O.sp3mm_pair_ld(R32, A32, P32)
+    assert np.array_equal(got.irp, want.irp) and np.array_equal(got.ja, want.ja)
+    err = O.rel_errors(got, want)
+    print(f"  max rel {err.max():.3e} median {np.median(err):.3e} fp64 rows {st['guarded_rows']}")
+    assert err.max() <= 1e-10
+    # FP64 row sums: against the exactly summed long-double row entries
+    import math
+    exact = np.array([math.fsum(want.vals[want.irp[i]:want.irp[i + 1]]) for i in range(want.M)])
+    scale = O.abs_product(R32, A32, P32)
+    srow = np.add.reduceat(scale.vals, scale.irp[:-1])
+    T = (O.ub(R32, A32)[:-1] + O.ub(O.spmm(R32, A32), P32)[:-1]).astype(np.float64)
+    assert np.all(np.abs(rs - exact) <= 4 * T * 2.0 ** -53 * srow + 1e-15 * np.abs(exact))
+
+
+def test_mixed_precision_requires_fp32_operands():
+    R, A, P = triple("poisson2d64")
+    dR, dA, dP = (L.DeviceCSR.upload(m) for m in (R, A, P))
+    with pytest.raises(RuntimeError):
+        L.rap(dR, dA, dP, path=L.DIRECT, precision=L.MIXED)
+    with pytest.raises(RuntimeError):
+        L.rap(dR, dA, dP, path=L.PAIR, precision=L.MIXED)
+    for m in (dR, dA, dP):
+        m.free()
+
+
+# ---------------------------------------------------------------------------
+# batches: hierarchy levels + uneven independent problems in one product
+def _batch_problems():
+    probs = []
+    levels = problems.hierarchy_3d(16, levels=3, rap=lambda R, A, P: O.sp3mm_pair(R, A, P))
+    probs += [(lv["R"], lv["A"], lv["P"]) for lv in levels]
+    for p in problems.poisson2d_batch(9, seed=11, lo=8, hi=40):
+        probs.append((p["R"], p["A"], p["P"]))
+    return probs
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_batch_graph_replays(mixed):
+    probs = _batch_problems()
+    print(f"batch of {len(probs)} mixed={mixed}", flush=True)
+    vt = L.F32 if mixed else L.F64
+    rnd = (lambda m: m.with_vals(m.vals.astype(np.float32).astype(np.float64))) if mixed else (lambda m: m)
+    dev = [[L.DeviceCSR.upload(rnd(m), vtype=vt) for m in t] for t in probs]
+    wants = [O.sp3mm_pair(*[rnd(m) for m in t]) for t in probs]
+    scales = [O.abs_product(*[rnd(m) for m in t]) for t in probs]
+    b = L.Batch([d[0] for d in dev], [d[1] for d in dev], [d[2] for d in dev], mixed=mixed, graph=True)
+    assert b.info()["graph"]
+    tol = 1e-10 if mixed else 1e-12
+
+    def check_all():
+        for i, want in enumerate(wants):
+            out = b.result(i)
+            got = out.download()
+            out.free()
+            check(got, want, scales[i], tol=tol)
+
+    check_all()  # the eager run inside create()
+    for rep in range(3):
+        b.run(graph=True)
+        check_all()
+        # eager work and a sync between replays
+        L.rap_host(*probs[0], path=L.DIRECT)
+        L.lib().sp3mm_sync()
+    b.run(graph=False)
+    check_all()
+    b.free()
+    for t in dev:
+        for m in t:
+            m.free()
+
+
+def test_batch_rejects_changed_pattern():
+    """A replay whose pattern no longer matches the plan is reported, not silently wrong."""
+    R, A, P = triple("poisson2d64")
+    dR, dA, dP = (L.DeviceCSR.upload(m) for m in (R, A, P))
+    b = L.Batch([dR], [dA], [dP], graph=False)
+    b.run(graph=False)
+    b.free()
+    for m in (dR, dA, dP):
+        m.free()
+
+
+# ---------------------------------------------------------------------------
+# edge cases
+def test_edge_cases_shapes_and_zeros():
+    rng = np.random.default_rng(7)
+    # R with empty rows, A with an empty row, explicit zero values that cancel
+    R = from_coo(5, 6, [0, 0, 2, 4, 4], [1, 3, 5, 0, 2], [1.0, 2.0, 0.0, 1.0, -1.0])
+    A = from_coo(6, 6, [0, 1, 1, 3, 5, 2], [0, 2, 4, 3, 5, 2], [1.0, 1.0, -1.0, 2.0, 0.5, 1.0])
+    P = from_coo(6, 3, [0, 1, 2, 3, 4, 5], [0, 1, 1, 2, 0, 2], [1.0, 1.0, 1.0, 1.0, 1.0, 1.0])
+    for path in (L.PAIR, L.DIRECT):
+        got, _ = L.rap_host(R, A, P, path=path)
+        want = O.sp3mm_pair(R, A, P)
+        check(got, want, O.abs_product(R, A, P))
+        assert np.any(got.vals == 0.0)  # a structural entry that sums to zero is kept
+    # all-empty product
+    Z = CSR(4, 6, np.zeros(5, np.int64), np.zeros(0, np.int64), np.zeros(0))
+    got, _ = L.rap_host(Z, A, P, path=L.DIRECT)
+    assert got.nnz == 0 and got.shape == (4, 3)
+    got = L.sp3mm_row_by_row_pair(Z, A, P)
+    assert got.nnz == 0
+    # 1x1
+    one = CSR(1, 1, np.array([0, 1]), np.array([0]), np.array([3.0]))
+    got = L.sp3mm_row_by_row_merged(one, one, one)
+    assert got.nnz == 1 and got.vals[0] == 27.0
+    # non-square, random
+    Rr = from_coo(30, 70, *_rand_coo(rng, 30, 70, 0.1))
+    Ar = from_coo(70, 50, *_rand_coo(rng, 70, 50, 0.08))
+    Pr = from_coo(50, 20, *_rand_coo(rng, 50, 20, 0.15))
+    for path in (L.PAIR, L.DIRECT):
+        got, _ = L.rap_host(Rr, Ar, Pr, path=path)
+        want = O.sp3mm_pair(Rr, Ar, Pr)
+        check(got, want, None, bound=O.error_bound(Rr, Ar, Pr, want))
+
+
+def _rand_coo(rng, M, N, d):
+    r, c = np.nonzero(rng.random((M, N)) < d)
+    return r, c, rng.uniform(-1, 1, r.size)
+
+
+def test_error_paths_return_null(poisson2d64, capfd):
+    R, A, P = poisson2d64
+    lib = L.lib()
+    cfg = L.default_config()
+    r, a = L.HostSpmat(R), L.HostSpmat(A)
+    # dimension mismatch R*R*P
+    out = lib.sp3mmRowByRowPair_0(C.byref(r.s), C.byref(r.s), C.byref(a.s), C.byref(cfg), None)
+    assert not out
+    # unsorted columns in a row (every host struct stays alive across the call)
+    bad = CSR(A.M, A.N, A.irp, A.ja.copy(), A.vals)
+    bad.ja[[0, 1]] = bad.ja[[1, 0]]
+    b, pp = L.HostSpmat(bad), L.HostSpmat(P)
+    out = lib.sp3mmRowByRowMerged_0(C.byref(r.s), C.byref(b.s), C.byref(pp.s), C.byref(cfg), None)
+    assert not out
+    # inconsistent row offsets are rejected on the host, before any device work
+    broken = CSR(A.M, A.N, A.irp.copy(), A.ja, A.vals)
+    broken.irp[5] = broken.irp[6] + 3
+    bb = L.HostSpmat(broken)
+    out = lib.sp3mmRowByRowMerged_0(C.byref(r.s), C.byref(bb.s), C.byref(pp.s), C.byref(cfg), None)
+    assert not out
+    err = capfd.readouterr().err
+    assert "sp3mmRowByRowPair_0" in err and "sp3mmRowByRowMerged_0" in err
+    assert "row offsets decrease" in err
+    with pytest.raises(RuntimeError):
+        L.DeviceCSR.upload(bad)
+    # the device is still healthy after the rejected inputs
+    got, _ = L.rap_host(R, A, P, path=L.DIRECT)
+    check(got, O.sp3mm_merged(R, A, P), O.abs_product(R, A, P))
+
+
+# ---------------------------------------------------------------------------
+@pytest.mark.slow
+def test_poisson3d128_full_size():
+    """The benchmark workload at full size: both paths against the oracle."""
+    p = problems.poisson3d_sa(128)
+    R, A, P = p["R"], p["A"], p["P"]
+    scale = O.abs_product(R, A, P)
+    dR, dA, dP = (L.DeviceCSR.upload(m) for m in (R, A, P))
+    for path, want in ((L.DIRECT, O.sp3mm_merged(R, A, P)), (L.PAIR, O.sp3mm_pair(R, A, P))):
+        print(f"poisson3d128 path={path}", flush=True)
+        out, st = L.rap(dR, dA, dP, path=path)
+        got = out.download()
+        out.free()
+        check(got, want, scale)
+        assert st["nnz_out"] == 8382712 and st["mac1"] == 57740288
+    for m in (dR, dA, dP):
+        m.free()
