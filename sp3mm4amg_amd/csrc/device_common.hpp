@@ -179,4 +179,28 @@ __device__ __forceinline__ int key_rank(const int *keys, int n, int k)
     return r;
 }
 
+// exclusive scan of one value per thread over the workgroup; `sh` holds one
+// long long per wavefront; *total receives the workgroup sum
+__device__ __forceinline__ long long block_exclusive_scan(long long v, long long *sh, long long &total)
+{
+    // wave inclusive scan
+    const int l = lane();
+    long long x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+        long long y = __shfl_up(x, o);
+        if (l >= o) x += y;
+    }
+    const int w = threadIdx.x / 64;
+    if (l == 63) sh[w] = x;
+    __syncthreads();
+    long long before = 0;
+    total = 0;
+    for (int i = 0; i < (int)(blockDim.x / 64); i++) {
+        before += (i < w) ? sh[i] : 0;
+        total += sh[i];
+    }
+    __syncthreads();
+    return before + x - v;
+}
+
 }  // namespace sp3mm

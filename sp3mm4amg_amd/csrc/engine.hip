@@ -432,8 +432,9 @@ int spgemm_impl(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &C, sp3mm_s
 
     const int nlists = (thr.nb > thr2.nb ? thr.nb : thr2.nb) + 1;
     Buf ctl(sizeof(Ctl)), ub((size_t)(M + 1) * 8), rown((size_t)(M + 1) * 4),
-        lists((size_t)nlists * (M ? M : 1) * 4), tiles((size_t)scan_tiles(M) * 8);
-    if (!ctl.p || !ub.p || !rown.p || !lists.p || !tiles.p) {
+        lists((size_t)nlists * (M ? M : 1) * 4), tiles((size_t)scan_tiles(M) * 8),
+        binscr((size_t)bin_scratch_ints(M) * 4);
+    if (!ctl.p || !ub.p || !rown.p || !lists.p || !tiles.p || !binscr.p) {
         set_error("scratch allocation failed");
         return -1;
     }
@@ -441,7 +442,7 @@ int spgemm_impl(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &C, sp3mm_s
     Ctl h;
     SP3MM_HIP(hipMemsetAsync(d, 0, sizeof(Ctl), s));
     launch_ub_bin(a, b, ub.as<int64_t>(), rown.as<int32_t>(), lists.as<int32_t>(), d->counts, thr,
-                  d->totals, s);
+                  d->totals, binscr.as<int>(), s);
     if (read_ctl(c, d, h)) return -1;
     const long long mac = (long long)h.totals[0];
 
@@ -467,7 +468,7 @@ int spgemm_impl(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &C, sp3mm_s
     if (dmat_alloc(C, M, B.N, 0, SP3MM_F64)) return -1;
     // C.rp32 was sized for nnz 0; it is re-checked once the total is known
     launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d->nnz, s);
-    launch_rebin(rown.as<int32_t>(), nullptr, M, lists.as<int32_t>(), d->counts2, thr2, s);
+    launch_rebin(rown.as<int32_t>(), nullptr, M, lists.as<int32_t>(), d->counts2, thr2, binscr.as<int>(), s);
     if (read_ctl(c, d, h)) return -1;
     if (h.err) {
         set_error("symbolic phase: %s", err_text(h.err));
@@ -623,8 +624,8 @@ static int rap_direct_twopass(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp
     const int nlists = (thr.nb > thr2.nb ? thr.nb : thr2.nb) + 1;
     Buf ctl(sizeof(Ctl)), ub((size_t)(M + 1) * 8), rown((size_t)(M + 1) * 4),
         n1((size_t)(M + 1) * 4), lists((size_t)nlists * (M ? M : 1) * 4),
-        retry((size_t)(M ? M : 1) * 4), tiles((size_t)scan_tiles(M) * 8);
-    if (!ctl.p || !ub.p || !rown.p || !n1.p || !lists.p || !retry.p || !tiles.p) {
+        retry((size_t)(M ? M : 1) * 4), tiles((size_t)scan_tiles(M) * 8), binscr((size_t)bin_scratch_ints(M) * 4);
+    if (!ctl.p || !ub.p || !rown.p || !n1.p || !lists.p || !retry.p || !tiles.p || !binscr.p) {
         set_error("scratch allocation failed");
         return -1;
     }
@@ -633,7 +634,7 @@ static int rap_direct_twopass(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp
     SP3MM_HIP(hipMemsetAsync(d, 0, sizeof(Ctl), s));
     launch_fill_i32(n1.as<int32_t>(), M, 0, s);
     launch_ub_bin(r, a, ub.as<int64_t>(), rown.as<int32_t>(), lists.as<int32_t>(), d->counts, thr,
-                  d->totals, s);
+                  d->totals, binscr.as<int>(), s);
     const int eu = tm.mark();
     if (read_ctl(c, d, h)) return -1;
     const long long mac1 = (long long)h.totals[0];
@@ -658,7 +659,7 @@ static int rap_direct_twopass(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp
     const int es1 = tm.mark();
     if (dmat_alloc(C, M, P.N, 0, SP3MM_F64)) return -1;
     launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d->nnz, s);
-    launch_rebin(n1.as<int32_t>(), rown.as<int32_t>(), M, lists.as<int32_t>(), d->counts2, thr2, s);
+    launch_rebin(n1.as<int32_t>(), rown.as<int32_t>(), M, lists.as<int32_t>(), d->counts2, thr2, binscr.as<int>(), s);
     if (read_ctl(c, d, h)) {
         dmat_release(C);
         return -1;
@@ -675,7 +676,7 @@ static int rap_direct_twopass(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp
                             &d->err, s);
         SP3MM_HIP(hipMemsetAsync(d->counts2, 0, sizeof(d->counts2), s));
         launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d->nnz, s);
-        launch_rebin(n1.as<int32_t>(), rown.as<int32_t>(), M, lists.as<int32_t>(), d->counts2, thr2, s);
+        launch_rebin(n1.as<int32_t>(), rown.as<int32_t>(), M, lists.as<int32_t>(), d->counts2, thr2, binscr.as<int>(), s);
         if (read_ctl(c, d, h)) {
             dmat_release(C);
             return -1;
@@ -797,7 +798,7 @@ struct FusedPlan {
     int64_t nnz = 0, stot = 0;
     long long mac1 = 0;
     int slog1 = 0, slog2 = 0, dlog1 = 0, dlog2 = 0;
-    Buf ctl, ub, rown, lists, retry, slotn, fix, tiles, scol, sval;
+    Buf ctl, ub, rown, lists, retry, slotn, fix, tiles, scol, sval, binscr;
     DenseDirectWs dws, rws;
     sp3mm_dmat C{};
     double ms_ub = 0, ms_num = 0, ms_scan = 0;
@@ -870,7 +871,8 @@ struct FusedPlan {
         slotn = Buf((size_t)(M ? M : 1) * 4);
         fix = Buf((size_t)(M ? M : 1) * 4);
         tiles = Buf((size_t)scan_tiles(M) * 8);
-        if (!ctl.p || !ub.p || !rown.p || !lists.p || !retry.p || !slotn.p || !fix.p || !tiles.p) {
+        binscr = Buf((size_t)bin_scratch_ints(M) * 4);
+        if (!ctl.p || !ub.p || !rown.p || !lists.p || !retry.p || !slotn.p || !fix.p || !tiles.p || !binscr.p) {
             set_error("scratch allocation failed");
             return -1;
         }
@@ -878,7 +880,7 @@ struct FusedPlan {
         const int e0 = tm.mark();
         SP3MM_HIP(hipMemsetAsync(d(), 0, sizeof(Ctl), s));
         launch_ub_bin(r(), a(), ub.as<int64_t>(), rown.as<int32_t>(), lists.as<int32_t>(), d()->counts, thr,
-                      d()->totals, s);
+                      d()->totals, binscr.as<int>(), s);
         const int eu = tm.mark();
         if (read_ctl(c, d(), h)) return -1;
         mac1 = (long long)h.totals[0];
@@ -966,7 +968,7 @@ struct FusedPlan {
         const int64_t M = R.M;
         launch_fill_i32(reinterpret_cast<int32_t *>(d()), (int64_t)(sizeof(Ctl) / 4), 0, s);
         launch_ub_bin(r(), a(), ub.as<int64_t>(), rown.as<int32_t>(), lists.as<int32_t>(), d()->counts, thr,
-                      d()->totals, s);
+                      d()->totals, binscr.as<int>(), s);
         for (int i = 0; i < thr.nb; i++)
             if (cnt[i]) launch_main(c, s, i);
         if (nlong > 0)
@@ -1298,8 +1300,9 @@ int row_ub(const sp3mm_dmat &A, const sp3mm_dmat &B, int64_t *host_ub)
     const int64_t M = A.M;
     BinThresholds thr{};
     thr.nb = 0;
-    Buf ctl(sizeof(Ctl)), ub((size_t)(M + 1) * 8), lists((size_t)(M ? M : 1) * 4);
-    if (!ctl.p || !ub.p || !lists.p) {
+    Buf ctl(sizeof(Ctl)), ub((size_t)(M + 1) * 8), lists((size_t)(M ? M : 1) * 4),
+        binscr((size_t)bin_scratch_ints(M) * 4);
+    if (!ctl.p || !ub.p || !lists.p || !binscr.p) {
         set_error("scratch allocation failed");
         return -1;
     }
@@ -1307,7 +1310,7 @@ int row_ub(const sp3mm_dmat &A, const sp3mm_dmat &B, int64_t *host_ub)
     Ctl h;
     SP3MM_HIP(hipMemsetAsync(d, 0, sizeof(Ctl), c->stream));
     launch_ub_bin(operand(A), operand(B), ub.as<int64_t>(), nullptr, lists.as<int32_t>(), d->counts, thr,
-                  d->totals, c->stream);
+                  d->totals, binscr.as<int>(), c->stream);
     if (M) SP3MM_HIP(hipMemcpyAsync(host_ub, ub.p, (size_t)M * 8, hipMemcpyDeviceToHost, c->stream));
     if (read_ctl(c, d, h)) return -1;
     host_ub[M] = (int64_t)h.totals[0];

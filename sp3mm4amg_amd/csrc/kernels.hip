@@ -28,15 +28,17 @@ namespace sp3mm {
 // ---------------------------------------------------------------------------
 // upper bounds + binning
 // ---------------------------------------------------------------------------
-// Binning.  A workgroup of 256 threads owns kBinRows consecutive rows (8 per
-// thread, strided so that neighbouring threads take neighbouring rows).  Bin
-// populations are first counted with LDS atomics, then each workgroup books
-// space in every bin it uses with ONE global atomic per bin, so no global
-// counter sees more than (rows / kBinRows) atomics.  Rows of one workgroup
-// stay together inside a bin (their relative order is not kept).
+// Binning in three short kernels with no contended global atomics:
+//   k_bin_hist    one thread per row: the row's measure (its upper bound
+//                 ub[r] = sum_{k in A_r} nnz(B_k), code/src/include/SpMMUtilsGeneric.h:112-139,
+//                 or a given exact length), its bin, a per-workgroup histogram
+//                 (LDS atomics) stored bin-major, and per-workgroup ub sum/max;
+//   k_bin_offsets one workgroup: exclusive scan of every bin's histogram over
+//                 the workgroups -> each workgroup's base inside the bin, bin
+//                 populations, ub totals;
+//   k_bin_scatter one thread per row: list[bin][base + rank] = row.
+// Rows with a zero measure get rownnz = 0 and no bin.
 constexpr int kBinThreads = 256;
-constexpr int kBinPer = 8;
-constexpr int kBinRows = kBinThreads * kBinPer;
 
 __device__ __forceinline__ int bin_of(long long u, const BinThresholds &thr)
 {
@@ -46,54 +48,51 @@ __device__ __forceinline__ int bin_of(long long u, const BinThresholds &thr)
     return thr.nb;
 }
 
-// scatter the block's binned rows; `b` and `pos` hold each thread's kBinPer entries
-__device__ __forceinline__ void bin_scatter(const int *b, const int *pos, int64_t r0, int64_t M,
-                                            int32_t *lists, int *counts, int nbins, int *hist, int *base)
+__device__ __forceinline__ long long row_measure(int64_t r, const int64_t *__restrict__ arp,
+                                                 const int32_t *__restrict__ acol, const int32_t *__restrict__ brp,
+                                                 const int32_t *__restrict__ meas1, const int32_t *__restrict__ meas2)
 {
-    __syncthreads();
-    if ((int)threadIdx.x < nbins) base[threadIdx.x] = hist[threadIdx.x] ? atomicAdd(counts + threadIdx.x, hist[threadIdx.x]) : 0;
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kBinPer; j++) {
-        const int64_t r = r0 + (int64_t)j * kBinThreads + threadIdx.x;
-        if (b[j] >= 0) lists[(int64_t)b[j] * M + base[b[j]] + pos[j]] = (int32_t)r;
+    if (meas1) {
+        int u = meas1[r];
+        if (meas2) u = max(u, meas2[r]);
+        return u;
     }
+    const int64_t a0 = arp[r], a1 = arp[r + 1];
+    long long u = 0;
+    int64_t e = a0;
+    for (; e + 4 <= a1; e += 4) {  // four independent gathers in flight
+        const int k0 = acol[e], k1 = acol[e + 1], k2 = acol[e + 2], k3 = acol[e + 3];
+        u += (brp[k0 + 1] - brp[k0]) + (brp[k1 + 1] - brp[k1]) + (brp[k2 + 1] - brp[k2]) + (brp[k3 + 1] - brp[k3]);
+    }
+    for (; e < a1; e++) {
+        const int k = acol[e];
+        u += brp[k + 1] - brp[k];
+    }
+    return u;
 }
 
-// ub[r] = sum_{k in A_r} nnz(B_k) (code/src/include/SpMMUtilsGeneric.h:112-139);
-// rows binned by ub (rows with ub == 0 get rownnz = 0 and no bin).
-__global__ void __launch_bounds__(kBinThreads) k_ub_bin(const int64_t *__restrict__ arp, const int32_t *__restrict__ acol,
-                         const int32_t *__restrict__ brp, int64_t M, int64_t *__restrict__ ub,
-                         int32_t *__restrict__ rownnz, int32_t *__restrict__ lists,
-                         int *__restrict__ counts, BinThresholds thr,
-                         unsigned long long *__restrict__ totals)
+__global__ void __launch_bounds__(kBinThreads) k_bin_hist(const int64_t *__restrict__ arp, const int32_t *__restrict__ acol,
+                                                           const int32_t *__restrict__ brp, const int32_t *__restrict__ meas1,
+                                                           const int32_t *__restrict__ meas2, int64_t M,
+                                                           int64_t *__restrict__ ub, int32_t *__restrict__ rownnz,
+                                                           int *__restrict__ hist, unsigned long long *__restrict__ part,
+                                                           BinThresholds thr)
 {
-    __shared__ int hist[kMaxBins + 1], base[kMaxBins + 1];
+    __shared__ int h[kMaxBins + 1];
     __shared__ unsigned long long red[2][kBinThreads / 64];
-    if ((int)threadIdx.x <= thr.nb) hist[threadIdx.x] = 0;
+    const int nbin = thr.nb + 1;
+    if ((int)threadIdx.x < nbin) h[threadIdx.x] = 0;
     __syncthreads();
-    const int64_t r0 = (int64_t)blockIdx.x * kBinRows;
-    int b[kBinPer], pos[kBinPer];
-    unsigned long long s = 0, mx = 0;
-#pragma unroll
-    for (int j = 0; j < kBinPer; j++) {
-        const int64_t r = r0 + (int64_t)j * kBinThreads + threadIdx.x;
-        long long u = 0;
-        b[j] = -1;
-        if (r < M) {
-            for (int64_t e = arp[r]; e < arp[r + 1]; e++) {
-                const int k = acol[e];
-                u += brp[k + 1] - brp[k];
-            }
-            ub[r] = u;
-            if (u == 0 && rownnz) rownnz[r] = 0;
-            b[j] = bin_of(u, thr);
-            if (b[j] >= 0) pos[j] = atomicAdd(hist + b[j], 1);
-            s += (unsigned long long)u;
-            mx = (unsigned long long)u > mx ? (unsigned long long)u : mx;
-        }
+    const int64_t r = (int64_t)blockIdx.x * kBinThreads + threadIdx.x;
+    long long u = 0;
+    if (r < M) {
+        u = row_measure(r, arp, acol, brp, meas1, meas2);
+        if (ub) ub[r] = u;
+        if (u <= 0 && rownnz) rownnz[r] = 0;
+        const int b = bin_of(u, thr);
+        if (b >= 0) atomicAdd(h + b, 1);
     }
-    bin_scatter(b, pos, r0, M, lists, counts, thr.nb + 1, hist, base);
+    unsigned long long s = (unsigned long long)(u > 0 ? u : 0), mx = s;
     for (int o = 32; o > 0; o >>= 1) {
         s += __shfl_xor(s, o);
         const unsigned long long t = __shfl_xor(mx, o);
@@ -104,40 +103,91 @@ __global__ void __launch_bounds__(kBinThreads) k_ub_bin(const int64_t *__restric
         red[1][threadIdx.x / 64] = mx;
     }
     __syncthreads();
+    if ((int)threadIdx.x < nbin) hist[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
     if (threadIdx.x == 0) {
         unsigned long long ts = 0, tm = 0;
         for (int i = 0; i < kBinThreads / 64; i++) {
             ts += red[0][i];
             tm = red[1][i] > tm ? red[1][i] : tm;
         }
-        if (ts) atomicAdd(totals, ts);
-        atomicMax(totals + 1, tm);
+        part[2 * (int64_t)blockIdx.x] = ts;
+        part[2 * (int64_t)blockIdx.x + 1] = tm;
     }
 }
 
-// Re-bin rows by an exact measure (row length, or max of two lengths); the
-// largest meas2 of every bin is recorded in maxes[] (when given).
-__global__ void __launch_bounds__(kBinThreads) k_rebin(const int32_t *__restrict__ meas1, const int32_t *__restrict__ meas2,
-                        int64_t M, int32_t *__restrict__ lists, int *__restrict__ counts,
-                        BinThresholds thr)
+// one workgroup of 1024 threads
+__global__ void __launch_bounds__(1024) k_bin_offsets(int *__restrict__ hist, int nblk, int nbin,
+                                                       const unsigned long long *__restrict__ part,
+                                                       int *__restrict__ counts,
+                                                       unsigned long long *__restrict__ totals)
 {
-    __shared__ int hist[kMaxBins + 1], base[kMaxBins + 1];
-    if ((int)threadIdx.x <= thr.nb) hist[threadIdx.x] = 0;
-    __syncthreads();
-    const int64_t r0 = (int64_t)blockIdx.x * kBinRows;
-    int b[kBinPer], pos[kBinPer];
-#pragma unroll
-    for (int j = 0; j < kBinPer; j++) {
-        const int64_t r = r0 + (int64_t)j * kBinThreads + threadIdx.x;
-        b[j] = -1;
-        if (r < M) {
-            int u = meas1[r];
-            if (meas2) u = max(u, meas2[r]);
-            b[j] = bin_of(u, thr);
-            if (b[j] >= 0) pos[j] = atomicAdd(hist + b[j], 1);
+    __shared__ long long sh[1024 / 64];
+    for (int b = 0; b < nbin; b++) {
+        int *hb = hist + (int64_t)b * nblk;
+        long long carry = 0;
+        for (int i0 = 0; i0 < nblk; i0 += 1024) {
+            const int i = i0 + threadIdx.x;
+            const long long v = i < nblk ? hb[i] : 0;
+            long long total;
+            const long long ex = block_exclusive_scan(v, sh, total);
+            if (i < nblk) hb[i] = (int)(ex + carry);
+            carry += total;
+        }
+        if (threadIdx.x == 0) counts[b] += (int)carry;
+        __syncthreads();
+    }
+    if (totals) {
+        unsigned long long s = 0, m = 0;
+        for (int i = threadIdx.x; i < nblk; i += 1024) {
+            s += part[2 * (int64_t)i];
+            m = part[2 * (int64_t)i + 1] > m ? part[2 * (int64_t)i + 1] : m;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            s += __shfl_xor(s, o);
+            const unsigned long long t = __shfl_xor(m, o);
+            m = t > m ? t : m;
+        }
+        __shared__ unsigned long long rs[16], rm[16];
+        if (lane() == 0) {
+            rs[threadIdx.x / 64] = s;
+            rm[threadIdx.x / 64] = m;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long ts = 0, tm = 0;
+            for (int w = 0; w < 16; w++) {
+                ts += rs[w];
+                tm = rm[w] > tm ? rm[w] : tm;
+            }
+            totals[0] += ts;
+            totals[1] = tm > totals[1] ? tm : totals[1];
         }
     }
-    bin_scatter(b, pos, r0, M, lists, counts, thr.nb + 1, hist, base);
+}
+
+__global__ void __launch_bounds__(kBinThreads) k_bin_scatter(const int64_t *__restrict__ ub, const int32_t *__restrict__ meas1,
+                                                              const int32_t *__restrict__ meas2, int64_t M,
+                                                              const int *__restrict__ hist, int32_t *__restrict__ lists,
+                                                              BinThresholds thr)
+{
+    __shared__ int h[kMaxBins + 1];
+    const int nbin = thr.nb + 1;
+    if ((int)threadIdx.x < nbin) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t r = (int64_t)blockIdx.x * kBinThreads + threadIdx.x;
+    if (r >= M) return;
+    long long u;
+    if (meas1) {
+        int v = meas1[r];
+        if (meas2) v = max(v, meas2[r]);
+        u = v;
+    } else {
+        u = ub[r];
+    }
+    const int b = bin_of(u, thr);
+    if (b < 0) return;
+    const int rank = atomicAdd(h + b, 1);
+    lists[(int64_t)b * M + hist[(int64_t)b * gridDim.x + blockIdx.x] + rank] = (int32_t)r;
 }
 
 // Upper bound split over column ranges of B: boff[k*G+g] .. boff[k*G+g+1] are
@@ -492,8 +542,8 @@ __global__ void __launch_bounds__(256) k_direct_fused(SpOperand R, SpOperand A, 
     }
 }
 
-// Move the rows of one bin from their scratch slots to the output (one
-// wavefront per row; slots marked -1 belong to the long-row path).
+// Move the rows of one bin from their scratch slots to the output (32 lanes
+// per row, two rows per wavefront; slots marked -1 belong to the long-row path).
 __global__ void __launch_bounds__(256) k_gather_rows(const int32_t *__restrict__ rows, const int *__restrict__ count,
                                                       const int32_t *__restrict__ slotn, int cap,
                                                       const int32_t *__restrict__ scol,
@@ -502,17 +552,18 @@ __global__ void __launch_bounds__(256) k_gather_rows(const int32_t *__restrict__
                                                       double *__restrict__ cval, int64_t out_cap,
                                                       int *__restrict__ err)
 {
-    const int wpb = blockDim.x / 64;
+    const int hpb = blockDim.x / 32;
+    const int hl = threadIdx.x & 31;
     const int n = *count;
-    for (int i = blockIdx.x * wpb + threadIdx.x / 64; i < n; i += gridDim.x * wpb) {
+    for (int i = blockIdx.x * hpb + threadIdx.x / 32; i < n; i += gridDim.x * hpb) {
         const int len = slotn[i];
         if (len <= 0) continue;
         const int64_t src = (int64_t)i * cap, dst = crp[rows[i]];
         if (dst + len > out_cap) {  // output smaller than this replay needs
-            if (lane() == 0) atomicOr(err, kErrOverflow);
+            if (hl == 0) atomicOr(err, kErrOverflow);
             continue;
         }
-        for (int j = lane(); j < len; j += 64) {
+        for (int j = hl; j < len; j += 32) {
             ccol[dst + j] = scol[src + j];
             cval[dst + j] = sval[src + j];
         }
@@ -525,28 +576,6 @@ __global__ void __launch_bounds__(256) k_gather_rows(const int32_t *__restrict__
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 8;
 constexpr int kScanTile = kScanThreads * kScanItems;
-
-__device__ __forceinline__ long long block_exclusive_scan(long long v, long long *sh, long long &total)
-{
-    // wave inclusive scan
-    const int l = lane();
-    long long x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-        long long y = __shfl_up(x, o);
-        if (l >= o) x += y;
-    }
-    const int w = threadIdx.x / 64;
-    if (l == 63) sh[w] = x;
-    __syncthreads();
-    long long before = 0;
-    total = 0;
-    for (int i = 0; i < (int)(blockDim.x / 64); i++) {
-        before += (i < w) ? sh[i] : 0;
-        total += sh[i];
-    }
-    __syncthreads();
-    return before + x - v;
-}
 
 __global__ void __launch_bounds__(kScanThreads) k_scan_tiles(const int32_t *__restrict__ in, int64_t n,
                                                               int64_t *__restrict__ out,
@@ -896,13 +925,28 @@ __global__ void k_copy_probe(const double4 *__restrict__ in, double4 *__restrict
 // ---------------------------------------------------------------------------
 static inline unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
+int64_t bin_scratch_ints(int64_t M) { return (kMaxBins + 1) * ((M + kBinThreads - 1) / kBinThreads + 1) + 4 * ((M + kBinThreads - 1) / kBinThreads + 1); }
+
+static void launch_bins(const SpOperand *A, const SpOperand *B, const int32_t *m1, const int32_t *m2, int64_t M,
+                        int64_t *ub, int32_t *rownnz, int32_t *lists, int *counts, const BinThresholds &thr,
+                        unsigned long long *totals, int *scratch, hipStream_t st)
+{
+    if (M == 0) return;
+    const int nblk = (int)((M + kBinThreads - 1) / kBinThreads);
+    int *hist = scratch;
+    unsigned long long *part = reinterpret_cast<unsigned long long *>(scratch + (int64_t)(kMaxBins + 1) * (nblk + 1) + 1) ;
+    part = reinterpret_cast<unsigned long long *>((reinterpret_cast<uintptr_t>(part) + 7) & ~(uintptr_t)7);
+    hipLaunchKernelGGL(k_bin_hist, dim3(nblk), dim3(kBinThreads), 0, st, A ? A->rp : nullptr, A ? A->col : nullptr,
+                       B ? B->rp32 : nullptr, m1, m2, M, ub, rownnz, hist, part, thr);
+    hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(1024), 0, st, hist, nblk, thr.nb + 1, part, counts, totals);
+    hipLaunchKernelGGL(k_bin_scatter, dim3(nblk), dim3(kBinThreads), 0, st, ub, m1, m2, M, hist, lists, thr);
+}
+
 void launch_ub_bin(const SpOperand &A, const SpOperand &B, int64_t *ub, int32_t *rownnz,
                    int32_t *lists, int *counts, const BinThresholds &thr,
-                   unsigned long long *totals, hipStream_t st)
+                   unsigned long long *totals, int *scratch, hipStream_t st)
 {
-    if (A.M == 0) return;
-    hipLaunchKernelGGL(k_ub_bin, dim3(blocks_for(A.M, kBinRows)), dim3(kBinThreads), 0, st, A.rp, A.col, B.rp32,
-                       A.M, ub, rownnz, lists, counts, thr, totals);
+    launch_bins(&A, &B, nullptr, nullptr, A.M, ub, rownnz, lists, counts, thr, totals, scratch, st);
 }
 
 void launch_ub_colparts(const int64_t *arp, const int32_t *acol, int64_t M, const int64_t *boff, int G,
@@ -914,11 +958,9 @@ void launch_ub_colparts(const int64_t *arp, const int32_t *acol, int64_t M, cons
 }
 
 void launch_rebin(const int32_t *m1, const int32_t *m2, int64_t M, int32_t *lists, int *counts,
-                  const BinThresholds &thr, hipStream_t st)
+                  const BinThresholds &thr, int *scratch, hipStream_t st)
 {
-    if (M == 0) return;
-    hipLaunchKernelGGL(k_rebin, dim3(blocks_for(M, kBinRows)), dim3(kBinThreads), 0, st, m1, m2, M, lists, counts,
-                       thr);
+    launch_bins(nullptr, nullptr, m1, m2, M, nullptr, nullptr, lists, counts, thr, nullptr, scratch, st);
 }
 
 static unsigned grid_for(int rows, int G, int max_blocks)
@@ -1053,7 +1095,9 @@ void launch_gather_rows(const int32_t *rows, const int *count, int rows_host, co
                         const int32_t *scol, const double *sval, const int64_t *crp, int32_t *ccol, double *cval,
                         int64_t out_cap, int *err, int max_blocks, hipStream_t st)
 {
-    const unsigned grid = grid_for(rows_host, 64, max_blocks);
+    unsigned grid = (unsigned)((rows_host + 7) / 8);
+    if (grid > (unsigned)max_blocks) grid = (unsigned)max_blocks;
+    if (grid < 1) grid = 1;
     hipLaunchKernelGGL(k_gather_rows, dim3(grid), dim3(256), 0, st, rows, count, slotn, cap, scol, sval, crp, ccol,
                        cval, out_cap, err);
 }

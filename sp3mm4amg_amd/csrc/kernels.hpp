@@ -24,13 +24,15 @@ struct BinThresholds {
     long long v[kMaxBins];
 };
 
+// binning scratch: bin_scratch_ints(M) ints of device memory
+int64_t bin_scratch_ints(int64_t M);
 void launch_ub_bin(const SpOperand &A, const SpOperand &B, int64_t *ub, int32_t *rownnz,
                    int32_t *lists, int *counts, const BinThresholds &thr,
-                   unsigned long long *totals, hipStream_t st);
+                   unsigned long long *totals, int *scratch, hipStream_t st);
 void launch_ub_colparts(const int64_t *arp, const int32_t *acol, int64_t M, const int64_t *boff, int G,
                         int64_t *out, unsigned long long *total, hipStream_t st);
 void launch_rebin(const int32_t *m1, const int32_t *m2, int64_t M, int32_t *lists, int *counts,
-                  const BinThresholds &thr, hipStream_t st);
+                  const BinThresholds &thr, int *scratch, hipStream_t st);
 
 size_t symb_lds(int G, int H);
 size_t num_lds(int G, int H);
