@@ -164,7 +164,14 @@ void Fortran_C_ShiftIdxs(spmat *m);
 
 enum { SP3MM_F64 = 0, SP3MM_F32 = 1 };
 enum { SP3MM_PATH_PAIR = 0, SP3MM_PATH_DIRECT = 1 };
-enum { SP3MM_PREC_FP64 = 0, SP3MM_PREC_MIXED = 1 };
+/* FP64: FP64 operands, FP64 arithmetic.
+ * MIXED: FP32 operands and multiplies; products formed exactly as FP32 pairs
+ *   (TwoProd) and accumulated as FP32 pairs (compensated, TwoSum); diagonal
+ *   and row-sum entries accumulated in FP64; rows with a cancelling
+ *   off-diagonal entry recomputed in FP64 (see sp3mm_opts.guard).
+ * MIXED64: FP32 operands; every FP32 x FP32 product is exact in FP64 and is
+ *   accumulated in FP64 (FP32 storage, FP64 sums; no guard needed). */
+enum { SP3MM_PREC_FP64 = 0, SP3MM_PREC_MIXED = 1, SP3MM_PREC_MIXED64 = 2 };
 
 /* A sorted CSR matrix in HBM, 0-based.  rp has M+1 int64 offsets; rp32 holds
  * the same offsets as int32 whenever nnz < 2^31 (it is what the gathers read);
@@ -182,8 +189,8 @@ typedef struct {
 
 typedef struct {
     int32_t path;        /* SP3MM_PATH_PAIR or SP3MM_PATH_DIRECT */
-    int32_t precision;   /* SP3MM_PREC_FP64, or SP3MM_PREC_MIXED (direct path, FP32
-                            operands, FP64 output) */
+    int32_t precision;   /* SP3MM_PREC_FP64, or SP3MM_PREC_MIXED / _MIXED64 (direct
+                            path, FP32 operands, FP64 output) */
     double guard;        /* MIXED: a row holding an off-diagonal entry with
                             |value| < guard * (|R||A||P|)_ij is recomputed in FP64;
                             diagonal entries are always summed in FP64.

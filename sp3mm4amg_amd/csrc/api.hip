@@ -192,12 +192,12 @@ int sp3mm_rap(const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P, sp3
     sp3mm_opts o{};
     if (opts) o = *opts;
     memset(out, 0, sizeof *out);
-    if (o.precision == SP3MM_PREC_MIXED) {
+    if (o.precision == SP3MM_PREC_MIXED || o.precision == SP3MM_PREC_MIXED64) {
         if (o.path != SP3MM_PATH_DIRECT) {
             set_error("rap: the mixed-precision product runs on the direct path only");
             return -1;
         }
-        return rap_direct_mixed(*R, *A, *P, *out, stats, o.guard, o.rowsum);
+        return rap_direct_mixed(*R, *A, *P, *out, stats, o.precision, o.guard, o.rowsum);
     }
     if (o.precision != SP3MM_PREC_FP64) {
         set_error("rap: unknown precision %d", o.precision);
@@ -271,7 +271,11 @@ int sp3mm_batch_create(int32_t count, const sp3mm_dmat *R, const sp3mm_dmat *A, 
         set_error("batch: batches run on the direct path");
         return -1;
     }
-    void *b = batch_create(count, R, A, P, o.precision == SP3MM_PREC_MIXED, o.guard, use_graph != 0);
+    if (o.precision != SP3MM_PREC_FP64 && o.precision != SP3MM_PREC_MIXED && o.precision != SP3MM_PREC_MIXED64) {
+        set_error("batch: unknown precision %d", o.precision);
+        return -1;
+    }
+    void *b = batch_create(count, R, A, P, o.precision, o.guard, use_graph != 0);
     *out = static_cast<sp3mm_batch *>(b);
     return b ? 0 : -1;
 }

@@ -1,7 +1,8 @@
 // device_common.hpp -- wavefront/workgroup helpers and LDS hash tables for the
-// gfx950 Sp3MM kernels.  CDNA wavefronts are 64 lanes; a "group" is either one
-// wavefront (G == 64, one output row per wavefront) or a whole workgroup
-// (G > 64, one output row per workgroup).
+// gfx950 Sp3MM kernels.  CDNA wavefronts are 64 lanes; a "group" of G lanes
+// works on one output row: a power-of-two part of a wavefront (G = 16 or 32,
+// several rows per wavefront advancing in lockstep), one wavefront (G == 64)
+// or a whole workgroup (G > 64).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -39,22 +40,36 @@ __device__ __forceinline__ void wave_sync()
 
 template <int G>
 struct Group {
-    static_assert(G % kWave == 0, "group must be whole wavefronts");
-    static constexpr int kWaves = G / kWave;
+    static_assert((G >= 16 && G <= kWave && (G & (G - 1)) == 0) || G % kWave == 0,
+                  "group must be a power-of-two part of a wavefront or whole wavefronts");
+    // wavefronts per group (1 for sub-wavefront groups)
+    static constexpr int kWaves = G >= kWave ? G / kWave : 1;
     __device__ static int tid() { return (int)(threadIdx.x % G); }
+    // first lane of this group inside its wavefront
+    __device__ static int lane0() { return G >= kWave ? 0 : (lane() & ~(G - 1)); }
+    // this group's bits of a wavefront-wide ballot, shifted down to bit 0
+    __device__ static uint64_t part(uint64_t m)
+    {
+        if constexpr (G >= kWave) return m;
+        else return (m >> lane0()) & ((1ull << G) - 1);
+    }
     __device__ static void sync()
     {
-        if constexpr (G == kWave) wave_sync();
+        if constexpr (G <= kWave) wave_sync();
         else __syncthreads();
     }
 };
+
+// wavefronts spanned by a group of G lanes (host and device)
+__host__ __device__ constexpr int group_waves(int G) { return G >= kWave ? G / kWave : 1; }
 
 // sum of an int over the group; `red` is kWaves ints of LDS private to the group
 template <int G>
 __device__ __forceinline__ long long group_sum(long long v, long long *red)
 {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    if constexpr (G == kWave) {
+    constexpr int top = G < kWave ? G / 2 : 32;
+    for (int o = top; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if constexpr (G <= kWave) {
         return v;
     } else {
         const int w = Group<G>::tid() / kWave;
@@ -71,11 +86,12 @@ __device__ __forceinline__ long long group_sum(long long v, long long *red)
 template <int G>
 __device__ __forceinline__ long long group_max(long long v, long long *red)
 {
-    for (int o = 32; o > 0; o >>= 1) {
+    constexpr int top = G < kWave ? G / 2 : 32;
+    for (int o = top; o > 0; o >>= 1) {
         long long u = __shfl_xor(v, o);
         v = u > v ? u : v;
     }
-    if constexpr (G == kWave) {
+    if constexpr (G <= kWave) {
         return v;
     } else {
         const int w = Group<G>::tid() / kWave;
@@ -139,8 +155,8 @@ __device__ int compact_table(int *keys, V *vals, int H, int *red)
         V v{};
         if (vals) v = vals[s];
         const bool occ = k != kEmpty;
-        const uint64_t m = ballot(occ);
-        int pos = lanes_below(m);
+        const uint64_t m = Group<G>::part(ballot(occ));
+        int pos = G >= kWave ? lanes_below(m) : __popcll(m & ((1ull << t) - 1));
         int tot = __popcll(m);
         if constexpr (G > kWave) {
             const int w = t / kWave;

@@ -280,13 +280,13 @@ static void dnum_bins(size_t max_lds, BinThresholds &thr, std::vector<BinCfg> &c
     cfg.push_back({0, 0});
 }
 
-// lanes per right-hand row: enough that 2^slog lanes x kChunk entries cover a
-// mean right row in one chunk (rounded up to a power of two, at most the group)
+// lanes per right-hand row: the mean right-row length rounded up to a power of
+// two (at most one wavefront), so one step of the sub-group covers a mean row
 static int sub_log(const sp3mm_dmat &B, int G)
 {
     const double avg = B.M ? (double)B.nnz / (double)B.M : 1.0;
     int l = 0;
-    while ((double)(1 << l) * 8.0 < avg && l < 6) l++;
+    while ((double)(1 << l) < avg && l < 6) l++;
     const int gl = ilog2((unsigned)G);
     return l > gl ? gl : l;
 }
@@ -766,6 +766,14 @@ struct FusedCfg {
 // bins by the upper bound of the R*A row: table 1 of H1 >= ub slots, table 2
 // of H1/4 (>= 64) slots; rows above the largest H1 that fits go to the
 // long-row kernels
+// largest stage-1 table handled by half-wavefront groups (two rows per
+// wavefront); tuning override SP3MM_SUBWAVE_H (0 disables)
+static int sub_wave_h()
+{
+    const char *e = getenv("SP3MM_SUBWAVE_H");
+    return e ? atoi(e) : 256;
+}
+
 static void fused_bins(size_t max_lds, bool mixed, BinThresholds &thr, std::vector<FusedCfg> &cfg)
 {
     thr.nb = 0;
@@ -773,7 +781,7 @@ static void fused_bins(size_t max_lds, bool mixed, BinThresholds &thr, std::vect
     for (long long v = 64; thr.nb < kMaxBins - 1; v *= 2) {
         const int H1 = (int)v;
         const int H2 = H1 / 4 > 64 ? H1 / 4 : 64;
-        const int G = H1 <= 512 ? 64 : 256;
+        const int G = H1 <= sub_wave_h() ? 32 : H1 <= 512 ? 64 : 256;
         if ((mixed ? mixed_lds(G, H1, H2) : fused_lds(G, H1, H2)) > max_lds) break;
         thr.v[thr.nb++] = v;
         cfg.push_back({G, H1, H2});
@@ -787,7 +795,8 @@ static void fused_bins(size_t max_lds, bool mixed, BinThresholds &thr, std::vect
 // synchronisation, so it can be captured into a HIP graph and replayed.
 struct FusedPlan {
     sp3mm_dmat R{}, A{}, P{};
-    bool mixed = false;
+    bool mixed = false;  // FP32-pair accumulation kernel
+    bool f32 = false;    // FP32 operands (mixed, or FP64 accumulation of FP32 inputs)
     double guard = 1e-2;
     double *rowsum = nullptr;
     BinThresholds thr{};
@@ -822,7 +831,7 @@ struct FusedPlan {
             launch_direct_fused(f.G, f.H1, f.H2, slog1, slog2, r(), a(), p(), lists.as<int32_t>() + (int64_t)i * M,
                                 d()->counts + i, cnt[i], rown.as<int32_t>(), slotn.as<int32_t>() + nbase[i],
                                 scol.as<int32_t>() + soff[i], sval.as<double>() + soff[i], retry.as<int32_t>(),
-                                &d()->nretry, &d()->err, max_blocks(c), s);
+                                &d()->nretry, &d()->err, max_blocks(c), s, f32);
     }
 
     void launch_tail(Context *c, hipStream_t s)
@@ -838,11 +847,11 @@ struct FusedPlan {
         if (nlong > 0)
             launch_dense_direct(true, dense_blocks(c, nlong), r(), a(), p(), lists.as<int32_t>() + (int64_t)thr.nb * M,
                                 d()->counts + thr.nb, dws.ws, dlog1, dlog2, nullptr, nullptr, C.rp, C.col,
-                                static_cast<double *>(C.val), &d()->err, s, mixed, rowsum);
+                                static_cast<double *>(C.val), &d()->err, s, f32, mixed ? rowsum : nullptr);
         if (nretry > 0)
             launch_dense_direct(true, dense_blocks(c, nretry), r(), a(), p(), retry.as<int32_t>(), &d()->nretry,
                                 rws.ws, dlog1, dlog2, nullptr, nullptr, C.rp, C.col, static_cast<double *>(C.val),
-                                &d()->err, s, mixed, rowsum);
+                                &d()->err, s, f32, mixed ? rowsum : nullptr);
     }
 
     void launch_fixups(Context *c, hipStream_t s, bool all_bins)
@@ -914,7 +923,7 @@ struct FusedPlan {
             if (dws.alloc(dense_blocks(c, nlong), A.N, P.N, true, s)) return -1;
             launch_dense_direct(false, dense_blocks(c, nlong), r(), a(), p(), lists.as<int32_t>() + (int64_t)thr.nb * M,
                                 d()->counts + thr.nb, dws.ws, dlog1, dlog2, nullptr, rown.as<int32_t>(), nullptr,
-                                nullptr, nullptr, &d()->err, s, mixed);
+                                nullptr, nullptr, &d()->err, s, f32);
         }
         if (dmat_alloc(C, M, P.N, 0, SP3MM_F64)) return -1;
         launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d()->nnz, s);
@@ -925,7 +934,7 @@ struct FusedPlan {
             if (rws.alloc(dense_blocks(c, nretry), A.N, P.N, true, s)) return -1;
             launch_dense_direct(false, dense_blocks(c, nretry), r(), a(), p(), retry.as<int32_t>(), &d()->nretry,
                                 rws.ws, dlog1, dlog2, nullptr, rown.as<int32_t>(), nullptr, nullptr, nullptr,
-                                &d()->err, s, mixed);
+                                &d()->err, s, f32);
             launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d()->nnz, s);
             if (read_ctl(c, d(), h)) return -1;
         }
@@ -974,11 +983,11 @@ struct FusedPlan {
         if (nlong > 0)
             launch_dense_direct(false, dense_blocks(c, nlong), r(), a(), p(), lists.as<int32_t>() + (int64_t)thr.nb * M,
                                 d()->counts + thr.nb, dws.ws, dlog1, dlog2, nullptr, rown.as<int32_t>(), nullptr,
-                                nullptr, nullptr, &d()->err, s, mixed);
+                                nullptr, nullptr, &d()->err, s, f32);
         if (nretry > 0)
             launch_dense_direct(false, dense_blocks(c, nretry), r(), a(), p(), retry.as<int32_t>(), &d()->nretry,
                                 rws.ws, dlog1, dlog2, nullptr, rown.as<int32_t>(), nullptr, nullptr, nullptr,
-                                &d()->err, s, mixed);
+                                &d()->err, s, f32);
         launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d()->nnz, s);
         launch_tail(c, s);
         launch_fixups(c, s, true);
@@ -1004,7 +1013,7 @@ struct FusedPlan {
 };
 
 static int rap_direct_fused(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, sp3mm_dmat &C,
-                            sp3mm_stats *st, bool mixed, double guard, double *rowsum)
+                            sp3mm_stats *st, int prec, double guard, double *rowsum)
 {
     Context *c = ctx();
     Timer tm(c->stream, true);
@@ -1013,7 +1022,8 @@ static int rap_direct_fused(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3m
     plan.R = R;
     plan.A = A;
     plan.P = P;
-    plan.mixed = mixed;
+    plan.mixed = prec == SP3MM_PREC_MIXED;
+    plan.f32 = prec != SP3MM_PREC_FP64;
     plan.guard = guard;
     plan.rowsum = rowsum;
     if (plan.build(c, tm)) {
@@ -1055,11 +1065,11 @@ int rap_direct(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, sp
     if (check_pair(R, A) || check_pair(A, P)) return -1;
     static const bool twopass = getenv("SP3MM_DIRECT_TWOPASS") != nullptr;
     if (symbolic_only || twopass) return rap_direct_twopass(R, A, P, C, st, timing, symbolic_only);
-    return rap_direct_fused(R, A, P, C, st, false, 0.0, nullptr);
+    return rap_direct_fused(R, A, P, C, st, SP3MM_PREC_FP64, 0.0, nullptr);
 }
 
 int rap_direct_mixed(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, sp3mm_dmat &C, sp3mm_stats *st,
-                     double guard, double *rowsum)
+                     int prec, double guard, double *rowsum)
 {
     Context *c = ctx();
     if (!c) return -1;
@@ -1076,7 +1086,11 @@ int rap_direct_mixed(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat 
         set_error("the mixed-precision product needs FP32 operands (sp3mm_dmat_convert)");
         return -1;
     }
-    return rap_direct_fused(R, A, P, C, st, true, guard > 0 ? guard : 1e-2, rowsum);
+    if (prec == SP3MM_PREC_MIXED64 && rowsum) {
+        set_error("row sums are produced by the FP32-pair form only");
+        return -1;
+    }
+    return rap_direct_fused(R, A, P, C, st, prec, guard > 0 ? guard : 1e-2, rowsum);
 }
 
 // --------------------------------------------------------------------------
@@ -1123,7 +1137,7 @@ static int pack(Context *c, int count, const sp3mm_dmat *const *ms, int vtype, s
     return 0;
 }
 
-void *batch_create(int count, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P, bool mixed,
+void *batch_create(int count, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P, int prec,
                    double guard, bool graph)
 {
     Context *c = ctx();
@@ -1133,6 +1147,7 @@ void *batch_create(int count, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp
         set_error("batch: empty batch");
         return nullptr;
     }
+    const bool mixed = prec != SP3MM_PREC_FP64;
     const int vt = mixed ? SP3MM_F32 : SP3MM_F64;
     for (int i = 0; i < count; i++) {
         if (R[i].N != A[i].M || A[i].N != P[i].M) {
@@ -1179,7 +1194,8 @@ void *batch_create(int count, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp
     b->plan.R = b->Rb;
     b->plan.A = b->Ab;
     b->plan.P = b->Pb;
-    b->plan.mixed = mixed;
+    b->plan.mixed = prec == SP3MM_PREC_MIXED;
+    b->plan.f32 = mixed;
     b->plan.guard = guard > 0 ? guard : 1e-2;
     if (b->plan.build(c, tm)) {
         b->plan.release_all();

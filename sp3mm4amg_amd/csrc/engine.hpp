@@ -78,9 +78,9 @@ int spgemm(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &C, sp3mm_stats 
 int rap_direct(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, sp3mm_dmat &C,
                sp3mm_stats *st, bool timing, bool symbolic_only = false);
 int rap_direct_mixed(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, sp3mm_dmat &C, sp3mm_stats *st,
-                     double guard, double *rowsum);
+                     int prec, double guard, double *rowsum);
 void dmat_release(sp3mm_dmat &m);
-void *batch_create(int count, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P, bool mixed,
+void *batch_create(int count, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P, int prec,
                    double guard, bool graph);
 int batch_run(void *b, bool use_graph, double *ms);
 int batch_result(void *b, int i, sp3mm_dmat &out);

@@ -22,6 +22,9 @@
 // runtime powers of two chosen by the host from the bin of the row.
 #include "device_common.hpp"
 #include "kernels.hpp"
+#include <algorithm>
+#include <stdlib.h>
+#include <string.h>
 
 namespace sp3mm {
 
@@ -217,15 +220,18 @@ __global__ void k_ub_colparts(const int64_t *__restrict__ arp, const int32_t *__
 // product expansion helpers
 // ---------------------------------------------------------------------------
 // Expand one left row (entries [e0,e1) of lcol/lval) against right operand B:
-// every product (b_col, lval*b_val) goes to the LDS table.  Lanes are split
-// into sub-groups of 2^slog lanes, one sub-group per left entry.  Each lane
-// first loads its left entry and the bounds of the right row, then walks the
-// right row in chunks of kChunk entries that are all loaded before any of
-// them is inserted, so a row costs three dependent global round trips (left
-// entries, right row bounds, right entries) however many entries it has.
-constexpr int kChunk = 8;
-
-template <int G, bool NUM, typename LV, typename BV = double>
+// every product (b_col, lval*b_val) goes to the LDS table.
+//
+// Lanes form sub-groups of S = 2^slog lanes; a sub-group owns one left entry at
+// a time and its lanes walk that entry's right row with stride S (S is chosen
+// near the typical right-row length, so one step covers most rows).  Every lane
+// holds U left entries at once (entries g, g + ngrp, ... of the pass), loads all
+// their row bounds, then loads one right entry per left entry and inserts the U
+// products: a pass over U*ngrp left entries costs three dependent global round
+// trips.  The first probe of every insert is one compare-and-swap issued back
+// to back with the others; only an insert whose first slot holds another key
+// enters the probing loop, so the common path has no divergent loops.
+template <int G, bool NUM, typename LV, typename BV = double, int U = 4>
 __device__ __forceinline__ void expand_into(const int32_t *lcol, const LV *lval,
                                             int64_t e0, int64_t e1, const int32_t *__restrict__ brp,
                                             const int32_t *__restrict__ bcol,
@@ -233,32 +239,73 @@ __device__ __forceinline__ void expand_into(const int32_t *lcol, const LV *lval,
                                             double *vals, unsigned mask, int shift, int &fresh_cnt,
                                             bool &full)
 {
+    constexpr int kLogG = G >= 256 ? 8 : G >= 128 ? 7 : G >= 64 ? 6 : G >= 32 ? 5 : 4;
+    if (slog > kLogG) slog = kLogG;  // a sub-group never spans more than the group
     const int t = Group<G>::tid();
-    const int sub = t & ((1 << slog) - 1);
-    const int step_e = G >> slog;
     const int S = 1 << slog;
-    for (int64_t e = e0 + (t >> slog); e < e1; e += step_e) {
-        const int k = lcol[e];
-        double lv = 0.0;
-        if constexpr (NUM) lv = (double)lval[e];
-        const int b1 = brp[k + 1];
-        for (int q0 = brp[k] + sub; q0 < b1; q0 += S * kChunk) {
-            int c[kChunk];
-            double w[kChunk];
+    const int sub = t & (S - 1);
+    const int ngrp = G >> slog;
+    for (int64_t eb = e0 + (t >> slog); eb < e1; eb += (int64_t)ngrp * U) {
+        int q[U], qe[U];
+        double lv[U];
 #pragma unroll
-            for (int u = 0; u < kChunk; u++) {
-                const int q = q0 + u * S;
-                c[u] = q < b1 ? bcol[q] : kEmpty;
-                if constexpr (NUM) w[u] = q < b1 ? (double)bval[q] : 0.0;
+        for (int u = 0; u < U; u++) {
+            const int64_t e = eb + (int64_t)u * ngrp;
+            const bool ok = e < e1;
+            const int k = ok ? lcol[e] : 0;
+            lv[u] = 0.0;
+            if constexpr (NUM) lv[u] = ok ? (double)lval[e] : 0.0;
+            q[u] = ok ? brp[k] + sub : 0;
+            qe[u] = ok ? brp[k + 1] : 0;
+        }
+        while (true) {
+            bool more = false;
+#pragma unroll
+            for (int u = 0; u < U; u++) more |= q[u] < qe[u];
+            if (!more) break;
+            int c[U], old[U];
+            unsigned sl[U];
+            double w[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const bool ok = q[u] < qe[u];
+                c[u] = ok ? bcol[q[u]] : kEmpty;
+                w[u] = 0.0;
+                if constexpr (NUM) w[u] = ok ? (double)bval[q[u]] : 0.0;
+                q[u] += S;
             }
 #pragma unroll
-            for (int u = 0; u < kChunk; u++) {
-                if (c[u] == kEmpty) continue;
-                int fresh;
-                const int s = hash_insert(keys, mask, shift, c[u], &fresh);
-                if (s < 0) { full = true; continue; }
-                fresh_cnt += fresh;
-                if constexpr (NUM) atomicAdd(vals + s, lv * w[u]);
+            for (int u = 0; u < U; u++) {
+                sl[u] = hslot(c[u], shift) & mask;
+                old[u] = c[u] != kEmpty ? atomicCAS(keys + sl[u], kEmpty, c[u]) : kEmpty;
+            }
+            bool pend = false;
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const bool hit = c[u] != kEmpty && (old[u] == kEmpty || old[u] == c[u]);
+                pend |= c[u] != kEmpty && !hit;
+                if (hit) {
+                    fresh_cnt += old[u] == kEmpty;
+                    if constexpr (NUM) atomicAdd(vals + sl[u], lv[u] * w[u]);
+                }
+            }
+            if (pend) {
+                for (int u = 0; u < U; u++) {
+                    if (c[u] == kEmpty || old[u] == kEmpty || old[u] == c[u]) continue;
+                    unsigned s2 = sl[u];
+                    bool done = false;
+                    for (unsigned probe = 1; probe <= mask; probe++) {
+                        s2 = (s2 + 1) & mask;
+                        const int o = atomicCAS(keys + s2, kEmpty, c[u]);
+                        if (o == kEmpty || o == c[u]) {
+                            fresh_cnt += o == kEmpty;
+                            if constexpr (NUM) atomicAdd(vals + s2, lv[u] * w[u]);
+                            done = true;
+                            break;
+                        }
+                    }
+                    if (!done) full = true;
+                }
             }
         }
     }
@@ -464,7 +511,7 @@ __global__ void __launch_bounds__(256) k_direct(SpOperand R, SpOperand A, SpOper
 // length goes to rownnz[r] and slotn[i].  A row whose table 2 fills is marked
 // slotn[i] = -1 and queued on `retry` for the long-row kernels.  After the
 // scan of rownnz, k_gather_rows moves every slot to its final place.
-template <int G>
+template <int G, int U1, int U2, typename VT = double>
 __global__ void __launch_bounds__(256) k_direct_fused(SpOperand R, SpOperand A, SpOperand P,
                                                        const int32_t *__restrict__ rows,
                                                        const int *__restrict__ count, int H1, int H2,
@@ -494,9 +541,9 @@ __global__ void __launch_bounds__(256) k_direct_fused(SpOperand R, SpOperand A, 
     const unsigned m1 = (unsigned)H1 - 1, m2 = (unsigned)H2 - 1;
     const int sh1 = 32 - ilog2((unsigned)H1), sh2 = 32 - ilog2((unsigned)H2);
     const int n = *count;
-    const double *rval = static_cast<const double *>(R.val);
-    const double *aval = static_cast<const double *>(A.val);
-    const double *pval = static_cast<const double *>(P.val);
+    const VT *rval = static_cast<const VT *>(R.val);
+    const VT *aval = static_cast<const VT *>(A.val);
+    const VT *pval = static_cast<const VT *>(P.val);
     for (int i = blockIdx.x * GPB + g; i < n; i += gridDim.x * GPB) {
         const int r = rows[i];
         for (int s = t; s < H1; s += G) {
@@ -510,12 +557,13 @@ __global__ void __launch_bounds__(256) k_direct_fused(SpOperand R, SpOperand A, 
         Group<G>::sync();
         int c = 0;
         bool full1 = false, full2 = false;
-        expand_into<G, true, double>(R.col, rval, R.rp[r], R.rp[r + 1], A.rp32, A.col, aval, slog1, k1, v1, m1,
-                                     sh1, c, full1);
+        expand_into<G, true, VT, VT, U1>(R.col, rval, R.rp[r], R.rp[r + 1], A.rp32, A.col, aval, slog1, k1, v1, m1,
+                                         sh1, c, full1);
         Group<G>::sync();
         const int nn1 = compact_table<G, double>(k1, v1, H1, ired);
         if (!full1)
-            expand_into<G, true, double>(k1, v1, 0, nn1, P.rp32, P.col, pval, slog2, k2, v2, m2, sh2, c, full2);
+            expand_into<G, true, double, VT, U2>(k1, v1, 0, nn1, P.rp32, P.col, pval, slog2, k2, v2, m2, sh2, c,
+                                                 full2);
         const long long anyfull = group_max<G>((full1 || full2) ? 1 : 0, red);
         if (anyfull) {
             if (t == 0) {
@@ -972,13 +1020,13 @@ static unsigned grid_for(int rows, int G, int max_blocks)
     return (unsigned)b;
 }
 
-size_t symb_lds(int G, int H) { return (size_t)(256 / G) * H * 4 + (size_t)(256 / G) * (G / 64) * 8; }
-size_t num_lds(int G, int H) { return (size_t)(256 / G) * H * 12 + (size_t)(256 / G) * (G / 64 + 1) * 4; }
+size_t symb_lds(int G, int H) { return (size_t)(256 / G) * H * 4 + (size_t)(256 / G) * group_waves(G) * 8; }
+size_t num_lds(int G, int H) { return (size_t)(256 / G) * H * 12 + (size_t)(256 / G) * (group_waves(G) + 1) * 4; }
 size_t direct_lds(int G, int H1, int H2, bool num)
 {
     const size_t gpb = 256 / G;
     size_t b = gpb * ((size_t)H1 + H2) * (num ? 12 : 4);
-    b += gpb * (G / 64) * 8 + gpb * (G / 64 + 1) * 4;
+    b += gpb * group_waves(G) * 8 + gpb * (group_waves(G) + 1) * 4;
     return b;
 }
 
@@ -1016,6 +1064,7 @@ void launch_direct(int G, bool num, int H1, int H2, int slog1, int slog2, const 
                    double *cval, int32_t *retry, int *nretry, int *err, int max_blocks,
                    hipStream_t st, bool f32in)
 {
+    if (G < 64) G = 64;  // the two-pass kernels run one row per wavefront or workgroup
     const unsigned grid = grid_for(rows_host, G, max_blocks);
     const size_t lds = direct_lds(G, H1, H2, num);
 #define SP3MM_DIRECT_ARGS R, A, P, rows, count, H1, H2, slog1, slog2, n1out, rownnz, crp, ccol, cval, \
@@ -1073,22 +1122,60 @@ void launch_dense_direct(bool num, int blocks, const SpOperand &R, const SpOpera
 size_t fused_lds(int G, int H1, int H2)
 {
     const size_t gpb = 256 / G;
-    return gpb * ((size_t)H1 + H2) * 12 + gpb * (G / 64) * 8 + gpb * (G / 64 + 1) * 4;
+    return gpb * ((size_t)H1 + H2) * 12 + gpb * group_waves(G) * 8 + gpb * (group_waves(G) + 1) * 4;
+}
+
+using FusedFn = void (*)(SpOperand, SpOperand, SpOperand, const int32_t *, const int *, int, int, int, int,
+                        int32_t *, int32_t *, int32_t *, double *, int32_t *, int *, int *);
+
+// left entries held per lane in stage 1 / stage 2 (index = log2(U) - 1)
+template <int G, typename VT>
+static FusedFn fused_variant(int u1, int u2)
+{
+    static const FusedFn tab[2][2] = {
+        {k_direct_fused<G, 2, 2, VT>, k_direct_fused<G, 2, 4, VT>},
+        {k_direct_fused<G, 4, 2, VT>, k_direct_fused<G, 4, 4, VT>},
+    };
+    return tab[u1][u2];
+}
+
+// tuning override "ab": a, b in {1,2,4} = left entries per lane in stage 1, 2
+static int ulog_env(int pos, int dflt)
+{
+    const char *e = getenv("SP3MM_FUSED_U");
+    if (!e || (int)strlen(e) < 2) return dflt;
+    const int u = e[pos] - '0';
+    return u == 2 ? 0 : u == 4 ? 1 : dflt;
+}
+
+// tuning override: largest sub-group (log2 lanes per right row) for groups
+// smaller than a wavefront
+static int slog_cap_env(int dflt)
+{
+    const char *e = getenv("SP3MM_SLOG_MAX");
+    return e && *e >= '0' && *e <= '6' ? *e - '0' : dflt;
 }
 
 void launch_direct_fused(int G, int H1, int H2, int slog1, int slog2, const SpOperand &R, const SpOperand &A,
                          const SpOperand &P, const int32_t *rows, const int *count, int rows_host,
                          int32_t *rownnz, int32_t *slotn, int32_t *scol, double *sval, int32_t *retry,
-                         int *nretry, int *err, int max_blocks, hipStream_t st)
+                         int *nretry, int *err, int max_blocks, hipStream_t st, bool f32in)
 {
     const unsigned grid = grid_for(rows_host, G, max_blocks);
     const size_t lds = fused_lds(G, H1, H2);
-    if (G == 64)
-        hipLaunchKernelGGL(k_direct_fused<64>, dim3(grid), dim3(256), lds, st, R, A, P, rows, count, H1, H2, slog1,
-                           slog2, rownnz, slotn, scol, sval, retry, nretry, err);
-    else
-        hipLaunchKernelGGL(k_direct_fused<256>, dim3(grid), dim3(256), lds, st, R, A, P, rows, count, H1, H2,
-                           slog1, slog2, rownnz, slotn, scol, sval, retry, nretry, err);
+    const int u1 = ulog_env(0, 1), u2 = ulog_env(1, 1);
+    int cap = ilog2((unsigned)G);
+    if (G < 64) cap = std::min(cap, slog_cap_env(cap));
+    slog1 = std::min(slog1, cap);
+    slog2 = std::min(slog2, cap);
+    const FusedFn fn = f32in ? (G == 32   ? fused_variant<32, float>(u1, u2)
+                                : G == 64 ? fused_variant<64, float>(u1, u2)
+                                          : fused_variant<256, float>(u1, u2))
+                             : (G == 32   ? fused_variant<32, double>(u1, u2)
+                                : G == 64 ? fused_variant<64, double>(u1, u2)
+                                          : fused_variant<256, double>(u1, u2));
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(256), lds, st, R, A, P, rows, count, H1, H2, slog1, slog2, rownnz,
+                       slotn, scol, sval, retry, nretry, err);
 }
 
 void launch_gather_rows(const int32_t *rows, const int *count, int rows_host, const int32_t *slotn, int cap,

@@ -2,9 +2,11 @@
 //
 // FP32 storage and FP32 multiplies; every product is formed exactly as an
 // FP32 pair with TwoProd (one FMA) and accumulated into its LDS slot as an
-// FP32 pair (hi, lo) with an error-free TwoSum-based addition, applied with a
-// 64-bit compare-and-swap on the packed pair.  Alongside every slot an FP32
-// sum of |terms| gives the magnitude scale (|R||A||P|)_ij of the entry.
+// FP32 pair (hi, lo): a returning FP32 atomic add on hi, TwoSum of the value
+// it returned with the addend to recover that addition's rounding error, and
+// a plain atomic add of the error (plus the addend's low part) on lo -- two
+// atomics per product and no compare-and-swap loop.  Alongside every slot an
+// FP32 sum of |terms| gives the magnitude scale (|R||A||P|)_ij of the entry.
 //
 // FP64 is used only where cancellation needs it:
 //   * products landing on the diagonal (column == row) are also summed in an
@@ -68,40 +70,33 @@ __device__ __forceinline__ FF ff_mul(FF x, float b)
     return quick_two_sum(p.h, p.l);
 }
 
-__device__ __forceinline__ unsigned long long ff_pack(FF v)
+// Adds the pair v to the slot (hi, lo) with two atomics and no retry loop:
+// the returning FP32 atomic add on hi yields the value it was added to, so
+// the rounding error of that addition is recovered exactly with TwoSum on the
+// lane and goes, together with v.l, into lo with a plain atomic add.  The LDS
+// adder rounds to nearest even like the lane's own addition, so the recovered
+// error is the one the hardware made.
+__device__ __forceinline__ void ff_atomic_accum(float *hi, float *lo, FF v)
 {
-    return (unsigned long long)__float_as_uint(v.h) | ((unsigned long long)__float_as_uint(v.l) << 32);
+    const float old = atomicAdd(hi, v.h);
+    const FF s = two_sum(old, v.h);
+    atomicAdd(lo, v.l + s.l);
 }
 
-__device__ __forceinline__ FF ff_unpack(unsigned long long u)
-{
-    return {__uint_as_float((unsigned)u), __uint_as_float((unsigned)(u >> 32))};
-}
-
-__device__ __forceinline__ void ff_atomic_add(unsigned long long *slot, FF v)
-{
-    unsigned long long old = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    unsigned long long assumed;
-    do {
-        assumed = old;
-        old = atomicCAS(slot, assumed, ff_pack(ff_add(ff_unpack(assumed), v)));
-    } while (old != assumed);
-}
-
-// compaction of keys with two value arrays (pair and scale), slot order kept
+// compaction of keys with three FP32 value arrays (pair hi, pair lo, scale),
+// slot order kept
 template <int G>
-__device__ int compact2(int *keys, unsigned long long *v, float *a, int H, int *red)
+__device__ int compact3(int *keys, float *h, float *l, float *a, int H, int *red)
 {
     const int t = Group<G>::tid();
     int base = 0;
     for (int s0 = 0; s0 < H; s0 += G) {
         const int s = s0 + t;
         const int k = keys[s];
-        const unsigned long long vv = v[s];
-        const float aa = a[s];
+        const float hh = h[s], ll = l[s], aa = a[s];
         const bool occ = k != kEmpty;
-        const uint64_t m = ballot(occ);
-        int pos = lanes_below(m);
+        const uint64_t m = Group<G>::part(ballot(occ));
+        int pos = G >= kWave ? lanes_below(m) : __popcll(m & ((1ull << t) - 1));
         int tot = __popcll(m);
         if constexpr (G > kWave) {
             const int w = t / kWave;
@@ -120,7 +115,8 @@ __device__ int compact2(int *keys, unsigned long long *v, float *a, int H, int *
         }
         if (occ) {
             keys[base + pos] = k;
-            v[base + pos] = vv;
+            h[base + pos] = hh;
+            l[base + pos] = ll;
             a[base + pos] = aa;
         }
         base += tot;
@@ -132,8 +128,94 @@ __device__ int compact2(int *keys, unsigned long long *v, float *a, int H, int *
 size_t mixed_lds(int G, int H1, int H2)
 {
     const size_t gpb = 256 / G;
-    return gpb * ((size_t)H1 + H2) * 16 + gpb * 8 + gpb * (G / 64) * 8 + gpb * (G / 64 + 1) * 4;
+    return gpb * ((size_t)H1 + H2) * 16 + gpb * 8 + gpb * group_waves(G) * 8 + gpb * (group_waves(G) + 1) * 4;
 }
+
+// Expansion of one left row into an LDS table, as in the FP64 kernel (sub-groups
+// of 2^slog lanes per left entry, U left entries per lane per pass, the first
+// probe of every insert issued back to back); `loadl(e)` and `loadb(q)` fetch
+// the left and right payloads and `acc(slot, left, right, col)` accumulates one
+// product into an occupied slot.
+template <int G, int U, typename LoadL, typename LoadB, typename Acc>
+__device__ __forceinline__ void expand_mixed(const int32_t *lcol, int64_t e0, int64_t e1,
+                                             const int32_t *__restrict__ brp, const int32_t *__restrict__ bcol,
+                                             int slog, int *keys, unsigned mask, int shift, bool &full,
+                                             LoadL loadl, LoadB loadb, Acc acc)
+{
+    using LT = decltype(loadl(int64_t{}));
+    using BT = decltype(loadb(int{}));
+    constexpr int kLogG = G >= 256 ? 8 : G >= 128 ? 7 : G >= 64 ? 6 : G >= 32 ? 5 : 4;
+    if (slog > kLogG) slog = kLogG;
+    const int t = Group<G>::tid();
+    const int S = 1 << slog;
+    const int sub = t & (S - 1);
+    const int ngrp = G >> slog;
+    for (int64_t eb = e0 + (t >> slog); eb < e1; eb += (int64_t)ngrp * U) {
+        int q[U], qe[U];
+        LT lv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t e = eb + (int64_t)u * ngrp;
+            const bool ok = e < e1;
+            const int k = ok ? lcol[e] : 0;
+            lv[u] = loadl(ok ? e : e0);
+            q[u] = ok ? brp[k] + sub : 0;
+            qe[u] = ok ? brp[k + 1] : 0;
+        }
+        while (true) {
+            bool more = false;
+#pragma unroll
+            for (int u = 0; u < U; u++) more |= q[u] < qe[u];
+            if (!more) break;
+            int c[U], old[U];
+            unsigned sl[U];
+            BT w[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const bool ok = q[u] < qe[u];
+                c[u] = ok ? bcol[q[u]] : kEmpty;
+                w[u] = loadb(ok ? q[u] : 0);
+                q[u] += S;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                sl[u] = hslot(c[u], shift) & mask;
+                old[u] = c[u] != kEmpty ? atomicCAS(keys + sl[u], kEmpty, c[u]) : kEmpty;
+            }
+            bool pend = false;
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const bool hit = c[u] != kEmpty && (old[u] == kEmpty || old[u] == c[u]);
+                pend |= c[u] != kEmpty && !hit;
+                if (hit) acc(sl[u], lv[u], w[u], c[u]);
+            }
+            if (pend) {
+                for (int u = 0; u < U; u++) {
+                    if (c[u] == kEmpty || old[u] == kEmpty || old[u] == c[u]) continue;
+                    unsigned s2 = sl[u];
+                    bool done = false;
+                    for (unsigned probe = 1; probe <= mask; probe++) {
+                        s2 = (s2 + 1) & mask;
+                        const int o = atomicCAS(keys + s2, kEmpty, c[u]);
+                        if (o == kEmpty || o == c[u]) {
+                            acc(s2, lv[u], w[u], c[u]);
+                            done = true;
+                            break;
+                        }
+                    }
+                    if (!done) full = true;
+                }
+            }
+        }
+    }
+}
+
+// stage-2 left payload: one entry of row r of R*A
+struct Mid {
+    FF x;       // value as a normalised FP32 pair
+    float xa;   // magnitude scale (|R||A|)_rj
+    double xd;  // value in FP64 (for the diagonal and the row sum)
+};
 
 template <int G>
 __global__ void __launch_bounds__(256) k_direct_mixed(SpOperand R, SpOperand A, SpOperand P,
@@ -149,30 +231,34 @@ __global__ void __launch_bounds__(256) k_direct_mixed(SpOperand R, SpOperand A, 
 {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int GPB = 256 / G;
+    constexpr int KW = Group<G>::kWaves;
     const int g = threadIdx.x / G;
     const int t = Group<G>::tid();
     unsigned char *p = smem;
-    unsigned long long *v1 = reinterpret_cast<unsigned long long *>(p) + g * H1;
-    p += (size_t)GPB * H1 * 8;
-    unsigned long long *v2 = reinterpret_cast<unsigned long long *>(p) + g * H2;
-    p += (size_t)GPB * H2 * 8;
     double *dg = reinterpret_cast<double *>(p) + g;
     p += (size_t)GPB * 8;
-    long long *red = reinterpret_cast<long long *>(p) + g * Group<G>::kWaves;
-    p += (size_t)GPB * Group<G>::kWaves * 8;
+    long long *red = reinterpret_cast<long long *>(p) + g * KW;
+    p += (size_t)GPB * KW * 8;
+    float *h1 = reinterpret_cast<float *>(p) + g * H1;
+    p += (size_t)GPB * H1 * 4;
+    float *l1 = reinterpret_cast<float *>(p) + g * H1;
+    p += (size_t)GPB * H1 * 4;
     float *a1 = reinterpret_cast<float *>(p) + g * H1;
     p += (size_t)GPB * H1 * 4;
-    float *a2 = reinterpret_cast<float *>(p) + g * H2;
-    p += (size_t)GPB * H2 * 4;
     int *k1 = reinterpret_cast<int *>(p) + g * H1;
     p += (size_t)GPB * H1 * 4;
+    float *h2 = reinterpret_cast<float *>(p) + g * H2;
+    p += (size_t)GPB * H2 * 4;
+    float *l2 = reinterpret_cast<float *>(p) + g * H2;
+    p += (size_t)GPB * H2 * 4;
+    float *a2 = reinterpret_cast<float *>(p) + g * H2;
+    p += (size_t)GPB * H2 * 4;
     int *k2 = reinterpret_cast<int *>(p) + g * H2;
     p += (size_t)GPB * H2 * 4;
-    int *ired = reinterpret_cast<int *>(p) + g * (Group<G>::kWaves + 1);
+    int *ired = reinterpret_cast<int *>(p) + g * (KW + 1);
 
     const unsigned m1 = (unsigned)H1 - 1, m2 = (unsigned)H2 - 1;
     const int sh1 = 32 - ilog2((unsigned)H1), sh2 = 32 - ilog2((unsigned)H2);
-    const int S1 = 1 << slog1, S2 = 1 << slog2;
     const int n = *count;
     const float *rv = static_cast<const float *>(R.val);
     const float *av = static_cast<const float *>(A.val);
@@ -181,75 +267,49 @@ __global__ void __launch_bounds__(256) k_direct_mixed(SpOperand R, SpOperand A, 
         const int r = rows[i];
         for (int s = t; s < H1; s += G) {
             k1[s] = kEmpty;
-            v1[s] = 0ull;
+            h1[s] = 0.f;
+            l1[s] = 0.f;
             a1[s] = 0.f;
         }
         for (int s = t; s < H2; s += G) {
             k2[s] = kEmpty;
-            v2[s] = 0ull;
+            h2[s] = 0.f;
+            l2[s] = 0.f;
             a2[s] = 0.f;
         }
         if (t == 0) *dg = 0.0;
         Group<G>::sync();
         bool full1 = false, full2 = false;
-        // stage 1: row r of R*A as FP32 pairs
-        for (int64_t e = R.rp[r] + (t >> slog1); e < R.rp[r + 1]; e += (G >> slog1)) {
-            const int k = R.col[e];
-            const float x = rv[e];
-            const int b1 = A.rp32[k + 1];
-            for (int q0 = A.rp32[k] + (t & (S1 - 1)); q0 < b1; q0 += S1 * 8) {
-                int c[8];
-                float w[8];
-#pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    const int q = q0 + u * S1;
-                    c[u] = q < b1 ? A.col[q] : kEmpty;
-                    w[u] = q < b1 ? av[q] : 0.f;
-                }
-#pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    if (c[u] == kEmpty) continue;
-                    int fresh;
-                    const int sl = hash_insert(k1, m1, sh1, c[u], &fresh);
-                    if (sl < 0) { full1 = true; continue; }
-                    ff_atomic_add(v1 + sl, two_prod(x, w[u]));
-                    atomicAdd(a1 + sl, fabsf(x) * fabsf(w[u]));
-                }
-            }
-        }
+        // stage 1: row r of R*A; every FP32 product is split exactly by TwoProd
+        expand_mixed<G, 4>(
+            R.col, R.rp[r], R.rp[r + 1], A.rp32, A.col, slog1, k1, m1, sh1, full1,
+            [&](int64_t e) { return rv[e]; }, [&](int q) { return av[q]; },
+            [&](unsigned sl, float x, float w, int) {
+                const FF pr = two_prod(x, w);
+                ff_atomic_accum(h1 + sl, l1 + sl, pr);
+                atomicAdd(a1 + sl, fabsf(pr.h));
+            });
         Group<G>::sync();
-        const int nn1 = compact2<G>(k1, v1, a1, H1, ired);
-        // stage 2: forward into P; diagonal and row sum also in FP64
+        const int nn1 = compact3<G>(k1, h1, l1, a1, H1, ired);
+        // stage 2: forward into P; the diagonal and the row sum also in FP64
         double rs = 0.0;
-        for (int e = t >> slog2; e < (full1 ? 0 : nn1); e += (G >> slog2)) {
-            const int j = k1[e];
-            const FF x = ff_unpack(v1[e]);
-            const float xa = a1[e];
-            const double xd = (double)x.h + (double)x.l;
-            const int b1 = P.rp32[j + 1];
-            for (int q0 = P.rp32[j] + (t & (S2 - 1)); q0 < b1; q0 += S2 * 8) {
-                int c[8];
-                float w[8];
-#pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    const int q = q0 + u * S2;
-                    c[u] = q < b1 ? P.col[q] : kEmpty;
-                    w[u] = q < b1 ? pv[q] : 0.f;
-                }
-#pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    if (c[u] == kEmpty) continue;
-                    const double pd = xd * (double)w[u];
-                    rs += pd;
-                    int fresh;
-                    const int sl = hash_insert(k2, m2, sh2, c[u], &fresh);
-                    if (sl < 0) { full2 = true; continue; }
-                    ff_atomic_add(v2 + sl, ff_mul(x, w[u]));
-                    atomicAdd(a2 + sl, xa * fabsf(w[u]));
-                    if (c[u] == r) atomicAdd(dg, pd);
-                }
-            }
-        }
+        expand_mixed<G, 2>(
+            k1, 0, full1 ? 0 : nn1, P.rp32, P.col, slog2, k2, m2, sh2, full2,
+            [&](int64_t e) {
+                Mid m;
+                m.x = two_sum(h1[e], l1[e]);
+                m.xa = a1[e];
+                m.xd = (double)m.x.h + (double)m.x.l;
+                return m;
+            },
+            [&](int q) { return pv[q]; },
+            [&](unsigned sl, const Mid &m, float w, int c) {
+                ff_atomic_accum(h2 + sl, l2 + sl, ff_mul(m.x, w));
+                atomicAdd(a2 + sl, m.xa * fabsf(w));
+                const double pd = m.xd * (double)w;
+                rs += pd;
+                if (c == r) atomicAdd(dg, pd);
+            });
         const long long anyfull = group_max<G>((full1 || full2) ? 1 : 0, red);
         if (anyfull) {
             if (t == 0) {
@@ -259,12 +319,12 @@ __global__ void __launch_bounds__(256) k_direct_mixed(SpOperand R, SpOperand A, 
             }
         } else {
             Group<G>::sync();
-            const int nn2 = compact2<G>(k2, v2, a2, H2, ired);
+            const int nn2 = compact3<G>(k2, h2, l2, a2, H2, ired);
             const int64_t off = (int64_t)i * H2;
             bool cancel = false;
             for (int j = t; j < nn2; j += G) {
                 const int k = k2[j];
-                const FF x = ff_unpack(v2[j]);
+                const FF x = two_sum(h2[j], l2[j]);
                 double val = (double)x.h + (double)x.l;
                 if (k == r) val = *dg;
                 else cancel |= fabs(val) < (double)guard * (double)a2[j];
@@ -281,14 +341,15 @@ __global__ void __launch_bounds__(256) k_direct_mixed(SpOperand R, SpOperand A, 
         }
         if (rowsum && !anyfull) {
             double tot = rs;
-            for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
+            constexpr int top = G < kWave ? G / 2 : 32;
+            for (int o = top; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
             if constexpr (G > kWave) {
                 double *dred = reinterpret_cast<double *>(red);
                 __syncthreads();
                 if (lane() == 0) dred[t / kWave] = tot;
                 __syncthreads();
                 tot = 0.0;
-                for (int w = 0; w < Group<G>::kWaves; w++) tot += dred[w];
+                for (int w = 0; w < KW; w++) tot += dred[w];
             }
             if (t == 0) rowsum[r] = tot;
         }
@@ -313,12 +374,12 @@ void launch_direct_mixed(int G, int H1, int H2, int slog1, int slog2, float guar
 {
     const unsigned grid = grid_for_m(rows_host, G, max_blocks);
     const size_t lds = mixed_lds(G, H1, H2);
-    if (G == 64)
-        hipLaunchKernelGGL(k_direct_mixed<64>, dim3(grid), dim3(256), lds, st, R, A, P, rows, count, H1, H2, slog1,
-                           slog2, guard, rownnz, slotn, scol, sval, rowsum, retry, nretry, fix, nfix, err);
-    else
-        hipLaunchKernelGGL(k_direct_mixed<256>, dim3(grid), dim3(256), lds, st, R, A, P, rows, count, H1, H2,
-                           slog1, slog2, guard, rownnz, slotn, scol, sval, rowsum, retry, nretry, fix, nfix, err);
+    const int cap = ilog2((unsigned)G);
+    if (slog1 > cap) slog1 = cap;
+    if (slog2 > cap) slog2 = cap;
+    auto fn = G == 32 ? k_direct_mixed<32> : G == 64 ? k_direct_mixed<64> : k_direct_mixed<256>;
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(256), lds, st, R, A, P, rows, count, H1, H2, slog1, slog2, guard,
+                       rownnz, slotn, scol, sval, rowsum, retry, nretry, fix, nfix, err);
 }
 
 }  // namespace sp3mm

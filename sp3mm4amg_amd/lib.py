@@ -26,7 +26,7 @@ _LIB = None
 
 F64, F32 = 0, 1
 PAIR, DIRECT = 0, 1
-FP64, MIXED = 0, 1
+FP64, MIXED, MIXED64 = 0, 1, 2
 RBTREE, IDXMAP = 0, 1
 
 
@@ -266,11 +266,14 @@ class Batch:
     """Many products R[i]*A[i]*P[i] packed into one block-diagonal product
     (optionally captured as one HIP graph)."""
 
-    def __init__(self, Rs, As, Ps, mixed: bool = False, guard: float = 0.0, graph: bool = True):
+    def __init__(self, Rs, As, Ps, mixed: bool = False, guard: float = 0.0, graph: bool = True,
+                 precision: int | None = None):
         n = len(Rs)
         arr = DMat * n
         self._R, self._A, self._P = arr(*[m.d for m in Rs]), arr(*[m.d for m in As]), arr(*[m.d for m in Ps])
-        o = Opts(DIRECT, MIXED if mixed else FP64, guard, 0, 0, None)
+        if precision is None:
+            precision = MIXED if mixed else FP64
+        o = Opts(DIRECT, precision, guard, 0, 0, None)
         h = C.c_void_p()
         _check(lib().sp3mm_batch_create(n, self._R, self._A, self._P, C.byref(o), int(graph), C.byref(h)),
                "sp3mm_batch_create")
@@ -304,12 +307,13 @@ class Batch:
             pass
 
 
-def rap_mixed_host(R: CSR, A: CSR, P: CSR, guard: float = 0.0, rowsum: bool = False):
+def rap_mixed_host(R: CSR, A: CSR, P: CSR, guard: float = 0.0, rowsum: bool = False,
+                   precision: int = MIXED):
     """Mixed-precision direct product from host matrices (values rounded to FP32
     on upload); returns (CSR with FP64 values, stats, row sums or None)."""
     dR, dA, dP = (DeviceCSR.upload(m, vtype=F32) for m in (R, A, P))
     rs = DeviceArray(R.M) if rowsum else None
-    out, st = rap(dR, dA, dP, path=DIRECT, precision=MIXED, guard=guard,
+    out, st = rap(dR, dA, dP, path=DIRECT, precision=precision, guard=guard,
                   rowsum_dev=rs.ptr if rs else None)
     res = out.download()
     sums = rs.numpy() if rs else None

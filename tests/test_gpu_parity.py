@@ -190,6 +190,27 @@ def test_mixed_precision_against_fp64_oracle(name, guard):
     assert np.all(np.abs(rs - exact) <= 4 * T * 2.0 ** -53 * srow + 1e-15 * np.abs(exact))
 
 
+@pytest.mark.parametrize("name", ["stencil27_15", "poisson3d32", "irregular3000"])
+def test_mixed64_against_fp64_oracle(name):
+    """FP32 storage with FP64 accumulation: every FP32 x FP32 product is exact in
+    FP64, so the result obeys the FP64 rounding-error model of the oracle on the
+    same FP32-rounded inputs; pattern bit-exact."""
+    R, A, P = (_rounded32(m) for m in triple(name))
+    print(f"mixed64 {name}", flush=True)
+    got, st, _ = L.rap_mixed_host(R, A, P, precision=L.MIXED64)
+    want = O.sp3mm_pair(R, A, P)
+    check(got, want, None, bound=O.error_bound(R, A, P, want))
+    err = O.rel_errors(got, O.sp3mm_pair_ld(R, A, P), O.abs_product(R, A, P))
+    print(f"  max rel vs long double {err.max():.3e}")
+    assert err.max() <= 1e-10
+
+
+def test_mixed64_rejects_rowsum():
+    R, A, P = triple("poisson2d64")
+    with pytest.raises(RuntimeError):
+        L.rap_mixed_host(R, A, P, rowsum=True, precision=L.MIXED64)
+
+
 def test_mixed_precision_requires_fp32_operands():
     R, A, P = triple("poisson2d64")
     dR, dA, dP = (L.DeviceCSR.upload(m) for m in (R, A, P))
@@ -212,18 +233,20 @@ def _batch_problems():
     return probs
 
 
-@pytest.mark.parametrize("mixed", [False, True])
-def test_batch_graph_replays(mixed):
+@pytest.mark.parametrize("prec", ["fp64", "mixed", "mixed64"])
+def test_batch_graph_replays(prec):
     probs = _batch_problems()
-    print(f"batch of {len(probs)} mixed={mixed}", flush=True)
+    print(f"batch of {len(probs)} precision={prec}", flush=True)
+    mixed = prec != "fp64"
     vt = L.F32 if mixed else L.F64
     rnd = (lambda m: m.with_vals(m.vals.astype(np.float32).astype(np.float64))) if mixed else (lambda m: m)
     dev = [[L.DeviceCSR.upload(rnd(m), vtype=vt) for m in t] for t in probs]
     wants = [O.sp3mm_pair(*[rnd(m) for m in t]) for t in probs]
     scales = [O.abs_product(*[rnd(m) for m in t]) for t in probs]
-    b = L.Batch([d[0] for d in dev], [d[1] for d in dev], [d[2] for d in dev], mixed=mixed, graph=True)
+    code = {"fp64": L.FP64, "mixed": L.MIXED, "mixed64": L.MIXED64}[prec]
+    b = L.Batch([d[0] for d in dev], [d[1] for d in dev], [d[2] for d in dev], graph=True, precision=code)
     assert b.info()["graph"]
-    tol = 1e-10 if mixed else 1e-12
+    tol = 1e-10 if prec == "mixed" else 1e-12
 
     def check_all():
         for i, want in enumerate(wants):
