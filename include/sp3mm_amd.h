@@ -257,7 +257,7 @@ int sp3mm_symbolic(const sp3mm_dmat *A, const sp3mm_dmat *B, const sp3mm_dmat *C
  * reported as an error) with one graph launch or a plain launch sequence, and
  * returns the device time in *ms.  result() copies product i into a new
  * device matrix.  Direct path; FP64 operands, or FP32 operands with
- * opts->precision == SP3MM_PREC_MIXED. */
+ * opts->precision == SP3MM_PREC_MIXED or SP3MM_PREC_MIXED64. */
 typedef struct sp3mm_batch sp3mm_batch;
 int sp3mm_batch_create(int32_t count, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P,
                        const sp3mm_opts *opts, int32_t use_graph, sp3mm_batch **out);
@@ -265,6 +265,29 @@ int sp3mm_batch_run(sp3mm_batch *b, int32_t use_graph, double *ms);
 int sp3mm_batch_result(sp3mm_batch *b, int32_t i, sp3mm_dmat *out);
 int sp3mm_batch_info(sp3mm_batch *b, double *ms_pack, double *ms_build, int64_t *nnz, int32_t *has_graph);
 void sp3mm_batch_free(sp3mm_batch *b);
+
+/* Scheduling of the one-pass direct kernels.  These parameters pick table sizes,
+ * group shapes and the spread of work over lanes; results are the same up to
+ * FP64 summation order. */
+typedef struct {
+    int32_t u1, u2;      /* left entries each lane holds in stage 1 / stage 2: 2 or 4 */
+    int32_t subwave_h;   /* largest stage-1 table run by 32-lane groups (two rows per
+                            wavefront); 0, or a power of two in [64, 4096] */
+    int32_t slog_cap;    /* largest lane sub-group (log2) inside 32-lane groups, 0..5 */
+    int32_t h1_shift;    /* stage-1 table = bin upper bound >> h1_shift, 0..2 (a row
+                            whose table fills is redone by the long-row kernel) */
+    int32_t h2_div;      /* stage-2 table = max(64, stage-1 table / h2_div): 1, 2, 4, 8 */
+} sp3mm_tuning;
+int sp3mm_get_tuning(sp3mm_tuning *t);
+/* NULL restores the defaults. */
+int sp3mm_set_tuning(const sp3mm_tuning *t);
+/* Autotuner: times the direct product R*A*P (opts->precision as in sp3mm_rap)
+ * under candidate schedules, `reps` products each, and installs the fastest
+ * for later calls; *best receives it, *ms_best and *ms_default the median
+ * device times of the chosen and of the starting schedule. */
+int sp3mm_autotune(const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P,
+                   const sp3mm_opts *opts, int32_t reps, sp3mm_tuning *best,
+                   double *ms_best, double *ms_default);
 
 /* Device memory from the library's pool (e.g. for sp3mm_opts.rowsum), its
  * release, and a blocking device-to-host copy. */

@@ -303,6 +303,75 @@ int sp3mm_batch_info(sp3mm_batch *b, double *ms_pack, double *ms_build, int64_t 
 
 void sp3mm_batch_free(sp3mm_batch *b) { batch_free(b); }
 
+static void to_c(const Tuning &t, sp3mm_tuning &o)
+{
+    o.u1 = t.u1;
+    o.u2 = t.u2;
+    o.subwave_h = t.subwave_h;
+    o.slog_cap = t.slog_cap;
+    o.h1_shift = t.h1_shift;
+    o.h2_div = t.h2_div;
+}
+
+int sp3mm_get_tuning(sp3mm_tuning *t)
+{
+    clear_error();
+    Context *c = ctx();
+    if (!c || !t) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    to_c(c->tune, *t);
+    return 0;
+}
+
+int sp3mm_set_tuning(const sp3mm_tuning *t)
+{
+    clear_error();
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    Tuning n{};
+    if (t) {
+        n.u1 = t->u1;
+        n.u2 = t->u2;
+        n.subwave_h = t->subwave_h;
+        n.slog_cap = t->slog_cap;
+        n.h1_shift = t->h1_shift;
+        n.h2_div = t->h2_div;
+        if (!tuning_valid(n)) {
+            set_error("set_tuning: parameter out of range");
+            return -1;
+        }
+    }
+    c->tune = n;
+    return 0;
+}
+
+int sp3mm_autotune(const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P, const sp3mm_opts *opts,
+                   int32_t reps, sp3mm_tuning *best, double *ms_best, double *ms_default)
+{
+    clear_error();
+    if (!R || !A || !P) {
+        set_error("autotune: null argument");
+        return -1;
+    }
+    sp3mm_opts o{};
+    if (opts) o = *opts;
+    o.path = SP3MM_PATH_DIRECT;
+    o.rowsum = nullptr;
+    o.timing = 0;
+    // one checked product first: validates the operands and warms the pool
+    sp3mm_dmat C{};
+    if (sp3mm_rap(R, A, P, &C, &o, nullptr)) return -1;
+    sp3mm_dmat_free(&C);
+    Tuning t{};
+    double mb = 0, md = 0;
+    if (autotune(*R, *A, *P, o.precision, o.guard, reps, t, mb, md)) return -1;
+    if (best) to_c(t, *best);
+    if (ms_best) *ms_best = mb;
+    if (ms_default) *ms_default = md;
+    return 0;
+}
+
 void *sp3mm_device_alloc(int64_t bytes)
 {
     Context *c = ctx();

@@ -4,6 +4,7 @@
 // scan + re-binning by exact sizes -> per-bin numeric kernels, all on one HIP
 // stream.  The host reads small counters back twice per product (bin
 // populations, then output size) to size grids and the output arrays.
+#include <algorithm>
 #include <cstdarg>
 #include <cstdlib>
 #include <cstdio>
@@ -95,6 +96,33 @@ void Pool::trim()
 static Context *g_ctx = nullptr;
 static std::mutex g_ctx_mu;
 
+int tuning_valid(const Tuning &t)
+{
+    const bool pow2h = t.subwave_h == 0 || (t.subwave_h >= 64 && t.subwave_h <= 4096 &&
+                                            (t.subwave_h & (t.subwave_h - 1)) == 0);
+    return (t.u1 == 2 || t.u1 == 4) && (t.u2 == 2 || t.u2 == 4) && pow2h && t.slog_cap >= 0 &&
+           t.slog_cap <= 5 && t.h1_shift >= 0 && t.h1_shift <= 2 &&
+           (t.h2_div == 1 || t.h2_div == 2 || t.h2_div == 4 || t.h2_div == 8);
+}
+
+// initial scheduling: defaults, overridden by SP3MM_FUSED_U ("ab", a and b in
+// {2,4}), SP3MM_SUBWAVE_H, SP3MM_SLOG_MAX, SP3MM_H1_SHIFT, SP3MM_H2_DIV
+static void tuning_from_env(Tuning &t)
+{
+    Tuning e = t;
+    if (const char *v = getenv("SP3MM_FUSED_U")) {
+        if (strlen(v) >= 2) {
+            e.u1 = v[0] - '0';
+            e.u2 = v[1] - '0';
+        }
+    }
+    if (const char *v = getenv("SP3MM_SUBWAVE_H")) e.subwave_h = atoi(v);
+    if (const char *v = getenv("SP3MM_SLOG_MAX")) e.slog_cap = atoi(v);
+    if (const char *v = getenv("SP3MM_H1_SHIFT")) e.h1_shift = atoi(v);
+    if (const char *v = getenv("SP3MM_H2_DIV")) e.h2_div = atoi(v);
+    if (tuning_valid(e)) t = e;
+}
+
 int init_device(int dev)
 {
     std::lock_guard<std::mutex> lk(g_ctx_mu);
@@ -122,6 +150,7 @@ int init_device(int dev)
         const size_t v = (size_t)strtoull(cap, nullptr, 10);
         if (v >= 16384 && v < c->max_lds) c->max_lds = v;
     }
+    tuning_from_env(c->tune);
     SP3MM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     SP3MM_HIP(hipHostMalloc((void **)&c->host_small, 4096, hipHostMallocDefault));
     g_ctx = c;
@@ -763,25 +792,21 @@ struct FusedCfg {
     int G, H1, H2;
 };
 
-// bins by the upper bound of the R*A row: table 1 of H1 >= ub slots, table 2
-// of H1/4 (>= 64) slots; rows above the largest H1 that fits go to the
-// long-row kernels
-// largest stage-1 table handled by half-wavefront groups (two rows per
-// wavefront); tuning override SP3MM_SUBWAVE_H (0 disables)
-static int sub_wave_h()
-{
-    const char *e = getenv("SP3MM_SUBWAVE_H");
-    return e ? atoi(e) : 256;
-}
-
-static void fused_bins(size_t max_lds, bool mixed, BinThresholds &thr, std::vector<FusedCfg> &cfg)
+// bins by the upper bound of the R*A row: bin b takes rows with ub <= v_b and
+// runs them with a stage-1 table of H1 = v_b >> h1_shift slots (with shift 0
+// the table cannot fill; a full table sends the row to the retry list) and a
+// stage-2 table of max(64, H1 / h2_div) slots; 32-lane groups up to
+// H1 = subwave_h, wavefronts up to 512 slots, workgroups beyond.  Rows above
+// the largest bin whose tables fit in LDS go to the long-row kernels.
+static void fused_bins(size_t max_lds, bool mixed, const Tuning &tu, BinThresholds &thr,
+                       std::vector<FusedCfg> &cfg)
 {
     thr.nb = 0;
     cfg.clear();
     for (long long v = 64; thr.nb < kMaxBins - 1; v *= 2) {
-        const int H1 = (int)v;
-        const int H2 = H1 / 4 > 64 ? H1 / 4 : 64;
-        const int G = H1 <= sub_wave_h() ? 32 : H1 <= 512 ? 64 : 256;
+        const int H1 = (int)std::max<long long>(64, v >> tu.h1_shift);
+        const int H2 = std::max(64, H1 / tu.h2_div);
+        const int G = H1 <= tu.subwave_h ? 32 : H1 <= 512 ? 64 : 256;
         if ((mixed ? mixed_lds(G, H1, H2) : fused_lds(G, H1, H2)) > max_lds) break;
         thr.v[thr.nb++] = v;
         cfg.push_back({G, H1, H2});
@@ -799,6 +824,7 @@ struct FusedPlan {
     bool f32 = false;    // FP32 operands (mixed, or FP64 accumulation of FP32 inputs)
     double guard = 1e-2;
     double *rowsum = nullptr;
+    Tuning tu{};
     BinThresholds thr{};
     std::vector<FusedCfg> cfg;
     std::vector<int> cnt, fcnt;
@@ -831,7 +857,7 @@ struct FusedPlan {
             launch_direct_fused(f.G, f.H1, f.H2, slog1, slog2, r(), a(), p(), lists.as<int32_t>() + (int64_t)i * M,
                                 d()->counts + i, cnt[i], rown.as<int32_t>(), slotn.as<int32_t>() + nbase[i],
                                 scol.as<int32_t>() + soff[i], sval.as<double>() + soff[i], retry.as<int32_t>(),
-                                &d()->nretry, &d()->err, max_blocks(c), s, f32);
+                                &d()->nretry, &d()->err, max_blocks(c), s, f32, tu);
     }
 
     void launch_tail(Context *c, hipStream_t s)
@@ -871,7 +897,7 @@ struct FusedPlan {
     {
         hipStream_t s = c->stream;
         const int64_t M = R.M;
-        fused_bins(c->max_lds, mixed, thr, cfg);
+        fused_bins(c->max_lds, mixed, tu, thr, cfg);
         ctl = Buf(sizeof(Ctl));
         ub = Buf((size_t)(M + 1) * 8);
         rown = Buf((size_t)(M + 1) * 4);
@@ -1024,6 +1050,7 @@ static int rap_direct_fused(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3m
     plan.P = P;
     plan.mixed = prec == SP3MM_PREC_MIXED;
     plan.f32 = prec != SP3MM_PREC_FP64;
+    plan.tu = c->tune;
     plan.guard = guard;
     plan.rowsum = rowsum;
     if (plan.build(c, tm)) {
@@ -1091,6 +1118,74 @@ int rap_direct_mixed(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat 
         return -1;
     }
     return rap_direct_fused(R, A, P, C, st, prec, guard > 0 ? guard : 1e-2, rowsum);
+}
+
+// --------------------------------------------------------------------------
+// autotuner: times the direct product of the caller's operands under candidate
+// schedules and installs the fastest in the context.  Coordinate descent from
+// the current setting, one parameter at a time; `reps` timed products per
+// candidate after one warm-up (device time by HIP events, median); a candidate
+// is taken only when it is at least 1% faster.
+// --------------------------------------------------------------------------
+int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int prec, double guard, int reps,
+             Tuning &best, double &ms_best, double &ms_default)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    reps = std::min(std::max(reps, 1), 50);
+    const Tuning saved = c->tune;
+    auto measure = [&](const Tuning &t, double &ms) -> int {
+        c->tune = t;
+        std::vector<double> v;
+        for (int k = 0; k <= reps; k++) {
+            sp3mm_dmat C{};
+            sp3mm_stats st{};
+            const int rc = rap_direct_fused(R, A, P, C, &st, prec, guard > 0 ? guard : 1e-2, nullptr);
+            dmat_release(C);
+            if (rc) return -1;
+            if (k > 0) v.push_back(st.ms_total);
+        }
+        std::sort(v.begin(), v.end());
+        ms = v[v.size() / 2];
+        return 0;
+    };
+    Tuning cur = saved;
+    double cur_ms = 0;
+    if (measure(cur, cur_ms)) {
+        c->tune = saved;
+        return -1;
+    }
+    ms_default = cur_ms;
+    struct Axis {
+        int Tuning::*field;
+        std::vector<int> values;
+    };
+    const Axis axes[] = {
+        {&Tuning::subwave_h, {0, 128, 256, 512}}, {&Tuning::u1, {2, 4}},       {&Tuning::u2, {2, 4}},
+        {&Tuning::slog_cap, {1, 2, 3, 5}},          {&Tuning::h1_shift, {0, 1}}, {&Tuning::h2_div, {2, 4, 8}},
+    };
+    for (const Axis &ax : axes) {
+        for (int v : ax.values) {
+            if (cur.*ax.field == v) continue;
+            Tuning t = cur;
+            t.*ax.field = v;
+            if (!tuning_valid(t)) continue;
+            double ms = 0;
+            if (measure(t, ms)) {
+                c->tune = saved;
+                return -1;
+            }
+            if (ms < 0.99 * cur_ms) {
+                cur = t;
+                cur_ms = ms;
+            }
+        }
+    }
+    c->tune = cur;
+    best = cur;
+    ms_best = cur_ms;
+    return 0;
 }
 
 // --------------------------------------------------------------------------
@@ -1196,6 +1291,7 @@ void *batch_create(int count, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp
     b->plan.P = b->Pb;
     b->plan.mixed = prec == SP3MM_PREC_MIXED;
     b->plan.f32 = mixed;
+    b->plan.tu = c->tune;
     b->plan.guard = guard > 0 ? guard : 1e-2;
     if (b->plan.build(c, tm)) {
         b->plan.release_all();

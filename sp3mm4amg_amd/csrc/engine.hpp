@@ -50,6 +50,8 @@ struct Context {
     size_t max_lds = 65536;
     size_t lds_device = 65536;
     Pool pool;
+    // scheduling of the one-pass direct kernels (sp3mm_set_tuning / sp3mm_autotune)
+    Tuning tune;
     // small pinned host mirror for counters read back between phases
     long long *host_small = nullptr;
     std::recursive_mutex mu;
@@ -83,6 +85,9 @@ void dmat_release(sp3mm_dmat &m);
 void *batch_create(int count, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P, int prec,
                    double guard, bool graph);
 int batch_run(void *b, bool use_graph, double *ms);
+int tuning_valid(const Tuning &t);
+int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int prec, double guard, int reps,
+             Tuning &best, double &ms_best, double &ms_default);
 int batch_result(void *b, int i, sp3mm_dmat &out);
 void batch_info(void *b, double *ms_pack, double *ms_build, int64_t *nnz, int32_t *has_graph);
 void batch_free(void *b);

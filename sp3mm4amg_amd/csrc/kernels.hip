@@ -1139,33 +1139,17 @@ static FusedFn fused_variant(int u1, int u2)
     return tab[u1][u2];
 }
 
-// tuning override "ab": a, b in {1,2,4} = left entries per lane in stage 1, 2
-static int ulog_env(int pos, int dflt)
-{
-    const char *e = getenv("SP3MM_FUSED_U");
-    if (!e || (int)strlen(e) < 2) return dflt;
-    const int u = e[pos] - '0';
-    return u == 2 ? 0 : u == 4 ? 1 : dflt;
-}
-
-// tuning override: largest sub-group (log2 lanes per right row) for groups
-// smaller than a wavefront
-static int slog_cap_env(int dflt)
-{
-    const char *e = getenv("SP3MM_SLOG_MAX");
-    return e && *e >= '0' && *e <= '6' ? *e - '0' : dflt;
-}
-
 void launch_direct_fused(int G, int H1, int H2, int slog1, int slog2, const SpOperand &R, const SpOperand &A,
                          const SpOperand &P, const int32_t *rows, const int *count, int rows_host,
                          int32_t *rownnz, int32_t *slotn, int32_t *scol, double *sval, int32_t *retry,
-                         int *nretry, int *err, int max_blocks, hipStream_t st, bool f32in)
+                         int *nretry, int *err, int max_blocks, hipStream_t st, bool f32in,
+                         const Tuning &tu)
 {
     const unsigned grid = grid_for(rows_host, G, max_blocks);
     const size_t lds = fused_lds(G, H1, H2);
-    const int u1 = ulog_env(0, 1), u2 = ulog_env(1, 1);
+    const int u1 = tu.u1 == 2 ? 0 : 1, u2 = tu.u2 == 2 ? 0 : 1;
     int cap = ilog2((unsigned)G);
-    if (G < 64) cap = std::min(cap, slog_cap_env(cap));
+    if (G < 64) cap = std::min(cap, std::max(0, tu.slog_cap));
     slog1 = std::min(slog1, cap);
     slog2 = std::min(slog2, cap);
     const FusedFn fn = f32in ? (G == 32   ? fused_variant<32, float>(u1, u2)

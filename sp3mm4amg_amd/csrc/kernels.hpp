@@ -72,10 +72,22 @@ void launch_dense_direct(bool num, int blocks, const SpOperand &R, const SpOpera
                          int *err, hipStream_t st, bool f32in = false, double *rowsum = nullptr);
 
 size_t fused_lds(int G, int H1, int H2);
+// Scheduling parameters of the one-pass direct kernels.  They decide table
+// sizes, group shapes and how work is spread over lanes; results are the same
+// up to FP64 summation order.
+struct Tuning {
+    int u1 = 4, u2 = 4;     // left entries held per lane in stage 1 / stage 2 (2 or 4)
+    int subwave_h = 256;    // largest stage-1 table run by 32-lane groups (0: none)
+    int slog_cap = 5;       // largest lane sub-group (log2) inside 32-lane groups
+    int h1_shift = 0;       // stage-1 table = bin upper bound >> h1_shift (a full table -> retry)
+    int h2_div = 4;         // stage-2 table = max(64, H1 / h2_div)
+};
+
 void launch_direct_fused(int G, int H1, int H2, int slog1, int slog2, const SpOperand &R, const SpOperand &A,
                          const SpOperand &P, const int32_t *rows, const int *count, int rows_host,
                          int32_t *rownnz, int32_t *slotn, int32_t *scol, double *sval, int32_t *retry,
-                         int *nretry, int *err, int max_blocks, hipStream_t st, bool f32in = false);
+                         int *nretry, int *err, int max_blocks, hipStream_t st, bool f32in,
+                         const Tuning &tu);
 void launch_gather_rows(const int32_t *rows, const int *count, int rows_host, const int32_t *slotn, int cap,
                         const int32_t *scol, const double *sval, const int64_t *crp, int32_t *ccol, double *cval,
                         int64_t out_cap, int *err, int max_blocks, hipStream_t st);

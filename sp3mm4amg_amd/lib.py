@@ -67,6 +67,14 @@ class Stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class Tuning(C.Structure):
+    _fields_ = [("u1", C.c_int32), ("u2", C.c_int32), ("subwave_h", C.c_int32),
+                ("slog_cap", C.c_int32), ("h1_shift", C.c_int32), ("h2_div", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 SPMM_NAMES = ["spmmSerial_", "spmmRowByRow_", "spmmRowByRow1DBlocks_", "spmmRowByRow2DBlocks_",
               "spmmRowByRow2DBlocksAllocated_", "spmmRowByRow_SymbNum_",
               "spmmRowByRow1DBlocks_SymbNum_", "spmmRowByRow2DBlocks_SymbNum_"]
@@ -128,6 +136,10 @@ def lib():
         L.sp3mm_batch_info.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_int64), P(C.c_int32)]
         L.sp3mm_batch_free.argtypes = [C.c_void_p]
         L.sp3mm_batch_free.restype = None
+        L.sp3mm_get_tuning.argtypes = [P(Tuning)]
+        L.sp3mm_set_tuning.argtypes = [P(Tuning)]
+        L.sp3mm_autotune.argtypes = [P(DMat), P(DMat), P(DMat), P(Opts), C.c_int32, P(Tuning),
+                                     P(C.c_double), P(C.c_double)]
         L.sp3mm_device_alloc.argtypes = [C.c_int64]
         L.sp3mm_device_alloc.restype = C.c_void_p
         L.sp3mm_device_free.argtypes = [C.c_void_p]
@@ -322,6 +334,34 @@ def rap_mixed_host(R: CSR, A: CSR, P: CSR, guard: float = 0.0, rowsum: bool = Fa
     if rs:
         rs.free()
     return res, st, sums
+
+
+def get_tuning() -> dict:
+    t = Tuning()
+    _check(lib().sp3mm_get_tuning(C.byref(t)), "sp3mm_get_tuning")
+    return t.as_dict()
+
+
+def set_tuning(t: dict | None) -> None:
+    """Install a schedule for the direct kernels (None restores the defaults)."""
+    if t is None:
+        _check(lib().sp3mm_set_tuning(None), "sp3mm_set_tuning")
+        return
+    cur = get_tuning()
+    cur.update(t)
+    _check(lib().sp3mm_set_tuning(C.byref(Tuning(**cur))), "sp3mm_set_tuning")
+
+
+def autotune(R: DeviceCSR, A: DeviceCSR, P: DeviceCSR, precision: int = FP64, guard: float = 0.0,
+             reps: int = 3):
+    """Time candidate schedules of the direct product and install the fastest;
+    returns (schedule, ms_best, ms_default)."""
+    o = Opts(DIRECT, precision, guard, 0, 0, None)
+    t = Tuning()
+    mb, md = C.c_double(), C.c_double()
+    _check(lib().sp3mm_autotune(C.byref(R.d), C.byref(A.d), C.byref(P.d), C.byref(o), int(reps),
+                                C.byref(t), C.byref(mb), C.byref(md)), "sp3mm_autotune")
+    return t.as_dict(), mb.value, md.value
 
 
 def copy_probe(nbytes: int, reps: int = 20) -> float:

@@ -223,6 +223,57 @@ def test_mixed_precision_requires_fp32_operands():
 
 
 # ---------------------------------------------------------------------------
+# schedules: every tuning gives the oracle's pattern and values
+SCHEDULES = [
+    {"subwave_h": 0},                                   # wavefront groups only
+    {"subwave_h": 512, "u1": 2, "u2": 2, "slog_cap": 1},
+    {"h1_shift": 1, "h2_div": 8},                       # tables that fill -> retry path
+    {"h1_shift": 2, "h2_div": 1, "u1": 2, "u2": 4},
+]
+
+
+@pytest.mark.parametrize("sched", range(len(SCHEDULES)))
+@pytest.mark.parametrize("name", ["poisson3d32", "irregular3000"])
+def test_schedules_keep_parity(name, sched):
+    R, A, P = triple(name)
+    print(f"schedule {SCHEDULES[sched]} on {name}", flush=True)
+    want = O.sp3mm_pair(R, A, P)
+    bound = O.error_bound(R, A, P, want)
+    L.set_tuning(SCHEDULES[sched])
+    try:
+        for path in (L.DIRECT,):
+            got, st = L.rap_host(R, A, P, path=path)
+            check(got, want, None, bound=bound)
+        R32, A32, P32 = _rounded32(R), _rounded32(A), _rounded32(P)
+        want32 = O.sp3mm_pair(R32, A32, P32)
+        got, st, _ = L.rap_mixed_host(R32, A32, P32, precision=L.MIXED64)
+        check(got, want32, None, bound=O.error_bound(R32, A32, P32, want32))
+    finally:
+        L.set_tuning(None)
+
+
+def test_autotune_installs_a_valid_schedule():
+    R, A, P = triple("poisson3d32")
+    d = [L.DeviceCSR.upload(m) for m in (R, A, P)]
+    default = L.get_tuning()
+    best, ms_best, ms_default = L.autotune(*d, reps=2)
+    print(f"autotune: {best} {ms_best:.4f} ms (start {ms_default:.4f} ms)")
+    assert ms_best <= ms_default
+    assert L.get_tuning() == best
+    out, _ = L.rap(*d, path=L.DIRECT)
+    got = out.download()
+    out.free()
+    want = O.sp3mm_pair(R, A, P)
+    check(got, want, O.abs_product(R, A, P))
+    L.set_tuning(None)
+    assert L.get_tuning() == default
+    with pytest.raises(RuntimeError):
+        L.set_tuning({"u1": 3})
+    for m in d:
+        m.free()
+
+
+# ---------------------------------------------------------------------------
 # batches: hierarchy levels + uneven independent problems in one product
 def _batch_problems():
     probs = []
