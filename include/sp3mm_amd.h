@@ -273,7 +273,9 @@ typedef struct {
     int32_t u1, u2;      /* left entries each lane holds in stage 1 / stage 2: 2 or 4 */
     int32_t subwave_h;   /* largest stage-1 table run by 32-lane groups (two rows per
                             wavefront); 0, or a power of two in [64, 4096] */
-    int32_t slog_cap;    /* largest lane sub-group (log2) inside 32-lane groups, 0..5 */
+    int32_t slog1_cap;   /* largest lane sub-group (log2) per left entry in stage 1, 0..8
+                            (the sub-group never exceeds the row's group) */
+    int32_t slog2_cap;   /* the same for stage 2 */
     int32_t h1_shift;    /* stage-1 table = bin upper bound >> h1_shift, 0..2 (a row
                             whose table fills is redone by the long-row kernel) */
     int32_t h2_div;      /* stage-2 table = max(64, stage-1 table / h2_div): 1, 2, 4, 8 */

@@ -308,7 +308,8 @@ static void to_c(const Tuning &t, sp3mm_tuning &o)
     o.u1 = t.u1;
     o.u2 = t.u2;
     o.subwave_h = t.subwave_h;
-    o.slog_cap = t.slog_cap;
+    o.slog1_cap = t.slog1_cap;
+    o.slog2_cap = t.slog2_cap;
     o.h1_shift = t.h1_shift;
     o.h2_div = t.h2_div;
 }
@@ -334,7 +335,8 @@ int sp3mm_set_tuning(const sp3mm_tuning *t)
         n.u1 = t->u1;
         n.u2 = t->u2;
         n.subwave_h = t->subwave_h;
-        n.slog_cap = t->slog_cap;
+        n.slog1_cap = t->slog1_cap;
+        n.slog2_cap = t->slog2_cap;
         n.h1_shift = t->h1_shift;
         n.h2_div = t->h2_div;
         if (!tuning_valid(n)) {

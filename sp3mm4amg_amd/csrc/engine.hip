@@ -100,13 +100,14 @@ int tuning_valid(const Tuning &t)
 {
     const bool pow2h = t.subwave_h == 0 || (t.subwave_h >= 64 && t.subwave_h <= 4096 &&
                                             (t.subwave_h & (t.subwave_h - 1)) == 0);
-    return (t.u1 == 2 || t.u1 == 4) && (t.u2 == 2 || t.u2 == 4) && pow2h && t.slog_cap >= 0 &&
-           t.slog_cap <= 5 && t.h1_shift >= 0 && t.h1_shift <= 2 &&
+    return (t.u1 == 2 || t.u1 == 4) && (t.u2 == 2 || t.u2 == 4) && pow2h && t.slog1_cap >= 0 &&
+           t.slog1_cap <= 8 && t.slog2_cap >= 0 && t.slog2_cap <= 8 && t.h1_shift >= 0 && t.h1_shift <= 2 &&
            (t.h2_div == 1 || t.h2_div == 2 || t.h2_div == 4 || t.h2_div == 8);
 }
 
 // initial scheduling: defaults, overridden by SP3MM_FUSED_U ("ab", a and b in
-// {2,4}), SP3MM_SUBWAVE_H, SP3MM_SLOG_MAX, SP3MM_H1_SHIFT, SP3MM_H2_DIV
+// {2,4}), SP3MM_SUBWAVE_H, SP3MM_SLOG_MAX (both stages), SP3MM_H1_SHIFT,
+// SP3MM_H2_DIV
 static void tuning_from_env(Tuning &t)
 {
     Tuning e = t;
@@ -117,7 +118,7 @@ static void tuning_from_env(Tuning &t)
         }
     }
     if (const char *v = getenv("SP3MM_SUBWAVE_H")) e.subwave_h = atoi(v);
-    if (const char *v = getenv("SP3MM_SLOG_MAX")) e.slog_cap = atoi(v);
+    if (const char *v = getenv("SP3MM_SLOG_MAX")) e.slog1_cap = e.slog2_cap = atoi(v);
     if (const char *v = getenv("SP3MM_H1_SHIFT")) e.h1_shift = atoi(v);
     if (const char *v = getenv("SP3MM_H2_DIV")) e.h2_div = atoi(v);
     if (tuning_valid(e)) t = e;
@@ -1163,7 +1164,8 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
     };
     const Axis axes[] = {
         {&Tuning::subwave_h, {0, 128, 256, 512}}, {&Tuning::u1, {2, 4}},       {&Tuning::u2, {2, 4}},
-        {&Tuning::slog_cap, {1, 2, 3, 5}},          {&Tuning::h1_shift, {0, 1}}, {&Tuning::h2_div, {2, 4, 8}},
+        {&Tuning::slog1_cap, {1, 2, 3}},            {&Tuning::slog2_cap, {1, 2}},
+        {&Tuning::h1_shift, {0, 1}},                {&Tuning::h2_div, {2, 4, 8}},
     };
     for (const Axis &ax : axes) {
         for (int v : ax.values) {

@@ -544,8 +544,18 @@ __global__ void __launch_bounds__(256) k_direct_fused(SpOperand R, SpOperand A, 
     const VT *rval = static_cast<const VT *>(R.val);
     const VT *aval = static_cast<const VT *>(A.val);
     const VT *pval = static_cast<const VT *>(P.val);
-    for (int i = blockIdx.x * GPB + g; i < n; i += gridDim.x * GPB) {
-        const int r = rows[i];
+    // The row index and R's row bounds of the group's next row are loaded while
+    // the current row is processed (two dependent loads off the critical path).
+    const int stride = gridDim.x * GPB;
+    int i = blockIdx.x * GPB + g;
+    int r = i < n ? rows[i] : 0;
+    int64_t rb = 0, re = 0;
+    if (i < n) {
+        rb = R.rp[r];
+        re = R.rp[r + 1];
+    }
+    for (; i < n; i += stride) {
+        const int rn = i + stride < n ? rows[i + stride] : 0;
         for (int s = t; s < H1; s += G) {
             k1[s] = kEmpty;
             v1[s] = 0.0;
@@ -557,8 +567,9 @@ __global__ void __launch_bounds__(256) k_direct_fused(SpOperand R, SpOperand A, 
         Group<G>::sync();
         int c = 0;
         bool full1 = false, full2 = false;
-        expand_into<G, true, VT, VT, U1>(R.col, rval, R.rp[r], R.rp[r + 1], A.rp32, A.col, aval, slog1, k1, v1, m1,
-                                         sh1, c, full1);
+        expand_into<G, true, VT, VT, U1>(R.col, rval, rb, re, A.rp32, A.col, aval, slog1, k1, v1, m1, sh1, c,
+                                         full1);
+        const int64_t rbn = R.rp[rn], ren = R.rp[rn + 1];
         Group<G>::sync();
         const int nn1 = compact_table<G, double>(k1, v1, H1, ired);
         if (!full1)
@@ -587,6 +598,9 @@ __global__ void __launch_bounds__(256) k_direct_fused(SpOperand R, SpOperand A, 
             }
         }
         Group<G>::sync();
+        r = rn;
+        rb = rbn;
+        re = ren;
     }
 }
 
@@ -1148,10 +1162,9 @@ void launch_direct_fused(int G, int H1, int H2, int slog1, int slog2, const SpOp
     const unsigned grid = grid_for(rows_host, G, max_blocks);
     const size_t lds = fused_lds(G, H1, H2);
     const int u1 = tu.u1 == 2 ? 0 : 1, u2 = tu.u2 == 2 ? 0 : 1;
-    int cap = ilog2((unsigned)G);
-    if (G < 64) cap = std::min(cap, std::max(0, tu.slog_cap));
-    slog1 = std::min(slog1, cap);
-    slog2 = std::min(slog2, cap);
+    const int cap = ilog2((unsigned)G);
+    slog1 = std::min(slog1, std::min(cap, std::max(0, tu.slog1_cap)));
+    slog2 = std::min(slog2, std::min(cap, std::max(0, tu.slog2_cap)));
     const FusedFn fn = f32in ? (G == 32   ? fused_variant<32, float>(u1, u2)
                                 : G == 64 ? fused_variant<64, float>(u1, u2)
                                           : fused_variant<256, float>(u1, u2))

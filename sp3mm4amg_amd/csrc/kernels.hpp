@@ -78,7 +78,8 @@ size_t fused_lds(int G, int H1, int H2);
 struct Tuning {
     int u1 = 4, u2 = 4;     // left entries held per lane in stage 1 / stage 2 (2 or 4)
     int subwave_h = 256;    // largest stage-1 table run by 32-lane groups (0: none)
-    int slog_cap = 5;       // largest lane sub-group (log2) inside 32-lane groups
+    int slog1_cap = 8;      // largest lane sub-group (log2) per left entry, stage 1
+    int slog2_cap = 8;      // the same, stage 2
     int h1_shift = 0;       // stage-1 table = bin upper bound >> h1_shift (a full table -> retry)
     int h2_div = 4;         // stage-2 table = max(64, H1 / h2_div)
 };

@@ -263,8 +263,17 @@ __global__ void __launch_bounds__(256) k_direct_mixed(SpOperand R, SpOperand A, 
     const float *rv = static_cast<const float *>(R.val);
     const float *av = static_cast<const float *>(A.val);
     const float *pv = static_cast<const float *>(P.val);
-    for (int i = blockIdx.x * GPB + g; i < n; i += gridDim.x * GPB) {
-        const int r = rows[i];
+    // the next row's index and bounds are loaded while this row is processed
+    const int stride = gridDim.x * GPB;
+    int i = blockIdx.x * GPB + g;
+    int r = i < n ? rows[i] : 0;
+    int64_t rb = 0, re = 0;
+    if (i < n) {
+        rb = R.rp[r];
+        re = R.rp[r + 1];
+    }
+    for (; i < n; i += stride) {
+        const int rn = i + stride < n ? rows[i + stride] : 0;
         for (int s = t; s < H1; s += G) {
             k1[s] = kEmpty;
             h1[s] = 0.f;
@@ -282,13 +291,14 @@ __global__ void __launch_bounds__(256) k_direct_mixed(SpOperand R, SpOperand A, 
         bool full1 = false, full2 = false;
         // stage 1: row r of R*A; every FP32 product is split exactly by TwoProd
         expand_mixed<G, 4>(
-            R.col, R.rp[r], R.rp[r + 1], A.rp32, A.col, slog1, k1, m1, sh1, full1,
+            R.col, rb, re, A.rp32, A.col, slog1, k1, m1, sh1, full1,
             [&](int64_t e) { return rv[e]; }, [&](int q) { return av[q]; },
             [&](unsigned sl, float x, float w, int) {
                 const FF pr = two_prod(x, w);
                 ff_atomic_accum(h1 + sl, l1 + sl, pr);
                 atomicAdd(a1 + sl, fabsf(pr.h));
             });
+        const int64_t rbn = R.rp[rn], ren = R.rp[rn + 1];
         Group<G>::sync();
         const int nn1 = compact3<G>(k1, h1, l1, a1, H1, ired);
         // stage 2: forward into P; the diagonal and the row sum also in FP64
@@ -354,6 +364,9 @@ __global__ void __launch_bounds__(256) k_direct_mixed(SpOperand R, SpOperand A, 
             if (t == 0) rowsum[r] = tot;
         }
         Group<G>::sync();
+        r = rn;
+        rb = rbn;
+        re = ren;
     }
 }
 

@@ -226,7 +226,7 @@ def test_mixed_precision_requires_fp32_operands():
 # schedules: every tuning gives the oracle's pattern and values
 SCHEDULES = [
     {"subwave_h": 0},                                   # wavefront groups only
-    {"subwave_h": 512, "u1": 2, "u2": 2, "slog_cap": 1},
+    {"subwave_h": 512, "u1": 2, "u2": 2, "slog1_cap": 1, "slog2_cap": 0},
     {"h1_shift": 1, "h2_div": 8},                       # tables that fill -> retry path
     {"h1_shift": 2, "h2_div": 1, "u1": 2, "u2": 4},
 ]
