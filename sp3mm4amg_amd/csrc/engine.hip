@@ -144,6 +144,9 @@ int init_device(int dev)
     hipDeviceProp_t prop;
     SP3MM_HIP(hipGetDeviceProperties(&prop, dev));
     c->num_cu = prop.multiProcessorCount;
+    // host waits spin instead of sleeping: the pipeline reads small counters
+    // back between phases, and a blocking wait adds wake-up latency to each
+    if (hipSetDeviceFlags(hipDeviceScheduleSpin) != hipSuccess) (void)hipGetLastError();
     // dynamic LDS per workgroup the bin tables may use (SP3MM_LDS_CAP overrides)
     c->lds_device = prop.sharedMemPerBlock;
     c->max_lds = prop.sharedMemPerBlock;
@@ -154,6 +157,10 @@ int init_device(int dev)
     tuning_from_env(c->tune);
     SP3MM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     SP3MM_HIP(hipHostMalloc((void **)&c->host_small, 4096, hipHostMallocDefault));
+    if (hipHostGetDevicePointer((void **)&c->host_small_dev, c->host_small, 0) != hipSuccess) {
+        (void)hipGetLastError();
+        c->host_small_dev = nullptr;
+    }
     g_ctx = c;
     return 0;
 }
@@ -418,8 +425,14 @@ struct Timer {
 
 static int read_ctl(Context *c, const Ctl *d, Ctl &h)
 {
+    static_assert(sizeof(Ctl) % 4 == 0 && sizeof(Ctl) <= 4096, "control block fits the host mirror");
     SP3MM_HIP(hipGetLastError());  // a kernel launch that was rejected
-    SP3MM_HIP(hipMemcpyAsync(c->host_small, d, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+    if (c->host_small_dev) {
+        launch_export_words(d, c->host_small_dev, (int)(sizeof(Ctl) / 4), c->stream);
+        SP3MM_HIP(hipGetLastError());
+    } else {
+        SP3MM_HIP(hipMemcpyAsync(c->host_small, d, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
+    }
     SP3MM_HIP(hipStreamSynchronize(c->stream));
     memcpy(&h, c->host_small, sizeof(Ctl));
     return 0;

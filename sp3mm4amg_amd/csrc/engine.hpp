@@ -54,6 +54,7 @@ struct Context {
     Tuning tune;
     // small pinned host mirror for counters read back between phases
     long long *host_small = nullptr;
+    long long *host_small_dev = nullptr;  // its device-side address (kernels write it directly)
     std::recursive_mutex mu;
 };
 

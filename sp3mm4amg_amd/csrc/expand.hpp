@@ -1,0 +1,119 @@
+// expand.hpp -- expansion of one sparse row product into an LDS hash table,
+// shared by the SpGEMM, one-pass direct and mixed-precision kernels.
+#pragma once
+#include "device_common.hpp"
+
+namespace sp3mm {
+
+// Expands left entries [e0, e1) against the right operand B: for every left
+// entry e with column k = loadk(e) and every entry q of row k of B, the
+// product of loadl(e) and loadb(q) lands in the slot of key bcol[q].
+//
+// Lanes form sub-groups of S = 2^slog lanes; a sub-group owns one left entry
+// at a time and its lanes walk that entry's right row with stride S (S is
+// chosen near the typical right-row length, so one step covers most rows).
+// Every lane holds U left entries at once (entries g, g + ngrp, ... of the
+// pass), loads all their row bounds, then loads one right entry per left entry
+// and inserts the U products: a pass over U*ngrp left entries costs three
+// dependent memory round trips.  The first probe of every insert is one
+// compare-and-swap issued back to back with the others; only an insert whose
+// first slot holds another key enters the probing loop, so the common path has
+// no divergent loops.  A probe sequence is bounded by the table size: when the
+// table is full the product is dropped and `full` is set (the caller redoes
+// the row elsewhere).
+//
+// acc(hit, slot, left, right, col, fresh) is called by every lane of the group
+// for each of its U products of a step (hit false: no product, or not inserted
+// yet) and again from the probing loop; with hit set it accumulates the
+// product into the occupied slot, fresh marking the insert that claimed it.
+template <int G, int U, typename LoadK, typename LoadL, typename LoadB, typename Acc>
+__device__ __forceinline__ void expand_gen(int64_t e0, int64_t e1, LoadK loadk, LoadL loadl,
+                                           const int32_t *__restrict__ brp, const int32_t *__restrict__ bcol,
+                                           LoadB loadb, int slog, int *keys, unsigned mask, int shift, bool &full,
+                                           Acc acc)
+{
+    using LT = decltype(loadl(int64_t{}));
+    using BT = decltype(loadb(int{}));
+    constexpr int kLogG = G >= 256 ? 8 : G >= 128 ? 7 : G >= 64 ? 6 : G >= 32 ? 5 : 4;
+    if (slog > kLogG) slog = kLogG;  // a sub-group never spans more than the group
+    const int t = Group<G>::tid();
+    const int S = 1 << slog;
+    const int sub = t & (S - 1);
+    const int ngrp = G >> slog;
+    for (int64_t eb = e0 + (t >> slog); eb < e1; eb += (int64_t)ngrp * U) {
+        int q[U], qe[U];
+        LT lv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t e = eb + (int64_t)u * ngrp;
+            const bool ok = e < e1;
+            const int k = ok ? loadk(e) : 0;
+            lv[u] = ok ? loadl(e) : LT{};
+            q[u] = ok ? brp[k] + sub : 0;
+            qe[u] = ok ? brp[k + 1] : 0;
+        }
+        while (true) {
+            bool more = false;
+#pragma unroll
+            for (int u = 0; u < U; u++) more |= q[u] < qe[u];
+            if (!more) break;
+            int c[U], old[U];
+            unsigned sl[U];
+            BT w[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const bool ok = q[u] < qe[u];
+                c[u] = ok ? bcol[q[u]] : kEmpty;
+                w[u] = ok ? loadb(q[u]) : BT{};
+                q[u] += S;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                sl[u] = hslot(c[u], shift) & mask;
+                old[u] = c[u] != kEmpty ? atomicCAS(keys + sl[u], kEmpty, c[u]) : kEmpty;
+            }
+            bool pend = false;
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const bool hit = c[u] != kEmpty && (old[u] == kEmpty || old[u] == c[u]);
+                pend |= c[u] != kEmpty && !hit;
+                acc(hit, sl[u], lv[u], w[u], c[u], hit && old[u] == kEmpty);
+            }
+            if (pend) {
+                for (int u = 0; u < U; u++) {
+                    if (c[u] == kEmpty || old[u] == kEmpty || old[u] == c[u]) continue;
+                    unsigned s2 = sl[u];
+                    bool done = false;
+                    for (unsigned probe = 1; probe <= mask; probe++) {
+                        s2 = (s2 + 1) & mask;
+                        const int o = atomicCAS(keys + s2, kEmpty, c[u]);
+                        if (o == kEmpty || o == c[u]) {
+                            acc(true, s2, lv[u], w[u], c[u], o == kEmpty);
+                            done = true;
+                            break;
+                        }
+                    }
+                    if (!done) full = true;
+                }
+            }
+        }
+    }
+}
+
+// Appends, for every lane of the group with `claim` set, `slot` to the group's
+// list of claimed slots (list[n++] = slot): one LDS atomic per wavefront.
+template <int G>
+__device__ __forceinline__ void list_append(bool claim, unsigned slot, int *list, int *count)
+{
+    uint64_t wm = ballot(claim);
+    if constexpr (G < kWave) wm &= ((1ull << G) - 1) << Group<G>::lane0();
+    if (wm) {
+        const int leader = __ffsll((unsigned long long)wm) - 1;
+        int base = 0;
+        if (lane() == leader) base = atomicAdd(count, __popcll(wm));
+        base = __shfl(base, leader);
+        if (claim) list[base + lanes_below(wm)] = (int)slot;
+    }
+}
+
+}  // namespace sp3mm

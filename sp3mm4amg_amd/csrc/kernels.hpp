@@ -126,6 +126,7 @@ void launch_rp_shift(const int64_t *rp, int64_t n, int base, int64_t *out, hipSt
 void launch_f64_to_f32(const double *in, int64_t n, float *out, hipStream_t st);
 void launch_f32_to_f64(const float *in, int64_t n, double *out, hipStream_t st);
 void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
+void launch_export_words(const void *src, void *host_dst, int words, hipStream_t st);
 void launch_copy_probe(const void *in, void *out, int64_t bytes, hipStream_t st);
 
 }  // namespace sp3mm
