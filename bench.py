@@ -25,6 +25,18 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
 
+def committed_traffic(path=os.path.join(ROOT, "profiles", "pmc_direct_poisson3d128.md")):
+    """HBM bytes per product of the main direct kernel(s), from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (calibrated with a copy kernel)."""
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -101,10 +113,10 @@ def time_form(lib, dR, dA, dP, steps, warmup, **kw):
     return wall, float(np.mean(main_ms)), float(np.median(total_ms)), st
 
 
-def mixed_leg(lib, problems, steps, warmup, n=100):
+def mixed_leg(lib, problems, steps, warmup, tune, n=100):
     """Mixed precision on the 27-point random stencil (100^3, 3x3x3 aggregates):
-    FP32 operands + FP32-pair accumulation vs the FP64 product of the same
-    FP32-rounded operands."""
+    FP32 operands with FP32-pair accumulation (MIXED) and with FP64 accumulation
+    (MIXED64) against the FP64 product of the same FP32-rounded operands."""
     import numpy as np
     prob = problems.stencil27_sa(n, seed=3)
     R, A, P = prob["R"], prob["A"], prob["P"]
@@ -115,31 +127,37 @@ def mixed_leg(lib, problems, steps, warmup, n=100):
     out.free()
     mac1, mac2, nnz_ra = st_pair["mac1"], st_pair["mac2"], st_pair["nnz_mid"]
     flop = 2 * (mac1 + mac2)
-    w64, m64, dev64, st64 = time_form(lib, *d64, steps, warmup, path=lib.DIRECT)
-    wmx, mmx, devmx, stmx = time_form(lib, *d32, steps, warmup, path=lib.DIRECT, precision=lib.MIXED)
     ref, _ = lib.rap(*d64, path=lib.DIRECT)
-    got, _ = lib.rap(*d32, path=lib.DIRECT, precision=lib.MIXED)
-    a, b = ref.download(), got.download()
+    a = ref.download()
     ref.free()
-    got.free()
-    same = np.array_equal(a.irp, b.irp) and np.array_equal(a.ja, b.ja)
-    rel = np.abs(b.vals - a.vals) / np.maximum(np.abs(a.vals), 1e-300)
+    rec = {"workload": f"{prob['name']} Galerkin R*A*P, direct path", "flop": flop, "mac1": mac1,
+           "mac2": mac2, "nnz_RAP": a.nnz}
+    for name, d, prec, vb in (("fp64", d64, lib.FP64, 8), ("mixed", d32, lib.MIXED, 4),
+                              ("mixed64", d32, lib.MIXED64, 4)):
+        lib.set_tuning(None)
+        sched = None
+        if tune:
+            sched, _, _ = lib.autotune(*d, precision=prec, reps=3)
+        wall, main, dev, st = time_form(lib, *d, steps, warmup, path=lib.DIRECT, precision=prec)
+        got, _ = lib.rap(*d, path=lib.DIRECT, precision=prec)
+        b = got.download()
+        got.free()
+        same = np.array_equal(a.irp, b.irp) and np.array_equal(a.ja, b.ja)
+        rel = np.abs(b.vals - a.vals) / np.maximum(np.abs(a.vals), 1e-300)
+        rec[name] = {"ms_per_product": round(wall, 4), "device_ms_median": round(dev, 4),
+                     "main_kernel_ms": round(main, 4), "gflops": round(flop / wall / 1e6, 2),
+                     "bytes_alg": alg_bytes(R, A, P, mac1, mac2, nnz_ra, a.nnz, vb), "schedule": sched,
+                     "pattern_equal": bool(same),
+                     "max_rel_err_vs_fp64": float(rel.max()) if rel.size else 0.0,
+                     "median_rel_err_vs_fp64": float(np.median(rel)) if rel.size else 0.0}
+        if name == "mixed":
+            rec[name]["fp64_rows"] = int(st["guarded_rows"])
+    lib.set_tuning(None)
     for m in d64 + d32:
         m.free()
-    nnz_out = stmx["nnz_out"]
-    return {"workload": f"{prob['name']} Galerkin R*A*P, direct path",
-            "flop": flop, "mac1": mac1, "mac2": mac2, "nnz_RAP": nnz_out,
-            "fp64": {"ms_per_product": round(w64, 4), "device_ms_median": round(dev64, 4),
-                     "gflops": round(flop / w64 / 1e6, 2),
-                     "bytes_alg": alg_bytes(R, A, P, mac1, mac2, nnz_ra, nnz_out, 8)},
-            "mixed": {"ms_per_product": round(wmx, 4), "device_ms_median": round(devmx, 4),
-                      "gflops": round(flop / wmx / 1e6, 2),
-                      "bytes_alg": alg_bytes(R, A, P, mac1, mac2, nnz_ra, nnz_out, 4),
-                      "fp64_rows": int(stmx["guarded_rows"]),
-                      "pattern_equal": bool(same),
-                      "max_rel_err_vs_fp64": float(rel.max()) if rel.size else 0.0,
-                      "median_rel_err_vs_fp64": float(np.median(rel)) if rel.size else 0.0},
-            "speedup_mixed_vs_fp64": round(w64 / wmx, 3)}
+    rec["speedup_mixed_vs_fp64"] = round(rec["fp64"]["ms_per_product"] / rec["mixed"]["ms_per_product"], 3)
+    rec["speedup_mixed64_vs_fp64"] = round(rec["fp64"]["ms_per_product"] / rec["mixed64"]["ms_per_product"], 3)
+    return rec
 
 
 def batch_leg(lib, problems, reps):
@@ -158,13 +176,15 @@ def batch_leg(lib, problems, reps):
                        "(sides U{16..256}, PCG64 seed 5, 2x2 SA), direct path",
            "products": len(items)}
     flop = 0
-    for form, vt in (("fp64", lib.F64), ("mixed", lib.F32)):
+    lib.set_tuning(None)
+    for form, vt, prec in (("fp64", lib.F64, lib.FP64), ("mixed", lib.F32, lib.MIXED),
+                           ("mixed64", lib.F32, lib.MIXED64)):
         mats = []
         for (R, A, P) in items:
             if vt == lib.F32:
                 R, A, P = (m.with_vals(m.vals.astype(np.float32).astype(np.float64)) for m in (R, A, P))
             mats.append(tuple(lib.DeviceCSR.upload(m, vtype=vt) for m in (R, A, P)))
-        kw = {"path": lib.DIRECT} if vt == lib.F64 else {"path": lib.DIRECT, "precision": lib.MIXED}
+        kw = {"path": lib.DIRECT, "precision": prec}
         if form == "fp64":
             for (dR, dA, dP) in mats:
                 out, st = lib.rap(dR, dA, dP, path=lib.PAIR)
@@ -183,8 +203,8 @@ def batch_leg(lib, problems, reps):
         lib.lib().sp3mm_sync()
         loop_ms = (time.perf_counter() - t0) * 1e3 / reps
         # batched: one packed product captured as a graph
-        b = lib.Batch([m[0] for m in mats], [m[1] for m in mats], [m[2] for m in mats],
-                      mixed=(vt == lib.F32), graph=True)
+        b = lib.Batch([m[0] for m in mats], [m[1] for m in mats], [m[2] for m in mats], graph=True,
+                      precision=prec)
         b.run(True)
         dev = []
         t0 = time.perf_counter()
@@ -215,8 +235,9 @@ def batch_leg(lib, problems, reps):
                      "graph_build_ms": round(info["ms_build"], 3),
                      "max_rel_diff_batched_vs_single": worst}
     rec["flop"] = flop
-    rec["speedup_mixed_graph_vs_fp64_loop"] = round(rec["fp64"]["loop_ms_per_batch"]
-                                                    / rec["mixed"]["graph_ms_per_batch"], 3)
+    for form in ("fp64", "mixed", "mixed64"):
+        rec[f"speedup_{form}_graph_vs_fp64_loop"] = round(rec["fp64"]["loop_ms_per_batch"]
+                                                          / rec[form]["graph_ms_per_batch"], 3)
     return rec
 
 
@@ -230,6 +251,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="use the default schedule instead of autotuning each workload (untimed setup)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the product is sharded by rows "
@@ -253,6 +276,12 @@ def main():
     out.free()
     mac1, mac2, nnz_ra = st_pair["mac1"], st_pair["mac2"], st_pair["nnz_mid"]
     flop = 2 * (mac1 + mac2)
+    schedule, tune_s = None, 0.0
+    if not args.no_tune and path == lib.DIRECT:
+        t0 = time.perf_counter()
+        schedule, ms_best, ms_default = lib.autotune(dR, dA, dP, reps=3)
+        tune_s = time.perf_counter() - t0
+        log(f"autotune: {schedule} {ms_best:.4f} ms (default schedule {ms_default:.4f} ms) in {tune_s:.1f}s")
 
     for _ in range(args.warmup):
         out, st = lib.rap(dR, dA, dP, path=path)
@@ -294,9 +323,11 @@ def main():
         "config": {"workload": f"{prob['name']} Galerkin R*A*P, FP64, {args.path} path",
                    "rows_fine": A.M, "rows_coarse": R.M, "nnz_R": R.nnz, "nnz_A": A.nnz,
                    "nnz_P": P.nnz, "nnz_RA": nnz_ra, "nnz_RAP": nnz_out, "mac1": mac1,
-                   "mac2": mac2, "flop": flop},
+                   "mac2": mac2, "flop": flop, "schedule": schedule, "autotune_s": round(tune_s, 2)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": committed_traffic() if args.path == "direct" and args.n == 128 else None,
+                     "traffic_source": "profiles/pmc_direct_poisson3d128.md",
                      "kernel": f"numeric {args.path} kernel(s)", "kernel_ms": round(main_avg, 4),
                      "bytes_alg": bytes_alg},
         "product": {"device_ms_median": round(dev_med, 4),
@@ -306,7 +337,7 @@ def main():
     }
     if not args.no_mixed:
         log("mixed-precision leg (27-point stencil) ...")
-        rec["mixed_precision"] = mixed_leg(lib, problems, max(3, args.steps // 2), 1)
+        rec["mixed_precision"] = mixed_leg(lib, problems, max(3, args.steps // 2), 1, not args.no_tune)
     if not args.no_batch:
         log("batched leg (hierarchy + 256 problems) ...")
         rec["batched"] = batch_leg(lib, problems, max(3, args.steps // 4))

@@ -141,9 +141,9 @@ __host__ __device__ __forceinline__ int ilog2(unsigned v)
     return r;
 }
 
-// In-place compaction of the occupied slots of a table of H slots (H a multiple
-// of G) to its front, preserving slot order.  Values (if any) move with keys.
-// Returns the number of occupied slots.  `red` holds kWaves+1 ints.
+// In-place compaction of the occupied slots of a table of H slots to its
+// front, preserving slot order.  Values (if any) move with keys.  Returns the
+// number of occupied slots.  `red` holds kWaves+1 ints.
 template <int G, typename V>
 __device__ int compact_table(int *keys, V *vals, int H, int *red)
 {
@@ -151,9 +151,9 @@ __device__ int compact_table(int *keys, V *vals, int H, int *red)
     int base = 0;
     for (int s0 = 0; s0 < H; s0 += G) {
         const int s = s0 + t;
-        const int k = keys[s];
+        const int k = s < H ? keys[s] : kEmpty;
         V v{};
-        if (vals) v = vals[s];
+        if (vals && s < H) v = vals[s];
         const bool occ = k != kEmpty;
         const uint64_t m = Group<G>::part(ballot(occ));
         int pos = G >= kWave ? lanes_below(m) : __popcll(m & ((1ull << t) - 1));

@@ -809,7 +809,7 @@ struct FusedCfg {
 // bins by the upper bound of the R*A row: bin b takes rows with ub <= v_b and
 // runs them with a stage-1 table of H1 = v_b >> h1_shift slots (with shift 0
 // the table cannot fill; a full table sends the row to the retry list) and a
-// stage-2 table of max(64, H1 / h2_div) slots; 32-lane groups up to
+// stage-2 table of max(64, G, H1 / h2_div) slots; 32-lane groups up to
 // H1 = subwave_h, wavefronts up to 512 slots, workgroups beyond.  Rows above
 // the largest bin whose tables fit in LDS go to the long-row kernels.
 static void fused_bins(size_t max_lds, bool mixed, const Tuning &tu, BinThresholds &thr,
@@ -819,8 +819,10 @@ static void fused_bins(size_t max_lds, bool mixed, const Tuning &tu, BinThreshol
     cfg.clear();
     for (long long v = 64; thr.nb < kMaxBins - 1; v *= 2) {
         const int H1 = (int)std::max<long long>(64, v >> tu.h1_shift);
-        const int H2 = std::max(64, H1 / tu.h2_div);
         const int G = H1 <= tu.subwave_h ? 32 : H1 <= 512 ? 64 : 256;
+        // both tables hold a whole number of group-widths (the compaction and
+        // clearing loops step through them G slots at a time)
+        const int H2 = std::max(std::max(64, G), H1 / tu.h2_div);
         if ((mixed ? mixed_lds(G, H1, H2) : fused_lds(G, H1, H2)) > max_lds) break;
         thr.v[thr.nb++] = v;
         cfg.push_back({G, H1, H2});
@@ -866,7 +868,7 @@ struct FusedPlan {
                                 lists.as<int32_t>() + (int64_t)i * M, d()->counts + i, cnt[i], rown.as<int32_t>(),
                                 slotn.as<int32_t>() + nbase[i], scol.as<int32_t>() + soff[i],
                                 sval.as<double>() + soff[i], rowsum, retry.as<int32_t>(), &d()->nretry,
-                                fix.as<int32_t>() + nbase[i], d()->counts2 + i, &d()->err, max_blocks(c), s);
+                                fix.as<int32_t>() + nbase[i], d()->counts2 + i, &d()->err, max_blocks(c), s, tu);
         else
             launch_direct_fused(f.G, f.H1, f.H2, slog1, slog2, r(), a(), p(), lists.as<int32_t>() + (int64_t)i * M,
                                 d()->counts + i, cnt[i], rown.as<int32_t>(), slotn.as<int32_t>() + nbase[i],

@@ -505,19 +505,15 @@ __global__ void __launch_bounds__(256) k_direct(SpOperand R, SpOperand A, SpOper
 // ---------------------------------------------------------------------------
 // direct triple product in ONE pass over the operands
 // ---------------------------------------------------------------------------
-// Row r of R*A is accumulated in table 1 (H1 slots; with H1 >= the row's upper
-// bound it cannot fill) and forwarded into P through table 2 of H2 slots.
-// Every slot claimed in table 1 is appended to the row's claim list (one LDS
-// atomic per wavefront), so stage 2 walks exactly the row's entries and only
-// those slots are cleared afterwards: the cost of table 1 follows the row's
-// size, not the table's.  Table 2 is compacted, rank-sorted and written to the
-// row's slot of `cap` = H2 entries in a scratch area (slot i of the bin list ->
-// scratch + i*cap); the row length goes to rownnz[r] and slotn[i].  A row
-// whose table fills is marked slotn[i] = -1 and queued on `retry` for the
-// long-row kernels.  After the scan of rownnz, k_gather_rows moves every slot
-// to its final place.
+// Row r of R*A is accumulated in table 1 (H1 >= the row's upper bound, so it
+// cannot fill), compacted, and forwarded into P through table 2 of H2 slots.
+// The finished row is rank-sorted and written to its slot of `cap` = H2
+// entries in a scratch area (slot i of the bin list -> scratch + i*cap); its
+// length goes to rownnz[r] and slotn[i].  A row whose table 2 fills is marked
+// slotn[i] = -1 and queued on `retry` for the long-row kernels.  After the
+// scan of rownnz, k_gather_rows moves every slot to its final place.
 template <int G, int U1, int U2, typename VT = double>
-__global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand A, SpOperand P,
+__global__ void __launch_bounds__(256) k_direct_fused(SpOperand R, SpOperand A, SpOperand P,
                                                        const int32_t *__restrict__ rows,
                                                        const int *__restrict__ count, int H1, int H2,
                                                        int slog1, int slog2, int32_t *__restrict__ rownnz,
@@ -528,7 +524,6 @@ __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand 
 {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int GPB = 256 / G;
-    constexpr int KW = Group<G>::kWaves;
     const int g = threadIdx.x / G;
     const int t = Group<G>::tid();
     unsigned char *p = smem;
@@ -538,14 +533,11 @@ __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand 
     p += (size_t)GPB * H2 * 8;
     int *k1 = reinterpret_cast<int *>(p) + g * H1;
     p += (size_t)GPB * H1 * 4;
-    int *L1 = reinterpret_cast<int *>(p) + g * H1;
-    p += (size_t)GPB * H1 * 4;
     int *k2 = reinterpret_cast<int *>(p) + g * H2;
     p += (size_t)GPB * H2 * 4;
-    long long *red = reinterpret_cast<long long *>(p) + g * KW;
-    p += (size_t)GPB * KW * 8;
-    int *ired = reinterpret_cast<int *>(p) + g * (KW + 2);
-    int *cnt1 = ired + KW + 1;
+    long long *red = reinterpret_cast<long long *>(p) + g * Group<G>::kWaves;
+    p += (size_t)GPB * Group<G>::kWaves * 8;
+    int *ired = reinterpret_cast<int *>(p) + g * (Group<G>::kWaves + 1);
 
     const unsigned m1 = (unsigned)H1 - 1, m2 = (unsigned)H2 - 1;
     const int sh1 = 32 - ilog2((unsigned)H1), sh2 = 32 - ilog2((unsigned)H2);
@@ -553,12 +545,6 @@ __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand 
     const VT *rval = static_cast<const VT *>(R.val);
     const VT *aval = static_cast<const VT *>(A.val);
     const VT *pval = static_cast<const VT *>(P.val);
-    // table 1 starts empty; afterwards each row clears the slots it claimed
-    for (int s = t; s < H1; s += G) {
-        k1[s] = kEmpty;
-        v1[s] = 0.0;
-    }
-    if (t == 0) *cnt1 = 0;
     // The row index and R's row bounds of the group's next row are loaded while
     // the current row is processed (two dependent loads off the critical path).
     const int stride = gridDim.x * GPB;
@@ -571,29 +557,25 @@ __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand 
     }
     for (; i < n; i += stride) {
         const int rn = i + stride < n ? rows[i + stride] : 0;
+        for (int s = t; s < H1; s += G) {
+            k1[s] = kEmpty;
+            v1[s] = 0.0;
+        }
         for (int s = t; s < H2; s += G) {
             k2[s] = kEmpty;
             v2[s] = 0.0;
         }
         Group<G>::sync();
+        int c = 0;
         bool full1 = false, full2 = false;
-        expand_gen<G, U1>(
-            rb, re, [&](int64_t e) { return R.col[e]; }, [&](int64_t e) { return (double)rval[e]; }, A.rp32, A.col,
-            [&](int q) { return (double)aval[q]; }, slog1, k1, m1, sh1, full1,
-            [&](bool hit, unsigned sl, double lv, double w, int, bool fresh) {
-                if (hit) atomicAdd(v1 + sl, lv * w);
-                list_append<G>(fresh, sl, L1, cnt1);
-            });
+        expand_into<G, true, VT, VT, U1>(R.col, rval, rb, re, A.rp32, A.col, aval, slog1, k1, v1, m1, sh1, c,
+                                         full1);
         const int64_t rbn = R.rp[rn], ren = R.rp[rn + 1];
         Group<G>::sync();
-        const int n1 = *cnt1;
+        const int nn1 = compact_table<G, double>(k1, v1, H1, ired);
         if (!full1)
-            expand_gen<G, U2>(
-                0, n1, [&](int64_t e) { return k1[L1[e]]; }, [&](int64_t e) { return v1[L1[e]]; }, P.rp32, P.col,
-                [&](int q) { return (double)pval[q]; }, slog2, k2, m2, sh2, full2,
-                [&](bool hit, unsigned sl, double lv, double w, int, bool) {
-                    if (hit) atomicAdd(v2 + sl, lv * w);
-                });
+            expand_into<G, true, double, VT, U2>(k1, v1, 0, nn1, P.rp32, P.col, pval, slog2, k2, v2, m2, sh2, c,
+                                                 full2);
         const long long anyfull = group_max<G>((full1 || full2) ? 1 : 0, red);
         if (anyfull) {
             if (t == 0) {
@@ -616,21 +598,13 @@ __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand 
                 rownnz[r] = nn2;
             }
         }
-        // clear the stage-1 slots of this row
-        for (int j = t; j < n1; j += G) {
-            const int s = L1[j];
-            k1[s] = kEmpty;
-            v1[s] = 0.0;
-        }
         Group<G>::sync();
-        if (t == 0) *cnt1 = 0;
         r = rn;
         rb = rbn;
         re = ren;
     }
 }
 
-// ---------------------------------------------------------------------------
 // Move the rows of one bin from their scratch slots to the output (32 lanes
 // per row, two rows per wavefront; slots marked -1 belong to the long-row path).
 __global__ void __launch_bounds__(256) k_gather_rows(const int32_t *__restrict__ rows, const int *__restrict__ count,
@@ -1170,7 +1144,7 @@ void launch_dense_direct(bool num, int blocks, const SpOperand &R, const SpOpera
 size_t fused_lds(int G, int H1, int H2)
 {
     const size_t gpb = 256 / G;
-    return gpb * ((size_t)H1 * 16 + (size_t)H2 * 12) + gpb * group_waves(G) * 8 + gpb * (group_waves(G) + 2) * 4;
+    return gpb * ((size_t)H1 + H2) * 12 + gpb * group_waves(G) * 8 + gpb * (group_waves(G) + 1) * 4;
 }
 
 using FusedFn = void (*)(SpOperand, SpOperand, SpOperand, const int32_t *, const int *, int, int, int, int,

@@ -78,8 +78,8 @@ size_t fused_lds(int G, int H1, int H2);
 struct Tuning {
     int u1 = 4, u2 = 4;     // left entries held per lane in stage 1 / stage 2 (2 or 4)
     int subwave_h = 256;    // largest stage-1 table run by 32-lane groups (0: none)
-    int slog1_cap = 8;      // largest lane sub-group (log2) per left entry, stage 1
-    int slog2_cap = 8;      // the same, stage 2
+    int slog1_cap = 3;      // largest lane sub-group (log2) per left entry, stage 1
+    int slog2_cap = 1;      // the same, stage 2
     int h1_shift = 0;       // stage-1 table = bin upper bound >> h1_shift (a full table -> retry)
     int h2_div = 4;         // stage-2 table = max(64, H1 / h2_div)
 };
@@ -99,7 +99,7 @@ void launch_direct_mixed(int G, int H1, int H2, int slog1, int slog2, float guar
                          const SpOperand &A, const SpOperand &P, const int32_t *rows, const int *count,
                          int rows_host, int32_t *rownnz, int32_t *slotn, int32_t *scol, double *sval,
                          double *rowsum, int32_t *retry, int *nretry, int32_t *fix, int *nfix, int *err,
-                         int max_blocks, hipStream_t st);
+                         int max_blocks, hipStream_t st, const Tuning &tu);
 
 struct PackItem {
     const int64_t *rp;

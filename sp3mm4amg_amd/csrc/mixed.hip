@@ -18,6 +18,8 @@
 //     overwrites it after the gather.
 // Structure, binning, scratch slots and gather are those of the FP64 one-pass
 // kernel (k_direct_fused).
+#include <algorithm>
+
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -92,8 +94,9 @@ __device__ int compact3(int *keys, float *h, float *l, float *a, int H, int *red
     int base = 0;
     for (int s0 = 0; s0 < H; s0 += G) {
         const int s = s0 + t;
-        const int k = keys[s];
-        const float hh = h[s], ll = l[s], aa = a[s];
+        const bool in = s < H;
+        const int k = in ? keys[s] : kEmpty;
+        const float hh = in ? h[s] : 0.f, ll = in ? l[s] : 0.f, aa = in ? a[s] : 0.f;
         const bool occ = k != kEmpty;
         const uint64_t m = Group<G>::part(ballot(occ));
         int pos = G >= kWave ? lanes_below(m) : __popcll(m & ((1ull << t) - 1));
@@ -383,13 +386,13 @@ void launch_direct_mixed(int G, int H1, int H2, int slog1, int slog2, float guar
                          const SpOperand &A, const SpOperand &P, const int32_t *rows, const int *count,
                          int rows_host, int32_t *rownnz, int32_t *slotn, int32_t *scol, double *sval,
                          double *rowsum, int32_t *retry, int *nretry, int32_t *fix, int *nfix, int *err,
-                         int max_blocks, hipStream_t st)
+                         int max_blocks, hipStream_t st, const Tuning &tu)
 {
     const unsigned grid = grid_for_m(rows_host, G, max_blocks);
     const size_t lds = mixed_lds(G, H1, H2);
     const int cap = ilog2((unsigned)G);
-    if (slog1 > cap) slog1 = cap;
-    if (slog2 > cap) slog2 = cap;
+    slog1 = std::min(slog1, std::min(cap, std::max(0, tu.slog1_cap)));
+    slog2 = std::min(slog2, std::min(cap, std::max(0, tu.slog2_cap)));
     auto fn = G == 32 ? k_direct_mixed<32> : G == 64 ? k_direct_mixed<64> : k_direct_mixed<256>;
     hipLaunchKernelGGL(fn, dim3(grid), dim3(256), lds, st, R, A, P, rows, count, H1, H2, slog1, slog2, guard,
                        rownnz, slotn, scol, sval, rowsum, retry, nretry, fix, nfix, err);

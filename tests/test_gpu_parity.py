@@ -226,6 +226,8 @@ def test_mixed_precision_requires_fp32_operands():
 # schedules: every tuning gives the oracle's pattern and values
 SCHEDULES = [
     {"subwave_h": 0},                                   # wavefront groups only
+    {"subwave_h": 0, "h1_shift": 1},
+    {"subwave_h": 128, "h2_div": 2},
     {"subwave_h": 512, "u1": 2, "u2": 2, "slog1_cap": 1, "slog2_cap": 0},
     {"h1_shift": 1, "h2_div": 8},                       # tables that fill -> retry path
     {"h1_shift": 2, "h2_div": 1, "u1": 2, "u2": 4},
@@ -233,7 +235,7 @@ SCHEDULES = [
 
 
 @pytest.mark.parametrize("sched", range(len(SCHEDULES)))
-@pytest.mark.parametrize("name", ["poisson3d32", "irregular3000"])
+@pytest.mark.parametrize("name", ["poisson3d32", "irregular3000", "stencil27_15"])
 def test_schedules_keep_parity(name, sched):
     R, A, P = triple(name)
     print(f"schedule {SCHEDULES[sched]} on {name}", flush=True)
@@ -248,8 +250,26 @@ def test_schedules_keep_parity(name, sched):
         want32 = O.sp3mm_pair(R32, A32, P32)
         got, st, _ = L.rap_mixed_host(R32, A32, P32, precision=L.MIXED64)
         check(got, want32, None, bound=O.error_bound(R32, A32, P32, want32))
+        for guard in (0.0, 1e-2):
+            print(f"  mixed guard={guard}", flush=True)
+            got, st, _ = L.rap_mixed_host(R32, A32, P32, guard=guard, precision=L.MIXED)
+            assert got.pattern_equal(want32)
+            err = O.rel_errors(got, want32, O.abs_product(R32, A32, P32))
+            assert err.max() <= 1e-10 or guard == 0.0
     finally:
         L.set_tuning(None)
+
+
+def test_autotune_mixed():
+    R, A, P = (_rounded32(m) for m in triple("stencil27_15"))
+    d = [L.DeviceCSR.upload(m, vtype=L.F32) for m in (R, A, P)]
+    try:
+        best, ms_best, ms_default = L.autotune(*d, precision=L.MIXED, reps=1)
+        print(f"autotune mixed: {best} {ms_best:.4f} ms (start {ms_default:.4f} ms)")
+    finally:
+        L.set_tuning(None)
+        for m in d:
+            m.free()
 
 
 def test_autotune_installs_a_valid_schedule():
