@@ -212,9 +212,11 @@ typedef struct {
     int64_t nnz_mid;       /* nnz of R*A (pair path) or its symbolic count      */
     int64_t nnz_out;
     int64_t guarded_rows;  /* MIXED: rows recomputed in FP64                    */
-    int64_t guarded_entries;
+    int64_t guarded_entries; /* direct path: rows computed off the hash tables
+                                (long rows and rows whose tables filled)       */
     int32_t main_kernel_launches;
-    int32_t reserved;
+    int32_t spilled_rows;  /* direct path: of those, rows whose column windows
+                              exceeded LDS (global dense accumulators)          */
 } sp3mm_stats;
 
 /* Select (and initialise) the HIP device; called implicitly with device 0. */

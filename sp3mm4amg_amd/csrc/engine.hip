@@ -334,7 +334,8 @@ struct Ctl {
     int counts2[16];
     int nretry;
     int err;
-    int pad[2];
+    int nspill;  // long rows whose column windows exceed the LDS window kernel
+    int pad;
     unsigned long long totals[2];
     int64_t nnz;
     int64_t pad2;
@@ -845,12 +846,13 @@ struct FusedPlan {
     std::vector<FusedCfg> cfg;
     std::vector<int> cnt, fcnt;
     std::vector<int64_t> soff, nbase;
-    int nlong = 0, nretry = 0;
+    int nlong = 0, nretry = 0, nspill = 0;
+    bool built = false;  // build() has read the retry and spill counts
     int64_t nnz = 0, stot = 0;
     long long mac1 = 0;
     int slog1 = 0, slog2 = 0, dlog1 = 0, dlog2 = 0;
-    Buf ctl, ub, rown, lists, retry, slotn, fix, tiles, scol, sval, binscr;
-    DenseDirectWs dws, rws;
+    Buf ctl, ub, rown, lists, retry, slotn, fix, tiles, scol, sval, binscr, spill;
+    DenseDirectWs sws;  // global dense workspace of the spilled rows
     sp3mm_dmat C{};
     double ms_ub = 0, ms_num = 0, ms_scan = 0;
 
@@ -886,14 +888,32 @@ struct FusedPlan {
                                sval.as<double>() + soff[i], C.rp, C.col, static_cast<double *>(C.val), nnz,
                                &d()->err, max_blocks(c), s);
         }
+        launch_long(c, s, true);
+    }
+
+    // rows beyond the hash tables (long rows of the last bin, and rows whose
+    // tables filled): the LDS window kernel, and the global dense kernel for
+    // the rows whose windows do not fit in LDS.  num = false: count pass.
+    void launch_long(Context *c, hipStream_t s, bool num)
+    {
+        const int64_t M = R.M;
+        const size_t wl = c->max_lds - 1024;
+        double *cv = num ? static_cast<double *>(C.val) : nullptr;
+        int32_t *rn = num ? nullptr : rown.as<int32_t>();
+        double *rs = num && mixed ? rowsum : nullptr;
         if (nlong > 0)
-            launch_dense_direct(true, dense_blocks(c, nlong), r(), a(), p(), lists.as<int32_t>() + (int64_t)thr.nb * M,
-                                d()->counts + thr.nb, dws.ws, dlog1, dlog2, nullptr, nullptr, C.rp, C.col,
-                                static_cast<double *>(C.val), &d()->err, s, f32, mixed ? rowsum : nullptr);
-        if (nretry > 0)
-            launch_dense_direct(true, dense_blocks(c, nretry), r(), a(), p(), retry.as<int32_t>(), &d()->nretry,
-                                rws.ws, dlog1, dlog2, nullptr, nullptr, C.rp, C.col, static_cast<double *>(C.val),
-                                &d()->err, s, f32, mixed ? rowsum : nullptr);
+            launch_window_direct(num, std::min(nlong, c->num_cu), wl, r(), a(), p(),
+                                 lists.as<int32_t>() + (int64_t)thr.nb * M, d()->counts + thr.nb, dlog1,
+                                 spill.as<int32_t>(), &d()->nspill, rn, num ? C.rp : nullptr, num ? C.col : nullptr,
+                                 cv, rs, &d()->err, s, f32);
+        if (nretry > 0 || (!num && !built))
+            launch_window_direct(num, c->num_cu, wl, r(), a(), p(), retry.as<int32_t>(), &d()->nretry, dlog1,
+                                 spill.as<int32_t>(), &d()->nspill, rn, num ? C.rp : nullptr, num ? C.col : nullptr,
+                                 cv, rs, &d()->err, s, f32);
+        if (nspill > 0)
+            launch_dense_direct(num, dense_blocks(c, nspill), r(), a(), p(), spill.as<int32_t>(), &d()->nspill,
+                                sws.ws, dlog1, dlog2, nullptr, rn, num ? C.rp : nullptr, num ? C.col : nullptr, cv,
+                                &d()->err, s, f32, rs);
     }
 
     void launch_fixups(Context *c, hipStream_t s, bool all_bins)
@@ -923,7 +943,9 @@ struct FusedPlan {
         fix = Buf((size_t)(M ? M : 1) * 4);
         tiles = Buf((size_t)scan_tiles(M) * 8);
         binscr = Buf((size_t)bin_scratch_ints(M) * 4);
-        if (!ctl.p || !ub.p || !rown.p || !lists.p || !retry.p || !slotn.p || !fix.p || !tiles.p || !binscr.p) {
+        spill = Buf((size_t)(M ? M : 1) * 4);
+        if (!ctl.p || !ub.p || !rown.p || !lists.p || !retry.p || !slotn.p || !fix.p || !tiles.p || !binscr.p ||
+            !spill.p) {
             set_error("scratch allocation failed");
             return -1;
         }
@@ -961,21 +983,19 @@ struct FusedPlan {
         for (int i = 0; i < thr.nb; i++)
             if (cnt[i]) launch_main(c, s, i);
         const int en1 = tm.mark();
-        if (nlong > 0) {
-            if (dws.alloc(dense_blocks(c, nlong), A.N, P.N, true, s)) return -1;
-            launch_dense_direct(false, dense_blocks(c, nlong), r(), a(), p(), lists.as<int32_t>() + (int64_t)thr.nb * M,
-                                d()->counts + thr.nb, dws.ws, dlog1, dlog2, nullptr, rown.as<int32_t>(), nullptr,
-                                nullptr, nullptr, &d()->err, s, f32);
-        }
+        // long and retried rows: count pass of the window kernel (it lists the
+        // rows whose windows exceed LDS), then the global dense count for those
+        launch_long(c, s, false);
         if (dmat_alloc(C, M, P.N, 0, SP3MM_F64)) return -1;
         launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d()->nnz, s);
         if (read_ctl(c, d(), h)) return -1;
         nretry = h.nretry;
-        if (nretry > 0 && !h.err) {
-            // rows whose tables filled: count them densely, then rescan
-            if (rws.alloc(dense_blocks(c, nretry), A.N, P.N, true, s)) return -1;
-            launch_dense_direct(false, dense_blocks(c, nretry), r(), a(), p(), retry.as<int32_t>(), &d()->nretry,
-                                rws.ws, dlog1, dlog2, nullptr, rown.as<int32_t>(), nullptr, nullptr, nullptr,
+        nspill = h.nspill;
+        built = true;
+        if (nspill > 0 && !h.err) {
+            if (sws.alloc(dense_blocks(c, nspill), A.N, P.N, true, s)) return -1;
+            launch_dense_direct(false, dense_blocks(c, nspill), r(), a(), p(), spill.as<int32_t>(), &d()->nspill,
+                                sws.ws, dlog1, dlog2, nullptr, rown.as<int32_t>(), nullptr, nullptr, nullptr,
                                 &d()->err, s, f32);
             launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d()->nnz, s);
             if (read_ctl(c, d(), h)) return -1;
@@ -1022,18 +1042,11 @@ struct FusedPlan {
                       d()->totals, binscr.as<int>(), s);
         for (int i = 0; i < thr.nb; i++)
             if (cnt[i]) launch_main(c, s, i);
-        if (nlong > 0)
-            launch_dense_direct(false, dense_blocks(c, nlong), r(), a(), p(), lists.as<int32_t>() + (int64_t)thr.nb * M,
-                                d()->counts + thr.nb, dws.ws, dlog1, dlog2, nullptr, rown.as<int32_t>(), nullptr,
-                                nullptr, nullptr, &d()->err, s, f32);
-        if (nretry > 0)
-            launch_dense_direct(false, dense_blocks(c, nretry), r(), a(), p(), retry.as<int32_t>(), &d()->nretry,
-                                rws.ws, dlog1, dlog2, nullptr, rown.as<int32_t>(), nullptr, nullptr, nullptr,
-                                &d()->err, s, f32);
+        launch_long(c, s, false);
         launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d()->nnz, s);
         launch_tail(c, s);
         launch_fixups(c, s, true);
-        launch_check_nnz(&d()->nnz, nnz, &d()->nretry, nretry, &d()->err, s);
+        launch_check_nnz(&d()->nnz, nnz, &d()->nretry, nretry, &d()->nspill, nspill, &d()->err, s);
     }
 
     // after a replay: did the device see the sizes this plan was built for?
@@ -1094,7 +1107,8 @@ static int rap_direct_fused(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3m
         }
         st->main_kernel_launches = launches;
         st->guarded_rows = nf;
-        st->guarded_entries = plan.nretry + plan.nlong;
+        st->guarded_entries = plan.nretry + plan.nlong;  // rows off the hash-table path
+        st->spilled_rows = plan.nspill;
     }
     return 0;
 }

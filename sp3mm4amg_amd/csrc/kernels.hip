@@ -24,6 +24,7 @@
 #include "kernels.hpp"
 #include "expand.hpp"
 #include <algorithm>
+#include <climits>
 #include <stdlib.h>
 #include <string.h>
 
@@ -1119,6 +1120,227 @@ void launch_dense_spgemm(bool num, int blocks, const SpOperand &A, const SpOpera
                            vals, nw, slog, rownnz, crp, ccol, cval, err);
 }
 
+// ---------------------------------------------------------------------------
+// long rows with narrow column windows: dense accumulators in LDS
+// ---------------------------------------------------------------------------
+// Rows too long for the hash tables often still touch a narrow band of
+// columns (coarse levels of a hierarchy, block-diagonal batches).  One
+// workgroup per row: stage 1 finds the window [lo1, hi1] of the R*A row from
+// the first and last column of every A row it references, clears that part of
+// a dense LDS array (FP64 values + presence bits) and accumulates into it with
+// native ds_add_f64; stage 2 finds the window of the forwarded row in P and
+// does the same in a second LDS region.  The presence bits of stage 2, read in
+// order, are the sorted output row.  A row whose windows exceed the LDS
+// capacity `cap` (values) goes to `spill` (count pass only), for the global
+// dense kernel.
+__device__ __forceinline__ void block_minmax(int &lo, int &hi, int *sh)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        lo = min(lo, __shfl_xor(lo, o));
+        hi = max(hi, __shfl_xor(hi, o));
+    }
+    __syncthreads();
+    if (lane() == 0) {
+        sh[threadIdx.x / 64] = lo;
+        sh[kDense / 64 + threadIdx.x / 64] = hi;
+    }
+    __syncthreads();
+    lo = sh[0];
+    hi = sh[kDense / 64];
+    for (int w = 1; w < kDense / 64; w++) {
+        lo = min(lo, sh[w]);
+        hi = max(hi, sh[kDense / 64 + w]);
+    }
+    __syncthreads();
+}
+
+template <bool NUM, typename VT>
+__global__ void __launch_bounds__(kDense) k_window_direct(SpOperand R, SpOperand A, SpOperand P,
+                                                           const int32_t *__restrict__ rows,
+                                                           const int *__restrict__ count, int cap, int slog1,
+                                                           int32_t *__restrict__ spill, int *__restrict__ nspill,
+                                                           int32_t *__restrict__ rownnz,
+                                                           const int64_t *__restrict__ crp,
+                                                           int32_t *__restrict__ ccol, double *__restrict__ cval,
+                                                           double *__restrict__ rowsum, int *__restrict__ err)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *V = reinterpret_cast<double *>(smem);                  // cap values
+    unsigned *B = reinterpret_cast<unsigned *>(smem + (size_t)cap * 8);  // cap/32 words
+    __shared__ int ish[2 * (kDense / 64)];
+    __shared__ long long lsh[kDense / 64];
+    __shared__ double dsh[kDense / 64];
+    const int t = threadIdx.x;
+    const VT *rval = static_cast<const VT *>(R.val);
+    const VT *aval = static_cast<const VT *>(A.val);
+    const VT *pval = static_cast<const VT *>(P.val);
+    const int S = 1 << slog1;
+    const int n = *count;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int r = rows[i];
+        const int64_t e0 = R.rp[r], e1 = R.rp[r + 1];
+        // window of the R*A row
+        int lo1 = INT_MAX, hi1 = -1;
+        for (int64_t e = e0 + t; e < e1; e += kDense) {
+            const int k = R.col[e];
+            const int a0 = A.rp32[k], a1 = A.rp32[k + 1];
+            if (a1 > a0) {
+                lo1 = min(lo1, A.col[a0]);
+                hi1 = max(hi1, A.col[a1 - 1]);
+            }
+        }
+        block_minmax(lo1, hi1, ish);
+        const int span1 = hi1 >= lo1 ? hi1 - lo1 + 1 : 0;
+        const int nw1 = (span1 + 31) / 32;
+        if (nw1 * 32 > cap) {
+            if (!NUM && t == 0) {
+                spill[atomicAdd(nspill, 1)] = r;
+                rownnz[r] = 0;
+            }
+            continue;
+        }
+        for (int w = t; w < nw1; w += kDense) B[w] = 0u;
+        if constexpr (NUM)
+            for (int j = t; j < span1; j += kDense) V[j] = 0.0;
+        __syncthreads();
+        // stage 1: sub-groups of S lanes per R entry walk its A row
+        for (int64_t e = e0 + (t >> slog1); e < e1; e += (kDense >> slog1)) {
+            const int k = R.col[e];
+            double rv = 0.0;
+            if constexpr (NUM) rv = (double)rval[e];
+            const int a1 = A.rp32[k + 1];
+            for (int q = A.rp32[k] + (t & (S - 1)); q < a1; q += S) {
+                const int j = A.col[q] - lo1;
+                if ((unsigned)j >= (unsigned)span1) {  // an unsorted A row
+                    atomicOr(err, kErrBadInput);
+                    continue;
+                }
+                atomicOr(B + (j >> 5), 1u << (j & 31));
+                if constexpr (NUM) atomicAdd(V + j, rv * (double)aval[q]);
+            }
+        }
+        __syncthreads();
+        // window of the forwarded row in P
+        int lo2 = INT_MAX, hi2 = -1;
+        for (int j = t; j < span1; j += kDense) {
+            if (B[j >> 5] & (1u << (j & 31))) {
+                const int c = lo1 + j;
+                const int p0 = P.rp32[c], p1 = P.rp32[c + 1];
+                if (p1 > p0) {
+                    lo2 = min(lo2, P.col[p0]);
+                    hi2 = max(hi2, P.col[p1 - 1]);
+                }
+            }
+        }
+        block_minmax(lo2, hi2, ish);
+        const int span2 = hi2 >= lo2 ? hi2 - lo2 + 1 : 0;
+        const int nw2 = (span2 + 31) / 32;
+        const int off2 = nw1 * 32;  // region 2 starts on a word boundary
+        if (off2 + nw2 * 32 > cap) {
+            if (!NUM && t == 0) {
+                spill[atomicAdd(nspill, 1)] = r;
+                rownnz[r] = 0;
+            }
+            continue;
+        }
+        unsigned *B2 = B + nw1;
+        double *V2 = V + off2;
+        for (int w = t; w < nw2; w += kDense) B2[w] = 0u;
+        if constexpr (NUM)
+            for (int j = t; j < span2; j += kDense) V2[j] = 0.0;
+        __syncthreads();
+        // stage 2: every present entry of the R*A row is forwarded by one lane
+        double ps = 0.0;
+        for (int j = t; j < span1; j += kDense) {
+            if (!(B[j >> 5] & (1u << (j & 31)))) continue;
+            const int c = lo1 + j;
+            double x = 0.0;
+            if constexpr (NUM) x = V[j];
+            const int p1 = P.rp32[c + 1];
+            for (int q = P.rp32[c]; q < p1; q++) {
+                const int j2 = P.col[q] - lo2;
+                if ((unsigned)j2 >= (unsigned)span2) {  // an unsorted P row
+                    atomicOr(err, kErrBadInput);
+                    continue;
+                }
+                atomicOr(B2 + (j2 >> 5), 1u << (j2 & 31));
+                if constexpr (NUM) {
+                    const double pr = x * (double)pval[q];
+                    atomicAdd(V2 + j2, pr);
+                    ps += pr;
+                }
+            }
+        }
+        __syncthreads();
+        if constexpr (!NUM) {
+            long long cnt = 0;
+            for (int w = t; w < nw2; w += kDense) cnt += __popc(B2[w]);
+            long long total;
+            (void)block_exclusive_scan(cnt, lsh, total);
+            if (t == 0) rownnz[r] = (int32_t)total;
+        } else {
+            if (rowsum) {
+                for (int o = 32; o > 0; o >>= 1) ps += __shfl_xor(ps, o);
+                if (lane() == 0) dsh[t / 64] = ps;
+                __syncthreads();
+                if (t == 0) {
+                    double tot = 0.0;
+                    for (int w = 0; w < kDense / 64; w++) tot += dsh[w];
+                    rowsum[r] = tot;
+                }
+            }
+            // sorted emission of region 2
+            const int64_t off = crp[r], len = crp[r + 1] - off;
+            long long base = 0;
+            for (int w0 = 0; w0 < nw2; w0 += kDense) {
+                const int w = w0 + t;
+                unsigned x = w < nw2 ? B2[w] : 0u;
+                long long total;
+                long long pos = base + block_exclusive_scan(__popc(x), lsh, total);
+                while (x) {
+                    const int b = __ffs(x) - 1;
+                    x &= x - 1;
+                    const int j2 = w * 32 + b;
+                    if (pos < len) {
+                        ccol[off + pos] = lo2 + j2;
+                        cval[off + pos] = V2[j2];
+                    }
+                    pos++;
+                }
+                base += total;
+            }
+            if (t == 0 && base != len) atomicOr(err, kErrOverflow);
+        }
+        __syncthreads();
+    }
+}
+
+// values the window kernel can hold in `lds` bytes (a multiple of 32)
+int window_cap(size_t lds)
+{
+    const size_t c = lds / (8 + 4.0 / 32);
+    return (int)(c / 32 * 32);
+}
+
+void launch_window_direct(bool num, int blocks, size_t lds, const SpOperand &R, const SpOperand &A,
+                          const SpOperand &P, const int32_t *rows, const int *count, int slog1, int32_t *spill,
+                          int *nspill, int32_t *rownnz, const int64_t *crp, int32_t *ccol, double *cval,
+                          double *rowsum, int *err, hipStream_t st, bool f32in)
+{
+    const int cap = window_cap(lds);
+    const size_t bytes = (size_t)cap * 8 + (size_t)cap / 32 * 4;
+    if (slog1 > 6) slog1 = 6;
+#define SP3MM_WIN_ARGS R, A, P, rows, count, cap, slog1, spill, nspill, rownnz, crp, ccol, cval, rowsum, err
+    if (f32in) {
+        if (num) hipLaunchKernelGGL((k_window_direct<true, float>), dim3(blocks), dim3(kDense), bytes, st, SP3MM_WIN_ARGS);
+        else hipLaunchKernelGGL((k_window_direct<false, float>), dim3(blocks), dim3(kDense), bytes, st, SP3MM_WIN_ARGS);
+    } else {
+        if (num) hipLaunchKernelGGL((k_window_direct<true, double>), dim3(blocks), dim3(kDense), bytes, st, SP3MM_WIN_ARGS);
+        else hipLaunchKernelGGL((k_window_direct<false, double>), dim3(blocks), dim3(kDense), bytes, st, SP3MM_WIN_ARGS);
+    }
+#undef SP3MM_WIN_ARGS
+}
+
 void launch_dense_direct(bool num, int blocks, const SpOperand &R, const SpOperand &A, const SpOperand &P,
                          const int32_t *rows, const int *count, const DenseWs &ws, int slog1, int slog2,
                          int32_t *n1out, int32_t *rownnz, const int64_t *crp, int32_t *ccol, double *cval,
@@ -1194,14 +1416,18 @@ void launch_gather_rows(const int32_t *rows, const int *count, int rows_host, co
                        cval, out_cap, err);
 }
 
-__global__ void k_check_nnz(const int64_t *nnz, int64_t want, const int *nretry, int want_retry, int *err)
+__global__ void k_check_nnz(const int64_t *nnz, int64_t want, const int *nretry, int want_retry,
+                            const int *nspill, int want_spill, int *err)
 {
-    if (threadIdx.x == 0 && (*nnz != want || *nretry != want_retry)) atomicOr(err, kErrOverflow);
+    if (threadIdx.x == 0 && (*nnz != want || *nretry != want_retry || *nspill != want_spill))
+        atomicOr(err, kErrOverflow);
 }
 
-void launch_check_nnz(const int64_t *nnz, int64_t want, const int *nretry, int want_retry, int *err, hipStream_t st)
+void launch_check_nnz(const int64_t *nnz, int64_t want, const int *nretry, int want_retry, const int *nspill,
+                      int want_spill, int *err, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_check_nnz, dim3(1), dim3(64), 0, st, nnz, want, nretry, want_retry, err);
+    hipLaunchKernelGGL(k_check_nnz, dim3(1), dim3(64), 0, st, nnz, want, nretry, want_retry, nspill, want_spill,
+                       err);
 }
 
 // ---------------------------------------------------------------------------

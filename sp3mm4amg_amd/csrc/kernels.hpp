@@ -66,6 +66,13 @@ int dense_words(int64_t ncols);
 void launch_dense_spgemm(bool num, int blocks, const SpOperand &A, const SpOperand &B, const int32_t *rows,
                          const int *count, unsigned *bits, double *vals, int nw, int slog, int32_t *rownnz,
                          const int64_t *crp, int32_t *ccol, double *cval, int *err, hipStream_t st);
+// long rows whose column windows fit in `lds` bytes of LDS (see k_window_direct);
+// the count pass (num = false) lists the rows that do not fit in `spill`
+int window_cap(size_t lds);
+void launch_window_direct(bool num, int blocks, size_t lds, const SpOperand &R, const SpOperand &A,
+                          const SpOperand &P, const int32_t *rows, const int *count, int slog1, int32_t *spill,
+                          int *nspill, int32_t *rownnz, const int64_t *crp, int32_t *ccol, double *cval,
+                          double *rowsum, int *err, hipStream_t st, bool f32in);
 void launch_dense_direct(bool num, int blocks, const SpOperand &R, const SpOperand &A, const SpOperand &P,
                          const int32_t *rows, const int *count, const DenseWs &ws, int slog1, int slog2,
                          int32_t *n1out, int32_t *rownnz, const int64_t *crp, int32_t *ccol, double *cval,
@@ -92,7 +99,8 @@ void launch_direct_fused(int G, int H1, int H2, int slog1, int slog2, const SpOp
 void launch_gather_rows(const int32_t *rows, const int *count, int rows_host, const int32_t *slotn, int cap,
                         const int32_t *scol, const double *sval, const int64_t *crp, int32_t *ccol, double *cval,
                         int64_t out_cap, int *err, int max_blocks, hipStream_t st);
-void launch_check_nnz(const int64_t *nnz, int64_t want, const int *nretry, int want_retry, int *err, hipStream_t st);
+void launch_check_nnz(const int64_t *nnz, int64_t want, const int *nretry, int want_retry, const int *nspill,
+                      int want_spill, int *err, hipStream_t st);
 
 size_t mixed_lds(int G, int H1, int H2);
 void launch_direct_mixed(int G, int H1, int H2, int slog1, int slog2, float guard, const SpOperand &R,

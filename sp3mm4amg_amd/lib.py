@@ -61,7 +61,7 @@ class Stats(C.Structure):
                 ("mac1", C.c_int64), ("mac2", C.c_int64), ("nnz_mid", C.c_int64),
                 ("nnz_out", C.c_int64), ("guarded_rows", C.c_int64),
                 ("guarded_entries", C.c_int64), ("main_kernel_launches", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("spilled_rows", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}

@@ -260,6 +260,31 @@ def test_schedules_keep_parity(name, sched):
         L.set_tuning(None)
 
 
+def test_long_rows_window_and_spill():
+    """Rows beyond the hash tables: the LDS window kernel takes rows whose
+    column windows fit in LDS, the global dense kernel the rest (spilled)."""
+    p = problems.irregular_sa(40000, seed=4)
+    R, A, P = p["R"], p["A"], p["P"]
+    print(f"irregular40000: {R.M} coarse rows", flush=True)
+    want = O.sp3mm_pair(R, A, P)
+    bound = O.error_bound(R, A, P, want)
+    got, st = L.rap_host(R, A, P, path=L.DIRECT)
+    print(f"  off-table rows {st['guarded_entries']} spilled {st['spilled_rows']}")
+    assert st["guarded_entries"] > 0 and st["spilled_rows"] > 0
+    check(got, want, None, bound=bound)
+    # windows that fit: a structured problem whose tables are forced to fill
+    R, A, P = triple("poisson3d32")
+    want = O.sp3mm_pair(R, A, P)
+    L.set_tuning({"h1_shift": 2, "h2_div": 8})
+    try:
+        got, st = L.rap_host(R, A, P, path=L.DIRECT)
+    finally:
+        L.set_tuning(None)
+    print(f"  poisson3d32 forced retries: off-table rows {st['guarded_entries']} spilled {st['spilled_rows']}")
+    assert st["guarded_entries"] > 0 and st["spilled_rows"] == 0
+    check(got, want, O.abs_product(R, A, P))
+
+
 def test_autotune_mixed():
     R, A, P = (_rounded32(m) for m in triple("stencil27_15"))
     d = [L.DeviceCSR.upload(m, vtype=L.F32) for m in (R, A, P)]
