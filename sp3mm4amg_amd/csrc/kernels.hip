@@ -1132,7 +1132,10 @@ void launch_dense_spgemm(bool num, int blocks, const SpOperand &A, const SpOpera
 // does the same in a second LDS region.  The presence bits of stage 2, read in
 // order, are the sorted output row.  A row whose windows exceed the LDS
 // capacity `cap` (values) goes to `spill` (count pass only), for the global
-// dense kernel.
+// dense kernel.  Workgroups of kWin lanes: with one row in LDS per workgroup,
+// 16 wavefronts per CU keep enough loads in flight.
+constexpr int kWin = 1024;
+
 __device__ __forceinline__ void block_minmax(int &lo, int &hi, int *sh)
 {
     for (int o = 32; o > 0; o >>= 1) {
@@ -1142,20 +1145,20 @@ __device__ __forceinline__ void block_minmax(int &lo, int &hi, int *sh)
     __syncthreads();
     if (lane() == 0) {
         sh[threadIdx.x / 64] = lo;
-        sh[kDense / 64 + threadIdx.x / 64] = hi;
+        sh[kWin / 64 + threadIdx.x / 64] = hi;
     }
     __syncthreads();
     lo = sh[0];
-    hi = sh[kDense / 64];
-    for (int w = 1; w < kDense / 64; w++) {
+    hi = sh[kWin / 64];
+    for (int w = 1; w < kWin / 64; w++) {
         lo = min(lo, sh[w]);
-        hi = max(hi, sh[kDense / 64 + w]);
+        hi = max(hi, sh[kWin / 64 + w]);
     }
     __syncthreads();
 }
 
 template <bool NUM, typename VT>
-__global__ void __launch_bounds__(kDense) k_window_direct(SpOperand R, SpOperand A, SpOperand P,
+__global__ void __launch_bounds__(kWin) k_window_direct(SpOperand R, SpOperand A, SpOperand P,
                                                            const int32_t *__restrict__ rows,
                                                            const int *__restrict__ count, int cap, int slog1,
                                                            int32_t *__restrict__ spill, int *__restrict__ nspill,
@@ -1167,9 +1170,9 @@ __global__ void __launch_bounds__(kDense) k_window_direct(SpOperand R, SpOperand
     extern __shared__ __align__(16) unsigned char smem[];
     double *V = reinterpret_cast<double *>(smem);                  // cap values
     unsigned *B = reinterpret_cast<unsigned *>(smem + (size_t)cap * 8);  // cap/32 words
-    __shared__ int ish[2 * (kDense / 64)];
-    __shared__ long long lsh[kDense / 64];
-    __shared__ double dsh[kDense / 64];
+    __shared__ int ish[2 * (kWin / 64)];
+    __shared__ long long lsh[kWin / 64];
+    __shared__ double dsh[kWin / 64];
     const int t = threadIdx.x;
     const VT *rval = static_cast<const VT *>(R.val);
     const VT *aval = static_cast<const VT *>(A.val);
@@ -1181,7 +1184,7 @@ __global__ void __launch_bounds__(kDense) k_window_direct(SpOperand R, SpOperand
         const int64_t e0 = R.rp[r], e1 = R.rp[r + 1];
         // window of the R*A row
         int lo1 = INT_MAX, hi1 = -1;
-        for (int64_t e = e0 + t; e < e1; e += kDense) {
+        for (int64_t e = e0 + t; e < e1; e += kWin) {
             const int k = R.col[e];
             const int a0 = A.rp32[k], a1 = A.rp32[k + 1];
             if (a1 > a0) {
@@ -1199,12 +1202,12 @@ __global__ void __launch_bounds__(kDense) k_window_direct(SpOperand R, SpOperand
             }
             continue;
         }
-        for (int w = t; w < nw1; w += kDense) B[w] = 0u;
+        for (int w = t; w < nw1; w += kWin) B[w] = 0u;
         if constexpr (NUM)
-            for (int j = t; j < span1; j += kDense) V[j] = 0.0;
+            for (int j = t; j < span1; j += kWin) V[j] = 0.0;
         __syncthreads();
         // stage 1: sub-groups of S lanes per R entry walk its A row
-        for (int64_t e = e0 + (t >> slog1); e < e1; e += (kDense >> slog1)) {
+        for (int64_t e = e0 + (t >> slog1); e < e1; e += (kWin >> slog1)) {
             const int k = R.col[e];
             double rv = 0.0;
             if constexpr (NUM) rv = (double)rval[e];
@@ -1215,14 +1218,15 @@ __global__ void __launch_bounds__(kDense) k_window_direct(SpOperand R, SpOperand
                     atomicOr(err, kErrBadInput);
                     continue;
                 }
-                atomicOr(B + (j >> 5), 1u << (j & 31));
+                const unsigned bit = 1u << (j & 31);
+                if (!(B[j >> 5] & bit)) atomicOr(B + (j >> 5), bit);  // set bits are never cleared here
                 if constexpr (NUM) atomicAdd(V + j, rv * (double)aval[q]);
             }
         }
         __syncthreads();
         // window of the forwarded row in P
         int lo2 = INT_MAX, hi2 = -1;
-        for (int j = t; j < span1; j += kDense) {
+        for (int j = t; j < span1; j += kWin) {
             if (B[j >> 5] & (1u << (j & 31))) {
                 const int c = lo1 + j;
                 const int p0 = P.rp32[c], p1 = P.rp32[c + 1];
@@ -1245,13 +1249,13 @@ __global__ void __launch_bounds__(kDense) k_window_direct(SpOperand R, SpOperand
         }
         unsigned *B2 = B + nw1;
         double *V2 = V + off2;
-        for (int w = t; w < nw2; w += kDense) B2[w] = 0u;
+        for (int w = t; w < nw2; w += kWin) B2[w] = 0u;
         if constexpr (NUM)
-            for (int j = t; j < span2; j += kDense) V2[j] = 0.0;
+            for (int j = t; j < span2; j += kWin) V2[j] = 0.0;
         __syncthreads();
         // stage 2: every present entry of the R*A row is forwarded by one lane
         double ps = 0.0;
-        for (int j = t; j < span1; j += kDense) {
+        for (int j = t; j < span1; j += kWin) {
             if (!(B[j >> 5] & (1u << (j & 31)))) continue;
             const int c = lo1 + j;
             double x = 0.0;
@@ -1263,7 +1267,8 @@ __global__ void __launch_bounds__(kDense) k_window_direct(SpOperand R, SpOperand
                     atomicOr(err, kErrBadInput);
                     continue;
                 }
-                atomicOr(B2 + (j2 >> 5), 1u << (j2 & 31));
+                const unsigned bit2 = 1u << (j2 & 31);
+                if (!(B2[j2 >> 5] & bit2)) atomicOr(B2 + (j2 >> 5), bit2);
                 if constexpr (NUM) {
                     const double pr = x * (double)pval[q];
                     atomicAdd(V2 + j2, pr);
@@ -1274,7 +1279,7 @@ __global__ void __launch_bounds__(kDense) k_window_direct(SpOperand R, SpOperand
         __syncthreads();
         if constexpr (!NUM) {
             long long cnt = 0;
-            for (int w = t; w < nw2; w += kDense) cnt += __popc(B2[w]);
+            for (int w = t; w < nw2; w += kWin) cnt += __popc(B2[w]);
             long long total;
             (void)block_exclusive_scan(cnt, lsh, total);
             if (t == 0) rownnz[r] = (int32_t)total;
@@ -1285,14 +1290,14 @@ __global__ void __launch_bounds__(kDense) k_window_direct(SpOperand R, SpOperand
                 __syncthreads();
                 if (t == 0) {
                     double tot = 0.0;
-                    for (int w = 0; w < kDense / 64; w++) tot += dsh[w];
+                    for (int w = 0; w < kWin / 64; w++) tot += dsh[w];
                     rowsum[r] = tot;
                 }
             }
             // sorted emission of region 2
             const int64_t off = crp[r], len = crp[r + 1] - off;
             long long base = 0;
-            for (int w0 = 0; w0 < nw2; w0 += kDense) {
+            for (int w0 = 0; w0 < nw2; w0 += kWin) {
                 const int w = w0 + t;
                 unsigned x = w < nw2 ? B2[w] : 0u;
                 long long total;
@@ -1332,11 +1337,11 @@ void launch_window_direct(bool num, int blocks, size_t lds, const SpOperand &R, 
     if (slog1 > 6) slog1 = 6;
 #define SP3MM_WIN_ARGS R, A, P, rows, count, cap, slog1, spill, nspill, rownnz, crp, ccol, cval, rowsum, err
     if (f32in) {
-        if (num) hipLaunchKernelGGL((k_window_direct<true, float>), dim3(blocks), dim3(kDense), bytes, st, SP3MM_WIN_ARGS);
-        else hipLaunchKernelGGL((k_window_direct<false, float>), dim3(blocks), dim3(kDense), bytes, st, SP3MM_WIN_ARGS);
+        if (num) hipLaunchKernelGGL((k_window_direct<true, float>), dim3(blocks), dim3(kWin), bytes, st, SP3MM_WIN_ARGS);
+        else hipLaunchKernelGGL((k_window_direct<false, float>), dim3(blocks), dim3(kWin), bytes, st, SP3MM_WIN_ARGS);
     } else {
-        if (num) hipLaunchKernelGGL((k_window_direct<true, double>), dim3(blocks), dim3(kDense), bytes, st, SP3MM_WIN_ARGS);
-        else hipLaunchKernelGGL((k_window_direct<false, double>), dim3(blocks), dim3(kDense), bytes, st, SP3MM_WIN_ARGS);
+        if (num) hipLaunchKernelGGL((k_window_direct<true, double>), dim3(blocks), dim3(kWin), bytes, st, SP3MM_WIN_ARGS);
+        else hipLaunchKernelGGL((k_window_direct<false, double>), dim3(blocks), dim3(kWin), bytes, st, SP3MM_WIN_ARGS);
     }
 #undef SP3MM_WIN_ARGS
 }
