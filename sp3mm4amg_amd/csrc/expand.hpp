@@ -100,6 +100,92 @@ __device__ __forceinline__ void expand_gen(int64_t e0, int64_t e1, LoadK loadk, 
     }
 }
 
+// Branch-free form of the FP64 expansion for the one-pass direct kernel.  Same
+// lane layout as expand_gen; on the common path every lane executes the same
+// instructions: loads read a valid index and their result is selected, and a
+// product that is absent (or not yet inserted) compare-and-swaps and adds into
+// the lane's own dummy slot just past the table (keys[H + lane] stays empty:
+// the swap writes kEmpty over kEmpty), so no exec-mask branches are needed.
+// Only inserts whose first slot holds another key take the probing branch.
+// `keys`/`vals` hold H + G slots; `dslot` = H + the lane's index in the group.
+template <int G, typename LV, typename BV, int U>
+__device__ __forceinline__ void expand_flat(const int32_t *lcol, const LV *lval, int64_t e0, int64_t e1,
+                                            const int32_t *__restrict__ brp, const int32_t *__restrict__ bcol,
+                                            const BV *__restrict__ bval, int slog, int *keys, double *vals,
+                                            unsigned mask, int shift, unsigned dslot, bool &full)
+{
+    constexpr int kLogG = G >= 256 ? 8 : G >= 128 ? 7 : G >= 64 ? 6 : G >= 32 ? 5 : 4;
+    if (slog > kLogG) slog = kLogG;
+    const int t = Group<G>::tid();
+    const int S = 1 << slog;
+    const int sub = t & (S - 1);
+    const int ngrp = G >> slog;
+    for (int64_t eb = e0 + (t >> slog); eb < e1; eb += (int64_t)ngrp * U) {
+        int q[U], qe[U];
+        double lv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t e = eb + (int64_t)u * ngrp;
+            const bool ok = e < e1;
+            const int64_t es = ok ? e : eb;  // eb < e1: a valid entry
+            const int k = lcol[es];
+            const double x = (double)lval[es];
+            const int b0 = brp[k], b1 = brp[k + 1];
+            lv[u] = ok ? x : 0.0;
+            q[u] = ok ? b0 + sub : 0;
+            qe[u] = ok ? b1 : 0;
+        }
+        while (true) {
+            bool more = false;
+#pragma unroll
+            for (int u = 0; u < U; u++) more |= q[u] < qe[u];
+            if (!more) break;
+            int c[U], old[U];
+            unsigned sl[U];
+            double w[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const bool ok = q[u] < qe[u];
+                const int qs = ok ? q[u] : qe[u] - 1 > 0 ? qe[u] - 1 : 0;  // some valid entry of B
+                const int cc = bcol[qs];
+                const double ww = (double)bval[qs];
+                c[u] = ok ? cc : kEmpty;
+                w[u] = ww;
+                q[u] += S;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                sl[u] = c[u] != kEmpty ? (hslot(c[u], shift) & mask) : dslot;
+                old[u] = atomicCAS(keys + sl[u], kEmpty, c[u]);
+            }
+            bool pend = false;
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const bool hit = c[u] != kEmpty && (old[u] == kEmpty || old[u] == c[u]);
+                pend |= c[u] != kEmpty && !hit;
+                atomicAdd(vals + (hit ? sl[u] : dslot), lv[u] * w[u]);
+            }
+            if (pend) {
+                for (int u = 0; u < U; u++) {
+                    if (c[u] == kEmpty || old[u] == kEmpty || old[u] == c[u]) continue;
+                    unsigned s2 = sl[u];
+                    bool done = false;
+                    for (unsigned probe = 1; probe <= mask; probe++) {
+                        s2 = (s2 + 1) & mask;
+                        const int o = atomicCAS(keys + s2, kEmpty, c[u]);
+                        if (o == kEmpty || o == c[u]) {
+                            atomicAdd(vals + s2, lv[u] * w[u]);
+                            done = true;
+                            break;
+                        }
+                    }
+                    if (!done) full = true;
+                }
+            }
+        }
+    }
+}
+
 // Appends, for every lane of the group with `claim` set, `slot` to the group's
 // list of claimed slots (list[n++] = slot): one LDS atomic per wavefront.
 template <int G>

@@ -528,14 +528,16 @@ __global__ void __launch_bounds__(256) k_direct_fused(SpOperand R, SpOperand A, 
     const int g = threadIdx.x / G;
     const int t = Group<G>::tid();
     unsigned char *p = smem;
-    double *v1 = reinterpret_cast<double *>(p) + g * H1;
-    p += (size_t)GPB * H1 * 8;
-    double *v2 = reinterpret_cast<double *>(p) + g * H2;
-    p += (size_t)GPB * H2 * 8;
-    int *k1 = reinterpret_cast<int *>(p) + g * H1;
-    p += (size_t)GPB * H1 * 4;
-    int *k2 = reinterpret_cast<int *>(p) + g * H2;
-    p += (size_t)GPB * H2 * 4;
+    // each table is followed by G dummy slots, one per lane (see expand_flat)
+    const int W1 = H1 + G, W2 = H2 + G;
+    double *v1 = reinterpret_cast<double *>(p) + g * W1;
+    p += (size_t)GPB * W1 * 8;
+    double *v2 = reinterpret_cast<double *>(p) + g * W2;
+    p += (size_t)GPB * W2 * 8;
+    int *k1 = reinterpret_cast<int *>(p) + g * W1;
+    p += (size_t)GPB * W1 * 4;
+    int *k2 = reinterpret_cast<int *>(p) + g * W2;
+    p += (size_t)GPB * W2 * 4;
     long long *red = reinterpret_cast<long long *>(p) + g * Group<G>::kWaves;
     p += (size_t)GPB * Group<G>::kWaves * 8;
     int *ired = reinterpret_cast<int *>(p) + g * (Group<G>::kWaves + 1);
@@ -558,25 +560,24 @@ __global__ void __launch_bounds__(256) k_direct_fused(SpOperand R, SpOperand A, 
     }
     for (; i < n; i += stride) {
         const int rn = i + stride < n ? rows[i + stride] : 0;
-        for (int s = t; s < H1; s += G) {
+        for (int s = t; s < W1; s += G) {
             k1[s] = kEmpty;
             v1[s] = 0.0;
         }
-        for (int s = t; s < H2; s += G) {
+        for (int s = t; s < W2; s += G) {
             k2[s] = kEmpty;
             v2[s] = 0.0;
         }
         Group<G>::sync();
-        int c = 0;
         bool full1 = false, full2 = false;
-        expand_into<G, true, VT, VT, U1>(R.col, rval, rb, re, A.rp32, A.col, aval, slog1, k1, v1, m1, sh1, c,
-                                         full1);
+        expand_flat<G, VT, VT, U1>(R.col, rval, rb, re, A.rp32, A.col, aval, slog1, k1, v1, m1, sh1,
+                                   (unsigned)(H1 + t), full1);
         const int64_t rbn = R.rp[rn], ren = R.rp[rn + 1];
         Group<G>::sync();
         const int nn1 = compact_table<G, double>(k1, v1, H1, ired);
         if (!full1)
-            expand_into<G, true, double, VT, U2>(k1, v1, 0, nn1, P.rp32, P.col, pval, slog2, k2, v2, m2, sh2, c,
-                                                 full2);
+            expand_flat<G, double, VT, U2>(k1, v1, 0, nn1, P.rp32, P.col, pval, slog2, k2, v2, m2, sh2,
+                                           (unsigned)(H2 + t), full2);
         const long long anyfull = group_max<G>((full1 || full2) ? 1 : 0, red);
         if (anyfull) {
             if (t == 0) {
@@ -1371,7 +1372,8 @@ void launch_dense_direct(bool num, int blocks, const SpOperand &R, const SpOpera
 size_t fused_lds(int G, int H1, int H2)
 {
     const size_t gpb = 256 / G;
-    return gpb * ((size_t)H1 + H2) * 12 + gpb * group_waves(G) * 8 + gpb * (group_waves(G) + 1) * 4;
+    return gpb * ((size_t)H1 + H2 + 2 * (size_t)G) * 12 + gpb * group_waves(G) * 8 +
+           gpb * (group_waves(G) + 1) * 4;
 }
 
 using FusedFn = void (*)(SpOperand, SpOperand, SpOperand, const int32_t *, const int *, int, int, int, int,
