@@ -38,7 +38,7 @@ namespace sp3mm {
 //                 ub[r] = sum_{k in A_r} nnz(B_k), code/src/include/SpMMUtilsGeneric.h:112-139,
 //                 or a given exact length), its bin, a per-workgroup histogram
 //                 (LDS atomics) stored bin-major, and per-workgroup ub sum/max;
-//   k_bin_offsets one workgroup: exclusive scan of every bin's histogram over
+//   k_bin_offsets one workgroup per bin: exclusive scan of the bin's histogram over
 //                 the workgroups -> each workgroup's base inside the bin, bin
 //                 populations, ub totals;
 //   k_bin_scatter one thread per row: list[bin][base + rank] = row.
@@ -127,7 +127,8 @@ __global__ void __launch_bounds__(1024) k_bin_offsets(int *__restrict__ hist, in
                                                        unsigned long long *__restrict__ totals)
 {
     __shared__ long long sh[1024 / 64];
-    for (int b = 0; b < nbin; b++) {
+    const int b = blockIdx.x;  // workgroup b < nbin scans bin b; the last one sums the bounds
+    if (b < nbin) {
         int *hb = hist + (int64_t)b * nblk;
         long long carry = 0;
         for (int i0 = 0; i0 < nblk; i0 += 1024) {
@@ -139,9 +140,7 @@ __global__ void __launch_bounds__(1024) k_bin_offsets(int *__restrict__ hist, in
             carry += total;
         }
         if (threadIdx.x == 0) counts[b] += (int)carry;
-        __syncthreads();
-    }
-    if (totals) {
+    } else if (totals) {
         unsigned long long s = 0, m = 0;
         for (int i = threadIdx.x; i < nblk; i += 1024) {
             s += part[2 * (int64_t)i];
@@ -607,8 +606,10 @@ __global__ void __launch_bounds__(256) k_direct_fused(SpOperand R, SpOperand A, 
     }
 }
 
-// Move the rows of one bin from their scratch slots to the output (32 lanes
-// per row, two rows per wavefront; slots marked -1 belong to the long-row path).
+// Move the rows of one bin from their scratch slots to the output, L lanes per
+// row (several rows per wavefront, so their metadata loads overlap); slots
+// marked -1 belong to the long-row path.
+template <int L>
 __global__ void __launch_bounds__(256) k_gather_rows(const int32_t *__restrict__ rows, const int *__restrict__ count,
                                                       const int32_t *__restrict__ slotn, int cap,
                                                       const int32_t *__restrict__ scol,
@@ -617,10 +618,10 @@ __global__ void __launch_bounds__(256) k_gather_rows(const int32_t *__restrict__
                                                       double *__restrict__ cval, int64_t out_cap,
                                                       int *__restrict__ err)
 {
-    const int hpb = blockDim.x / 32;
-    const int hl = threadIdx.x & 31;
+    const int gpb = blockDim.x / L;
+    const int hl = threadIdx.x & (L - 1);
     const int n = *count;
-    for (int i = blockIdx.x * hpb + threadIdx.x / 32; i < n; i += gridDim.x * hpb) {
+    for (int i = blockIdx.x * gpb + threadIdx.x / L; i < n; i += gridDim.x * gpb) {
         const int len = slotn[i];
         if (len <= 0) continue;
         const int64_t src = (int64_t)i * cap, dst = crp[rows[i]];
@@ -628,7 +629,7 @@ __global__ void __launch_bounds__(256) k_gather_rows(const int32_t *__restrict__
             if (hl == 0) atomicOr(err, kErrOverflow);
             continue;
         }
-        for (int j = hl; j < len; j += 32) {
+        for (int j = hl; j < len; j += L) {
             ccol[dst + j] = scol[src + j];
             cval[dst + j] = sval[src + j];
         }
@@ -1010,7 +1011,8 @@ static void launch_bins(const SpOperand *A, const SpOperand *B, const int32_t *m
     part = reinterpret_cast<unsigned long long *>((reinterpret_cast<uintptr_t>(part) + 7) & ~(uintptr_t)7);
     hipLaunchKernelGGL(k_bin_hist, dim3(nblk), dim3(kBinThreads), 0, st, A ? A->rp : nullptr, A ? A->col : nullptr,
                        B ? B->rp32 : nullptr, m1, m2, M, ub, rownnz, hist, part, thr);
-    hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(1024), 0, st, hist, nblk, thr.nb + 1, part, counts, totals);
+    hipLaunchKernelGGL(k_bin_offsets, dim3(thr.nb + 2), dim3(1024), 0, st, hist, nblk, thr.nb + 1, part, counts,
+                       totals);
     hipLaunchKernelGGL(k_bin_scatter, dim3(nblk), dim3(kBinThreads), 0, st, ub, m1, m2, M, hist, lists, thr);
 }
 
@@ -1416,11 +1418,16 @@ void launch_gather_rows(const int32_t *rows, const int *count, int rows_host, co
                         const int32_t *scol, const double *sval, const int64_t *crp, int32_t *ccol, double *cval,
                         int64_t out_cap, int *err, int max_blocks, hipStream_t st)
 {
-    unsigned grid = (unsigned)((rows_host + 7) / 8);
+    const int L = cap <= 128 ? 16 : 32;
+    unsigned grid = (unsigned)((rows_host + 256 / L - 1) / (256 / L));
     if (grid > (unsigned)max_blocks) grid = (unsigned)max_blocks;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(k_gather_rows, dim3(grid), dim3(256), 0, st, rows, count, slotn, cap, scol, sval, crp, ccol,
-                       cval, out_cap, err);
+    if (L == 16)
+        hipLaunchKernelGGL(k_gather_rows<16>, dim3(grid), dim3(256), 0, st, rows, count, slotn, cap, scol, sval, crp,
+                           ccol, cval, out_cap, err);
+    else
+        hipLaunchKernelGGL(k_gather_rows<32>, dim3(grid), dim3(256), 0, st, rows, count, slotn, cap, scol, sval, crp,
+                           ccol, cval, out_cap, err);
 }
 
 __global__ void k_check_nnz(const int64_t *nnz, int64_t want, const int *nretry, int want_retry,
