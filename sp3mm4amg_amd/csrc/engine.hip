@@ -573,6 +573,143 @@ int spgemm_impl(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &C, sp3mm_s
     return 0;
 }
 
+// one-pass SpGEMM bins: table of H >= ub slots; 32-lane groups up to 256
+// slots, wavefronts up to 1024, workgroups beyond
+static void spg_bins(size_t max_lds, BinThresholds &thr, std::vector<BinCfg> &cfg)
+{
+    thr.nb = 0;
+    cfg.clear();
+    for (long long v = 64; thr.nb < kMaxBins - 1; v *= 2) {
+        const int H = (int)v;
+        const int G = H <= 256 ? 32 : H <= 1024 ? 64 : 256;
+        if (spgemm_fused_lds(G, H) > max_lds) break;
+        thr.v[thr.nb++] = v;
+        cfg.push_back({G, H});
+    }
+    cfg.push_back({0, 0});
+}
+
+// C = A*B in one pass over the operands: rows land in scratch slots, then a
+// scan and a gather build the output.  Rows beyond the largest LDS table run
+// on the dense long-row kernels (count, then numeric).
+static int spgemm_onepass(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &C, sp3mm_stats *st, Timer *tm,
+                          double *ms_num)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    if (check_pair(A, B)) return -1;
+    hipStream_t s = c->stream;
+    const int64_t M = A.M;
+    const SpOperand a = operand(A), b = operand(B);
+    BinThresholds thr{};
+    std::vector<BinCfg> cfg;
+    spg_bins(c->max_lds, thr, cfg);
+    Buf ctl(sizeof(Ctl)), ub((size_t)(M + 1) * 8), rown((size_t)(M + 1) * 4),
+        lists((size_t)(thr.nb + 1) * (M ? M : 1) * 4), slotn((size_t)(M ? M : 1) * 4),
+        tiles((size_t)scan_tiles(M) * 8), binscr((size_t)bin_scratch_ints(M) * 4);
+    if (!ctl.p || !ub.p || !rown.p || !lists.p || !slotn.p || !tiles.p || !binscr.p) {
+        set_error("scratch allocation failed");
+        return -1;
+    }
+    Ctl *d = ctl.as<Ctl>();
+    Ctl h;
+    SP3MM_HIP(hipMemsetAsync(d, 0, sizeof(Ctl), s));
+    launch_ub_bin(a, b, ub.as<int64_t>(), rown.as<int32_t>(), lists.as<int32_t>(), d->counts, thr, d->totals,
+                  binscr.as<int>(), s);
+    if (read_ctl(c, d, h)) return -1;
+    const long long mac = (long long)h.totals[0];
+    std::vector<int64_t> soff(thr.nb + 1, 0), nbase(thr.nb + 1, 0);
+    int64_t stot = 0, ntot = 0;
+    for (int i = 0; i < thr.nb; i++) {
+        soff[i] = stot;
+        nbase[i] = ntot;
+        stot += (int64_t)h.counts[i] * cfg[i].H;
+        ntot += h.counts[i];
+    }
+    Buf scol((size_t)(stot ? stot : 1) * 4), sval((size_t)(stot ? stot : 1) * 8);
+    if (!scol.p || !sval.p) {
+        set_error("scratch allocation (%lld entries) failed", (long long)stot);
+        return -1;
+    }
+    const int t0 = tm ? tm->mark() : -1;
+    for (int i = 0; i < thr.nb; i++) {
+        if (h.counts[i] == 0) continue;
+        launch_spgemm_fused(cfg[i].G, cfg[i].H, sub_log(B, cfg[i].G), a, b, lists.as<int32_t>() + (int64_t)i * M,
+                            d->counts + i, h.counts[i], rown.as<int32_t>(), slotn.as<int32_t>() + nbase[i],
+                            scol.as<int32_t>() + soff[i], sval.as<double>() + soff[i], &d->err, max_blocks(c), s);
+    }
+    const int t1 = tm ? tm->mark() : -1;
+    const int nlong = h.counts[thr.nb];
+    const int nwB = dense_words(B.N);
+    Buf dbits, dvals;
+    const int wb = dense_blocks(c, nlong);
+    if (nlong > 0) {  // long rows: dense bitmap (+ values) in HBM
+        dbits = Buf((size_t)wb * nwB * 4);
+        dvals = Buf((size_t)wb * nwB * 32 * 8);
+        if (!dbits.p || !dvals.p) {
+            set_error("dense workspace allocation failed");
+            return -1;
+        }
+        SP3MM_HIP(hipMemsetAsync(dbits.p, 0, dbits.bytes, s));
+        SP3MM_HIP(hipMemsetAsync(dvals.p, 0, dvals.bytes, s));
+        launch_dense_spgemm(false, wb, a, b, lists.as<int32_t>() + (int64_t)thr.nb * M, d->counts + thr.nb,
+                            dbits.as<unsigned>(), nullptr, nwB, sub_log(B, kDenseThreads), rown.as<int32_t>(),
+                            nullptr, nullptr, nullptr, &d->err, s);
+    }
+    if (dmat_alloc(C, M, B.N, 0, SP3MM_F64)) return -1;
+    launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d->nnz, s);
+    if (read_ctl(c, d, h)) return -1;
+    if (h.err) {
+        set_error("SpGEMM: %s", err_text(h.err));
+        return -1;
+    }
+    const int64_t nnz = h.nnz;
+    C.nnz = nnz;
+    c->pool.release(C.col);
+    c->pool.release(C.val);
+    C.col = static_cast<int32_t *>(c->pool.alloc((size_t)(nnz ? nnz : 1) * 4));
+    C.val = c->pool.alloc((size_t)(nnz ? nnz : 1) * 8);
+    if (!C.col || !C.val) {
+        set_error("output allocation (%lld nnz) failed", (long long)nnz);
+        return -1;
+    }
+    if (nnz >= INT32_MAX) {
+        c->pool.release(C.rp32);
+        C.rp32 = nullptr;
+    }
+    for (int i = 0; i < thr.nb; i++) {
+        if (h.counts[i] == 0) continue;
+        launch_gather_rows(lists.as<int32_t>() + (int64_t)i * M, d->counts + i, h.counts[i],
+                           slotn.as<int32_t>() + nbase[i], cfg[i].H, scol.as<int32_t>() + soff[i],
+                           sval.as<double>() + soff[i], C.rp, C.col, static_cast<double *>(C.val), nnz, &d->err,
+                           max_blocks(c), s);
+    }
+    if (nlong > 0)
+        launch_dense_spgemm(true, wb, a, b, lists.as<int32_t>() + (int64_t)thr.nb * M, d->counts + thr.nb,
+                            dbits.as<unsigned>(), dvals.as<double>(), nwB, sub_log(B, kDenseThreads), nullptr, C.rp,
+                            C.col, static_cast<double *>(C.val), &d->err, s);
+    if (read_ctl(c, d, h)) return -1;
+    if (h.err) {
+        set_error("SpGEMM: %s", err_text(h.err));
+        return -1;
+    }
+    if (tm && ms_num) *ms_num += tm->ms(t0, t1);
+    if (st) {
+        st->mac1 += mac;
+        st->nnz_out = nnz;
+    }
+    return 0;
+}
+
+// the one-pass SpGEMM unless SP3MM_SPGEMM_TWOPASS is set (symbolic + numeric)
+static int spgemm_any(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &C, sp3mm_stats *st, Timer *tm,
+                      double *ms_num)
+{
+    static const bool twopass = getenv("SP3MM_SPGEMM_TWOPASS") != nullptr;
+    if (twopass) return spgemm_impl(A, B, C, st, tm, ms_num);
+    return spgemm_onepass(A, B, C, st, tm, ms_num);
+}
+
 int spgemm(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &C, sp3mm_stats *st, bool symbolic_only)
 {
     Context *c = ctx();
@@ -582,7 +719,8 @@ int spgemm(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &C, sp3mm_stats 
     const int e0 = tm.mark();
     double ms_num = 0;
     sp3mm_stats local{};
-    const int rc = spgemm_impl(A, B, C, &local, &tm, &ms_num, symbolic_only);
+    const int rc = symbolic_only ? spgemm_impl(A, B, C, &local, &tm, &ms_num, true)
+                                 : spgemm_any(A, B, C, &local, &tm, &ms_num);
     if (rc) {
         dmat_release(C);
         return rc;
@@ -613,11 +751,11 @@ int rap_pair(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, sp3m
     double ms1 = 0, ms2 = 0;
     sp3mm_dmat RA;
     memset(&RA, 0, sizeof RA);
-    if (spgemm_impl(R, A, RA, &s1, &tm, &ms1)) {
+    if (spgemm_any(R, A, RA, &s1, &tm, &ms1)) {
         dmat_release(RA);
         return -1;
     }
-    const int rc = spgemm_impl(RA, P, C, &s2, &tm, &ms2);
+    const int rc = spgemm_any(RA, P, C, &s2, &tm, &ms2);
     const int64_t mid = RA.nnz;
     dmat_release(RA);
     if (rc) {

@@ -606,6 +606,99 @@ __global__ void __launch_bounds__(256) k_direct_fused(SpOperand R, SpOperand A, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// SpGEMM C = A*B in one pass
+// ---------------------------------------------------------------------------
+// Row r of A*B is accumulated in an LDS table of H >= its upper bound slots
+// (branch-free expansion, so it cannot fill), compacted, rank-sorted and
+// written to its scratch slot of H entries (slot i of the bin list -> scratch
+// + i*H); its length goes to rownnz[r] and slotn[i].  After the scan of
+// rownnz, k_gather_rows moves every slot to its final place.
+template <int G, int U>
+__global__ void __launch_bounds__(256) k_spgemm_fused(SpOperand A, SpOperand B, const int32_t *__restrict__ rows,
+                                                       const int *__restrict__ count, int H, int slog,
+                                                       int32_t *__restrict__ rownnz, int32_t *__restrict__ slotn,
+                                                       int32_t *__restrict__ scol, double *__restrict__ sval,
+                                                       int *__restrict__ err)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int GPB = 256 / G;
+    const int g = threadIdx.x / G;
+    const int t = Group<G>::tid();
+    const int W = H + G;  // H slots + one dummy slot per lane
+    unsigned char *p = smem;
+    double *v = reinterpret_cast<double *>(p) + g * W;
+    p += (size_t)GPB * W * 8;
+    int *k = reinterpret_cast<int *>(p) + g * W;
+    p += (size_t)GPB * W * 4;
+    int *ired = reinterpret_cast<int *>(p) + g * (Group<G>::kWaves + 1);
+    const unsigned m = (unsigned)H - 1;
+    const int sh = 32 - ilog2((unsigned)H);
+    const int n = *count;
+    const double *aval = static_cast<const double *>(A.val);
+    const double *bval = static_cast<const double *>(B.val);
+    const int stride = gridDim.x * GPB;
+    int i = blockIdx.x * GPB + g;
+    int r = i < n ? rows[i] : 0;
+    int64_t rb = 0, re = 0;
+    if (i < n) {
+        rb = A.rp[r];
+        re = A.rp[r + 1];
+    }
+    for (; i < n; i += stride) {
+        const int rn = i + stride < n ? rows[i + stride] : 0;
+        for (int s = t; s < W; s += G) {
+            k[s] = kEmpty;
+            v[s] = 0.0;
+        }
+        Group<G>::sync();
+        bool full = false;
+        expand_flat<G, double, double, U>(A.col, aval, rb, re, B.rp32, B.col, bval, slog, k, v, m, sh,
+                                          (unsigned)(H + t), full);
+        const int64_t rbn = A.rp[rn], ren = A.rp[rn + 1];
+        if (full) atomicOr(err, kErrHashFull);  // H >= the row's upper bound: cannot happen
+        Group<G>::sync();
+        const int nn = compact_table<G, double>(k, v, H, ired);
+        const int64_t off = (int64_t)i * H;
+        for (int j = t; j < nn; j += G) {
+            const int key = k[j];
+            const int rk = key_rank(k, nn, key);
+            scol[off + rk] = key;
+            sval[off + rk] = v[j];
+        }
+        if (t == 0) {
+            slotn[i] = nn;
+            rownnz[r] = nn;
+        }
+        Group<G>::sync();
+        r = rn;
+        rb = rbn;
+        re = ren;
+    }
+}
+
+size_t spgemm_fused_lds(int G, int H)
+{
+    const size_t gpb = 256 / G;
+    return gpb * ((size_t)H + G) * 12 + gpb * (group_waves(G) + 1) * 4;
+}
+
+static unsigned grid_for(int rows, int G, int max_blocks);
+
+void launch_spgemm_fused(int G, int H, int slog, const SpOperand &A, const SpOperand &B, const int32_t *rows,
+                         const int *count, int rows_host, int32_t *rownnz, int32_t *slotn, int32_t *scol,
+                         double *sval, int *err, int max_blocks, hipStream_t st)
+{
+    const unsigned grid = grid_for(rows_host, G, max_blocks);
+    const size_t lds = spgemm_fused_lds(G, H);
+    slog = std::min(slog, ilog2((unsigned)G));
+#define SP3MM_SPG_ARGS A, B, rows, count, H, slog, rownnz, slotn, scol, sval, err
+    if (G == 32) hipLaunchKernelGGL((k_spgemm_fused<32, 4>), dim3(grid), dim3(256), lds, st, SP3MM_SPG_ARGS);
+    else if (G == 64) hipLaunchKernelGGL((k_spgemm_fused<64, 4>), dim3(grid), dim3(256), lds, st, SP3MM_SPG_ARGS);
+    else hipLaunchKernelGGL((k_spgemm_fused<256, 4>), dim3(grid), dim3(256), lds, st, SP3MM_SPG_ARGS);
+#undef SP3MM_SPG_ARGS
+}
+
 // Move the rows of one bin from their scratch slots to the output, L lanes per
 // row (several rows per wavefront, so their metadata loads overlap); slots
 // marked -1 belong to the long-row path.

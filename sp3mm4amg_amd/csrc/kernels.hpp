@@ -96,6 +96,11 @@ void launch_direct_fused(int G, int H1, int H2, int slog1, int slog2, const SpOp
                          int32_t *rownnz, int32_t *slotn, int32_t *scol, double *sval, int32_t *retry,
                          int *nretry, int *err, int max_blocks, hipStream_t st, bool f32in,
                          const Tuning &tu);
+// one-pass SpGEMM rows into scratch slots of H entries (see k_spgemm_fused)
+size_t spgemm_fused_lds(int G, int H);
+void launch_spgemm_fused(int G, int H, int slog, const SpOperand &A, const SpOperand &B, const int32_t *rows,
+                         const int *count, int rows_host, int32_t *rownnz, int32_t *slotn, int32_t *scol,
+                         double *sval, int *err, int max_blocks, hipStream_t st);
 void launch_gather_rows(const int32_t *rows, const int *count, int rows_host, const int32_t *slotn, int cap,
                         const int32_t *scol, const double *sval, const int64_t *crp, int32_t *ccol, double *cval,
                         int64_t out_cap, int *err, int max_blocks, hipStream_t st);
