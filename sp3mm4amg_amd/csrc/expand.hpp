@@ -72,6 +72,7 @@ __device__ __forceinline__ void expand_gen(int64_t e0, int64_t e1, LoadK loadk, 
                 sl[u] = hslot(c[u], shift) & mask;
                 old[u] = c[u] != kEmpty ? atomicCAS(keys + sl[u], kEmpty, c[u]) : kEmpty;
             }
+            __builtin_amdgcn_sched_barrier(0);  // the U swaps stay in flight together
             bool pend = false;
 #pragma unroll
             for (int u = 0; u < U; u++) {
@@ -154,10 +155,12 @@ __device__ __forceinline__ void expand_flat(const int32_t *lcol, const LV *lval,
                 q[u] += S;
             }
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                sl[u] = c[u] != kEmpty ? (hslot(c[u], shift) & mask) : dslot;
-                old[u] = atomicCAS(keys + sl[u], kEmpty, c[u]);
-            }
+            for (int u = 0; u < U; u++) sl[u] = c[u] != kEmpty ? (hslot(c[u], shift) & mask) : dslot;
+#pragma unroll
+            for (int u = 0; u < U; u++) old[u] = atomicCAS(keys + sl[u], kEmpty, c[u]);
+            // keep the U compare-and-swaps in flight together: nothing that
+            // consumes their results is scheduled above this point
+            __builtin_amdgcn_sched_barrier(0);
             bool pend = false;
 #pragma unroll
             for (int u = 0; u < U; u++) {

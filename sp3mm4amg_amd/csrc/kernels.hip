@@ -513,7 +513,7 @@ __global__ void __launch_bounds__(256) k_direct(SpOperand R, SpOperand A, SpOper
 // slotn[i] = -1 and queued on `retry` for the long-row kernels.  After the
 // scan of rownnz, k_gather_rows moves every slot to its final place.
 template <int G, int U1, int U2, typename VT = double>
-__global__ void __launch_bounds__(256) k_direct_fused(SpOperand R, SpOperand A, SpOperand P,
+__global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand A, SpOperand P,
                                                        const int32_t *__restrict__ rows,
                                                        const int *__restrict__ count, int H1, int H2,
                                                        int slog1, int slog2, int32_t *__restrict__ rownnz,

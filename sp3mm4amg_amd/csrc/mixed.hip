@@ -185,6 +185,7 @@ __device__ __forceinline__ void expand_mixed(const int32_t *lcol, int64_t e0, in
                 sl[u] = hslot(c[u], shift) & mask;
                 old[u] = c[u] != kEmpty ? atomicCAS(keys + sl[u], kEmpty, c[u]) : kEmpty;
             }
+            __builtin_amdgcn_sched_barrier(0);  // the U swaps stay in flight together
             bool pend = false;
 #pragma unroll
             for (int u = 0; u < U; u++) {
