@@ -986,6 +986,23 @@ struct FusedPlan {
     std::vector<int64_t> soff, nbase;
     int nlong = 0, nretry = 0, nspill = 0;
     bool built = false;  // build() has read the retry and spill counts
+    bool spec = false;   // tail launched before the retry count is known
+    int64_t ccap = 0;    // entries allocated for C.col / C.val
+
+    // (re)allocates the output arrays for `cap` entries
+    int alloc_out(Context *c, int64_t cap)
+    {
+        c->pool.release(C.col);
+        c->pool.release(C.val);
+        C.col = static_cast<int32_t *>(c->pool.alloc((size_t)(cap ? cap : 1) * 4));
+        C.val = c->pool.alloc((size_t)(cap ? cap : 1) * 8);
+        ccap = cap;
+        if (!C.col || !C.val) {
+            set_error("output allocation (%lld entries) failed", (long long)cap);
+            return -1;
+        }
+        return 0;
+    }
     int64_t nnz = 0, stot = 0;
     long long mac1 = 0;
     int slog1 = 0, slog2 = 0, dlog1 = 0, dlog2 = 0;
@@ -1023,7 +1040,7 @@ struct FusedPlan {
             if (cnt[i] == 0) continue;
             launch_gather_rows(lists.as<int32_t>() + (int64_t)i * M, d()->counts + i, cnt[i],
                                slotn.as<int32_t>() + nbase[i], cfg[i].H2, scol.as<int32_t>() + soff[i],
-                               sval.as<double>() + soff[i], C.rp, C.col, static_cast<double *>(C.val), nnz,
+                               sval.as<double>() + soff[i], C.rp, C.col, static_cast<double *>(C.val), ccap,
                                &d()->err, max_blocks(c), s);
         }
         launch_long(c, s, true);
@@ -1039,15 +1056,16 @@ struct FusedPlan {
         double *cv = num ? static_cast<double *>(C.val) : nullptr;
         int32_t *rn = num ? nullptr : rown.as<int32_t>();
         double *rs = num && mixed ? rowsum : nullptr;
+        const int64_t cap = num ? ccap : 0;
         if (nlong > 0)
             launch_window_direct(num, std::min(nlong, c->num_cu), wl, r(), a(), p(),
                                  lists.as<int32_t>() + (int64_t)thr.nb * M, d()->counts + thr.nb, dlog1,
                                  spill.as<int32_t>(), &d()->nspill, rn, num ? C.rp : nullptr, num ? C.col : nullptr,
-                                 cv, rs, &d()->err, s, f32);
-        if (nretry > 0 || (!num && !built))
+                                 cv, rs, cap, &d()->err, s, f32);
+        if (nretry > 0 || (!num && !built) || (num && spec))
             launch_window_direct(num, c->num_cu, wl, r(), a(), p(), retry.as<int32_t>(), &d()->nretry, dlog1,
                                  spill.as<int32_t>(), &d()->nspill, rn, num ? C.rp : nullptr, num ? C.col : nullptr,
-                                 cv, rs, &d()->err, s, f32);
+                                 cv, rs, cap, &d()->err, s, f32);
         if (nspill > 0)
             launch_dense_direct(num, dense_blocks(c, nspill), r(), a(), p(), spill.as<int32_t>(), &d()->nspill,
                                 sws.ws, dlog1, dlog2, nullptr, rn, num ? C.rp : nullptr, num ? C.col : nullptr, cv,
@@ -1126,6 +1144,36 @@ struct FusedPlan {
         launch_long(c, s, false);
         if (dmat_alloc(C, M, P.N, 0, SP3MM_F64)) return -1;
         launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d()->nnz, s);
+        // Without long rows every row that stayed on the hash tables fits its
+        // scratch slot, so the slot total is a likely output size: allocate it
+        // and run the tail without reading the counters back.  Every writer
+        // checks the capacity; if a retried row outgrew it, or a row spilled
+        // to the global dense kernel, the exact path below redoes the tail.
+        if (nlong == 0 && !mixed && stot > 0 && stot < INT32_MAX) {
+            if (alloc_out(c, stot)) return -1;
+            spec = true;
+            launch_tail(c, s);
+            spec = false;
+            if (read_ctl(c, d(), h)) return -1;
+            if (!h.err && h.nspill == 0 && h.nnz <= ccap) {
+                nretry = h.nretry;
+                nspill = 0;
+                built = true;
+                fcnt.assign(h.counts2, h.counts2 + thr.nb + 1);
+                nnz = h.nnz;
+                C.nnz = nnz;
+                const int es = tm.mark();
+                ms_ub = tm.ms(e0, eu);
+                ms_num = tm.ms(en0, en1);
+                ms_scan = tm.ms(en1, es);
+                return 0;
+            }
+            if (h.err & ~kErrOverflow) {
+                set_error("direct product: %s", err_text(h.err));
+                return -1;
+            }
+            SP3MM_HIP(hipMemsetAsync(&d()->err, 0, sizeof(int), s));
+        }
         if (read_ctl(c, d(), h)) return -1;
         nretry = h.nretry;
         nspill = h.nspill;
@@ -1146,14 +1194,7 @@ struct FusedPlan {
         fcnt.assign(h.counts2, h.counts2 + thr.nb + 1);
         nnz = h.nnz;
         C.nnz = nnz;
-        c->pool.release(C.col);
-        c->pool.release(C.val);
-        C.col = static_cast<int32_t *>(c->pool.alloc((size_t)(nnz ? nnz : 1) * 4));
-        C.val = c->pool.alloc((size_t)(nnz ? nnz : 1) * 8);
-        if (!C.col || !C.val) {
-            set_error("output allocation (%lld nnz) failed", (long long)nnz);
-            return -1;
-        }
+        if (alloc_out(c, nnz)) return -1;
         if (nnz >= INT32_MAX) {
             c->pool.release(C.rp32);
             C.rp32 = nullptr;

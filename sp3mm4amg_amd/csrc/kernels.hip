@@ -1261,7 +1261,8 @@ __global__ void __launch_bounds__(kWin) k_window_direct(SpOperand R, SpOperand A
                                                            int32_t *__restrict__ rownnz,
                                                            const int64_t *__restrict__ crp,
                                                            int32_t *__restrict__ ccol, double *__restrict__ cval,
-                                                           double *__restrict__ rowsum, int *__restrict__ err)
+                                                           double *__restrict__ rowsum, int64_t out_cap,
+                                                           int *__restrict__ err)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     double *V = reinterpret_cast<double *>(smem);                  // cap values
@@ -1392,6 +1393,11 @@ __global__ void __launch_bounds__(kWin) k_window_direct(SpOperand R, SpOperand A
             }
             // sorted emission of region 2
             const int64_t off = crp[r], len = crp[r + 1] - off;
+            if (off + len > out_cap) {  // output allocated smaller than this row needs
+                if (t == 0) atomicOr(err, kErrOverflow);
+                __syncthreads();
+                continue;
+            }
             long long base = 0;
             for (int w0 = 0; w0 < nw2; w0 += kWin) {
                 const int w = w0 + t;
@@ -1426,12 +1432,12 @@ int window_cap(size_t lds)
 void launch_window_direct(bool num, int blocks, size_t lds, const SpOperand &R, const SpOperand &A,
                           const SpOperand &P, const int32_t *rows, const int *count, int slog1, int32_t *spill,
                           int *nspill, int32_t *rownnz, const int64_t *crp, int32_t *ccol, double *cval,
-                          double *rowsum, int *err, hipStream_t st, bool f32in)
+                          double *rowsum, int64_t out_cap, int *err, hipStream_t st, bool f32in)
 {
     const int cap = window_cap(lds);
     const size_t bytes = (size_t)cap * 8 + (size_t)cap / 32 * 4;
     if (slog1 > 6) slog1 = 6;
-#define SP3MM_WIN_ARGS R, A, P, rows, count, cap, slog1, spill, nspill, rownnz, crp, ccol, cval, rowsum, err
+#define SP3MM_WIN_ARGS R, A, P, rows, count, cap, slog1, spill, nspill, rownnz, crp, ccol, cval, rowsum, out_cap, err
     if (f32in) {
         if (num) hipLaunchKernelGGL((k_window_direct<true, float>), dim3(blocks), dim3(kWin), bytes, st, SP3MM_WIN_ARGS);
         else hipLaunchKernelGGL((k_window_direct<false, float>), dim3(blocks), dim3(kWin), bytes, st, SP3MM_WIN_ARGS);
