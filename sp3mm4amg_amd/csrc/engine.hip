@@ -658,22 +658,27 @@ static int spgemm_onepass(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &
     }
     if (dmat_alloc(C, M, B.N, 0, SP3MM_F64)) return -1;
     launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d->nnz, s);
-    if (read_ctl(c, d, h)) return -1;
-    if (h.err) {
-        set_error("SpGEMM: %s", err_text(h.err));
-        return -1;
+    // Without long rows every row fits its slot of H >= its upper bound, so the
+    // slot total bounds nnz: allocate it and gather without reading the count.
+    const bool bounded = nlong == 0 && stot < INT32_MAX;
+    int64_t cap = stot;
+    if (!bounded) {
+        if (read_ctl(c, d, h)) return -1;
+        if (h.err) {
+            set_error("SpGEMM: %s", err_text(h.err));
+            return -1;
+        }
+        cap = h.nnz;
     }
-    const int64_t nnz = h.nnz;
-    C.nnz = nnz;
     c->pool.release(C.col);
     c->pool.release(C.val);
-    C.col = static_cast<int32_t *>(c->pool.alloc((size_t)(nnz ? nnz : 1) * 4));
-    C.val = c->pool.alloc((size_t)(nnz ? nnz : 1) * 8);
+    C.col = static_cast<int32_t *>(c->pool.alloc((size_t)(cap ? cap : 1) * 4));
+    C.val = c->pool.alloc((size_t)(cap ? cap : 1) * 8);
     if (!C.col || !C.val) {
-        set_error("output allocation (%lld nnz) failed", (long long)nnz);
+        set_error("output allocation (%lld entries) failed", (long long)cap);
         return -1;
     }
-    if (nnz >= INT32_MAX) {
+    if (!bounded && cap >= INT32_MAX) {
         c->pool.release(C.rp32);
         C.rp32 = nullptr;
     }
@@ -681,7 +686,7 @@ static int spgemm_onepass(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &
         if (h.counts[i] == 0) continue;
         launch_gather_rows(lists.as<int32_t>() + (int64_t)i * M, d->counts + i, h.counts[i],
                            slotn.as<int32_t>() + nbase[i], cfg[i].H, scol.as<int32_t>() + soff[i],
-                           sval.as<double>() + soff[i], C.rp, C.col, static_cast<double *>(C.val), nnz, &d->err,
+                           sval.as<double>() + soff[i], C.rp, C.col, static_cast<double *>(C.val), cap, &d->err,
                            max_blocks(c), s);
     }
     if (nlong > 0)
@@ -693,10 +698,11 @@ static int spgemm_onepass(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &
         set_error("SpGEMM: %s", err_text(h.err));
         return -1;
     }
+    C.nnz = h.nnz;
     if (tm && ms_num) *ms_num += tm->ms(t0, t1);
     if (st) {
         st->mac1 += mac;
-        st->nnz_out = nnz;
+        st->nnz_out = h.nnz;
     }
     return 0;
 }
