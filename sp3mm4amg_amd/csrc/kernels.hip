@@ -527,16 +527,15 @@ __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand 
     const int g = threadIdx.x / G;
     const int t = Group<G>::tid();
     unsigned char *p = smem;
-    // each table is followed by G dummy slots, one per lane (see expand_flat)
-    const int W1 = H1 + G, W2 = H2 + G;
-    double *v1 = reinterpret_cast<double *>(p) + g * W1;
-    p += (size_t)GPB * W1 * 8;
-    double *v2 = reinterpret_cast<double *>(p) + g * W2;
-    p += (size_t)GPB * W2 * 8;
-    int *k1 = reinterpret_cast<int *>(p) + g * W1;
-    p += (size_t)GPB * W1 * 4;
-    int *k2 = reinterpret_cast<int *>(p) + g * W2;
-    p += (size_t)GPB * W2 * 4;
+    // per group: values [table 1 | table 2 | G dummy slots], keys the same;
+    // the dummy slots (one per lane, see expand_flat) serve both tables
+    const int W = H1 + H2 + G;
+    double *v1 = reinterpret_cast<double *>(p) + g * W;
+    double *v2 = v1 + H1;
+    p += (size_t)GPB * W * 8;
+    int *k1 = reinterpret_cast<int *>(p) + g * W;
+    int *k2 = k1 + H1;
+    p += (size_t)GPB * W * 4;
     long long *red = reinterpret_cast<long long *>(p) + g * Group<G>::kWaves;
     p += (size_t)GPB * Group<G>::kWaves * 8;
     int *ired = reinterpret_cast<int *>(p) + g * (Group<G>::kWaves + 1);
@@ -559,18 +558,14 @@ __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand 
     }
     for (; i < n; i += stride) {
         const int rn = i + stride < n ? rows[i + stride] : 0;
-        for (int s = t; s < W1; s += G) {
+        for (int s = t; s < W; s += G) {
             k1[s] = kEmpty;
             v1[s] = 0.0;
-        }
-        for (int s = t; s < W2; s += G) {
-            k2[s] = kEmpty;
-            v2[s] = 0.0;
         }
         Group<G>::sync();
         bool full1 = false, full2 = false;
         expand_flat<G, VT, VT, U1>(R.col, rval, rb, re, A.rp32, A.col, aval, slog1, k1, v1, m1, sh1,
-                                   (unsigned)(H1 + t), full1);
+                                   (unsigned)(H1 + H2 + t), full1);
         const int64_t rbn = R.rp[rn], ren = R.rp[rn + 1];
         Group<G>::sync();
         const int nn1 = compact_table<G, double>(k1, v1, H1, ired);
@@ -1473,8 +1468,7 @@ void launch_dense_direct(bool num, int blocks, const SpOperand &R, const SpOpera
 size_t fused_lds(int G, int H1, int H2)
 {
     const size_t gpb = 256 / G;
-    return gpb * ((size_t)H1 + H2 + 2 * (size_t)G) * 12 + gpb * group_waves(G) * 8 +
-           gpb * (group_waves(G) + 1) * 4;
+    return gpb * ((size_t)H1 + H2 + (size_t)G) * 12 + gpb * group_waves(G) * 8 + gpb * (group_waves(G) + 1) * 4;
 }
 
 using FusedFn = void (*)(SpOperand, SpOperand, SpOperand, const int32_t *, const int *, int, int, int, int,
