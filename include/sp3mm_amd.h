@@ -272,7 +272,8 @@ void sp3mm_batch_free(sp3mm_batch *b);
  * group shapes and the spread of work over lanes; results are the same up to
  * FP64 summation order. */
 typedef struct {
-    int32_t u1, u2;      /* left entries each lane holds in stage 1 / stage 2: 2 or 4 */
+    int32_t u1, u2;      /* left entries each lane holds in stage 1 (2 or 4) / stage 2
+                            (2, 3 or 4) */
     int32_t subwave_h;   /* largest stage-1 table run by 32-lane groups (two rows per
                             wavefront); 0, or a power of two in [64, 4096] */
     int32_t slog1_cap;   /* largest lane sub-group (log2) per left entry in stage 1, 0..8

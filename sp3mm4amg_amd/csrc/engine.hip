@@ -100,7 +100,7 @@ int tuning_valid(const Tuning &t)
 {
     const bool pow2h = t.subwave_h == 0 || (t.subwave_h >= 64 && t.subwave_h <= 4096 &&
                                             (t.subwave_h & (t.subwave_h - 1)) == 0);
-    return (t.u1 == 2 || t.u1 == 4) && (t.u2 == 2 || t.u2 == 4) && pow2h && t.slog1_cap >= 0 &&
+    return (t.u1 == 2 || t.u1 == 4) && (t.u2 >= 2 && t.u2 <= 4) && pow2h && t.slog1_cap >= 0 &&
            t.slog1_cap <= 8 && t.slog2_cap >= 0 && t.slog2_cap <= 8 && t.h1_shift >= 0 && t.h1_shift <= 2 &&
            (t.h2_div == 1 || t.h2_div == 2 || t.h2_div == 4 || t.h2_div == 8);
 }
@@ -1377,7 +1377,7 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
         std::vector<int> values;
     };
     const Axis axes[] = {
-        {&Tuning::subwave_h, {0, 128, 256, 512}}, {&Tuning::u1, {2, 4}},       {&Tuning::u2, {2, 4}},
+        {&Tuning::subwave_h, {0, 128, 256, 512}}, {&Tuning::u1, {2, 4}},       {&Tuning::u2, {2, 3, 4}},
         {&Tuning::slog1_cap, {1, 2, 3}},            {&Tuning::slog2_cap, {1, 2}},
         {&Tuning::h1_shift, {0, 1}},                {&Tuning::h2_div, {2, 4, 8}},
     };

@@ -147,7 +147,8 @@ __device__ __forceinline__ void expand_flat(const int32_t *lcol, const LV *lval,
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const bool ok = q[u] < qe[u];
-                const int qs = ok ? q[u] : qe[u] - 1 > 0 ? qe[u] - 1 : 0;  // some valid entry of B
+                // some valid entry of B (the loop runs only while B has entries)
+                const unsigned qs = (unsigned)max(min(q[u], qe[u] - 1), 0);
                 const int cc = bcol[qs];
                 const double ww = (double)bval[qs];
                 c[u] = ok ? cc : kEmpty;
@@ -161,16 +162,18 @@ __device__ __forceinline__ void expand_flat(const int32_t *lcol, const LV *lval,
             // keep the U compare-and-swaps in flight together: nothing that
             // consumes their results is scheduled above this point
             __builtin_amdgcn_sched_barrier(0);
+            // An absent product swapped its dummy slot (old == kEmpty), so it
+            // counts as a hit on sl == dslot and needs no separate test.
             bool pend = false;
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                const bool hit = c[u] != kEmpty && (old[u] == kEmpty || old[u] == c[u]);
-                pend |= c[u] != kEmpty && !hit;
+                const bool hit = old[u] == kEmpty || old[u] == c[u];
+                pend |= !hit;
                 atomicAdd(vals + (hit ? sl[u] : dslot), lv[u] * w[u]);
             }
             if (pend) {
                 for (int u = 0; u < U; u++) {
-                    if (c[u] == kEmpty || old[u] == kEmpty || old[u] == c[u]) continue;
+                    if (old[u] == kEmpty || old[u] == c[u]) continue;
                     unsigned s2 = sl[u];
                     bool done = false;
                     for (unsigned probe = 1; probe <= mask; probe++) {

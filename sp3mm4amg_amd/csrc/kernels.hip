@@ -1474,15 +1474,15 @@ size_t fused_lds(int G, int H1, int H2)
 using FusedFn = void (*)(SpOperand, SpOperand, SpOperand, const int32_t *, const int *, int, int, int, int,
                         int32_t *, int32_t *, int32_t *, double *, int32_t *, int *, int *);
 
-// left entries held per lane in stage 1 / stage 2 (index = log2(U) - 1)
+// left entries held per lane: stage 1 U1 in {2, 4}, stage 2 U2 in {2, 3, 4}
 template <int G, typename VT>
 static FusedFn fused_variant(int u1, int u2)
 {
-    static const FusedFn tab[2][2] = {
-        {k_direct_fused<G, 2, 2, VT>, k_direct_fused<G, 2, 4, VT>},
-        {k_direct_fused<G, 4, 2, VT>, k_direct_fused<G, 4, 4, VT>},
+    static const FusedFn tab[2][3] = {
+        {k_direct_fused<G, 2, 2, VT>, k_direct_fused<G, 2, 3, VT>, k_direct_fused<G, 2, 4, VT>},
+        {k_direct_fused<G, 4, 2, VT>, k_direct_fused<G, 4, 3, VT>, k_direct_fused<G, 4, 4, VT>},
     };
-    return tab[u1][u2];
+    return tab[u1 == 4 ? 1 : 0][u2 == 2 ? 0 : u2 == 3 ? 1 : 2];
 }
 
 void launch_direct_fused(int G, int H1, int H2, int slog1, int slog2, const SpOperand &R, const SpOperand &A,
@@ -1493,7 +1493,7 @@ void launch_direct_fused(int G, int H1, int H2, int slog1, int slog2, const SpOp
 {
     const unsigned grid = grid_for(rows_host, G, max_blocks);
     const size_t lds = fused_lds(G, H1, H2);
-    const int u1 = tu.u1 == 2 ? 0 : 1, u2 = tu.u2 == 2 ? 0 : 1;
+    const int u1 = tu.u1, u2 = tu.u2;
     const int cap = ilog2((unsigned)G);
     slog1 = std::min(slog1, std::min(cap, std::max(0, tu.slog1_cap)));
     slog2 = std::min(slog2, std::min(cap, std::max(0, tu.slog2_cap)));

@@ -83,7 +83,7 @@ size_t fused_lds(int G, int H1, int H2);
 // sizes, group shapes and how work is spread over lanes; results are the same
 // up to FP64 summation order.
 struct Tuning {
-    int u1 = 4, u2 = 4;     // left entries held per lane in stage 1 / stage 2 (2 or 4)
+    int u1 = 4, u2 = 4;     // left entries held per lane in stage 1 (2, 4) / stage 2 (2, 3, 4)
     int subwave_h = 256;    // largest stage-1 table run by 32-lane groups (0: none)
     int slog1_cap = 3;      // largest lane sub-group (log2) per left entry, stage 1
     int slog2_cap = 1;      // the same, stage 2
