@@ -347,7 +347,8 @@ constexpr int kDenseThreads = 256;
 // persistent workgroups (each with its own dense workspace) for the long-row kernels
 static int dense_blocks(const Context *c, int rows)
 {
-    const int cap = c->num_cu * 2;
+    static const int env = getenv("SP3MM_DENSE_BLOCKS") ? atoi(getenv("SP3MM_DENSE_BLOCKS")) : 0;
+    const int cap = env > 0 ? env : c->num_cu * 2;
     return rows < cap ? (rows > 0 ? rows : 1) : cap;
 }
 

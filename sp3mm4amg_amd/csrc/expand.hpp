@@ -113,7 +113,7 @@ template <int G, typename LV, typename BV, int U>
 __device__ __forceinline__ void expand_flat(const int32_t *lcol, const LV *lval, int64_t e0, int64_t e1,
                                             const int32_t *__restrict__ brp, const int32_t *__restrict__ bcol,
                                             const BV *__restrict__ bval, int slog, int *keys, double *vals,
-                                            unsigned mask, int shift, unsigned dslot, bool &full)
+                                            unsigned mask, int shift, unsigned dslot, bool &full, int *fullflag)
 {
     constexpr int kLogG = G >= 256 ? 8 : G >= 128 ? 7 : G >= 64 ? 6 : G >= 32 ? 5 : 4;
     if (slog > kLogG) slog = kLogG;
@@ -174,6 +174,12 @@ __device__ __forceinline__ void expand_flat(const int32_t *lcol, const LV *lval,
             if (pend) {
                 for (int u = 0; u < U; u++) {
                     if (old[u] == kEmpty || old[u] == c[u]) continue;
+                    // once any lane of the group found the table full, the row
+                    // is redone elsewhere: do not walk the probe sequence again
+                    if (__hip_atomic_load(fullflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                        full = true;
+                        continue;
+                    }
                     unsigned s2 = sl[u];
                     bool done = false;
                     for (unsigned probe = 1; probe <= mask; probe++) {
@@ -185,7 +191,10 @@ __device__ __forceinline__ void expand_flat(const int32_t *lcol, const LV *lval,
                             break;
                         }
                     }
-                    if (!done) full = true;
+                    if (!done) {
+                        full = true;
+                        __hip_atomic_store(fullflag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
                 }
             }
         }

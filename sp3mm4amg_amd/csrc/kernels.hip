@@ -538,7 +538,8 @@ __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand 
     p += (size_t)GPB * W * 4;
     long long *red = reinterpret_cast<long long *>(p) + g * Group<G>::kWaves;
     p += (size_t)GPB * Group<G>::kWaves * 8;
-    int *ired = reinterpret_cast<int *>(p) + g * (Group<G>::kWaves + 1);
+    int *ired = reinterpret_cast<int *>(p) + g * (Group<G>::kWaves + 2);
+    int *fullflag = ired + Group<G>::kWaves + 1;  // a table of this row filled
 
     const unsigned m1 = (unsigned)H1 - 1, m2 = (unsigned)H2 - 1;
     const int sh1 = 32 - ilog2((unsigned)H1), sh2 = 32 - ilog2((unsigned)H2);
@@ -562,16 +563,17 @@ __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand 
             k1[s] = kEmpty;
             v1[s] = 0.0;
         }
+        if (t == 0) *fullflag = 0;
         Group<G>::sync();
         bool full1 = false, full2 = false;
         expand_flat<G, VT, VT, U1>(R.col, rval, rb, re, A.rp32, A.col, aval, slog1, k1, v1, m1, sh1,
-                                   (unsigned)(H1 + H2 + t), full1);
+                                   (unsigned)(H1 + H2 + t), full1, fullflag);
         const int64_t rbn = R.rp[rn], ren = R.rp[rn + 1];
         Group<G>::sync();
         const int nn1 = compact_table<G, double>(k1, v1, H1, ired);
-        if (!full1)
+        if (!full1 && !__hip_atomic_load(fullflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
             expand_flat<G, double, VT, U2>(k1, v1, 0, nn1, P.rp32, P.col, pval, slog2, k2, v2, m2, sh2,
-                                           (unsigned)(H2 + t), full2);
+                                           (unsigned)(H2 + t), full2, fullflag);
         const long long anyfull = group_max<G>((full1 || full2) ? 1 : 0, red);
         if (anyfull) {
             if (t == 0) {
@@ -626,7 +628,8 @@ __global__ void __launch_bounds__(256) k_spgemm_fused(SpOperand A, SpOperand B, 
     p += (size_t)GPB * W * 8;
     int *k = reinterpret_cast<int *>(p) + g * W;
     p += (size_t)GPB * W * 4;
-    int *ired = reinterpret_cast<int *>(p) + g * (Group<G>::kWaves + 1);
+    int *ired = reinterpret_cast<int *>(p) + g * (Group<G>::kWaves + 2);
+    int *fullflag = ired + Group<G>::kWaves + 1;
     const unsigned m = (unsigned)H - 1;
     const int sh = 32 - ilog2((unsigned)H);
     const int n = *count;
@@ -646,10 +649,11 @@ __global__ void __launch_bounds__(256) k_spgemm_fused(SpOperand A, SpOperand B, 
             k[s] = kEmpty;
             v[s] = 0.0;
         }
+        if (t == 0) *fullflag = 0;
         Group<G>::sync();
         bool full = false;
         expand_flat<G, double, double, U>(A.col, aval, rb, re, B.rp32, B.col, bval, slog, k, v, m, sh,
-                                          (unsigned)(H + t), full);
+                                          (unsigned)(H + t), full, fullflag);
         const int64_t rbn = A.rp[rn], ren = A.rp[rn + 1];
         if (full) atomicOr(err, kErrHashFull);  // H >= the row's upper bound: cannot happen
         Group<G>::sync();
@@ -675,7 +679,7 @@ __global__ void __launch_bounds__(256) k_spgemm_fused(SpOperand A, SpOperand B, 
 size_t spgemm_fused_lds(int G, int H)
 {
     const size_t gpb = 256 / G;
-    return gpb * ((size_t)H + G) * 12 + gpb * (group_waves(G) + 1) * 4;
+    return gpb * ((size_t)H + G) * 12 + gpb * (group_waves(G) + 2) * 4;
 }
 
 static unsigned grid_for(int rows, int G, int max_blocks);
@@ -1468,7 +1472,7 @@ void launch_dense_direct(bool num, int blocks, const SpOperand &R, const SpOpera
 size_t fused_lds(int G, int H1, int H2)
 {
     const size_t gpb = 256 / G;
-    return gpb * ((size_t)H1 + H2 + (size_t)G) * 12 + gpb * group_waves(G) * 8 + gpb * (group_waves(G) + 1) * 4;
+    return gpb * ((size_t)H1 + H2 + (size_t)G) * 12 + gpb * group_waves(G) * 8 + gpb * (group_waves(G) + 2) * 4;
 }
 
 using FusedFn = void (*)(SpOperand, SpOperand, SpOperand, const int32_t *, const int *, int, int, int, int,

@@ -131,7 +131,7 @@ __device__ int compact3(int *keys, float *h, float *l, float *a, int H, int *red
 size_t mixed_lds(int G, int H1, int H2)
 {
     const size_t gpb = 256 / G;
-    return gpb * ((size_t)H1 + H2) * 16 + gpb * 8 + gpb * group_waves(G) * 8 + gpb * (group_waves(G) + 1) * 4;
+    return gpb * ((size_t)H1 + H2) * 16 + gpb * 8 + gpb * group_waves(G) * 8 + gpb * (group_waves(G) + 2) * 4;
 }
 
 // Expansion of one left row into an LDS table, as in the FP64 kernel (sub-groups
@@ -143,7 +143,7 @@ template <int G, int U, typename LoadL, typename LoadB, typename Acc>
 __device__ __forceinline__ void expand_mixed(const int32_t *lcol, int64_t e0, int64_t e1,
                                              const int32_t *__restrict__ brp, const int32_t *__restrict__ bcol,
                                              int slog, int *keys, unsigned mask, int shift, bool &full,
-                                             LoadL loadl, LoadB loadb, Acc acc)
+                                             int *fullflag, LoadL loadl, LoadB loadb, Acc acc)
 {
     using LT = decltype(loadl(int64_t{}));
     using BT = decltype(loadb(int{}));
@@ -196,6 +196,10 @@ __device__ __forceinline__ void expand_mixed(const int32_t *lcol, int64_t e0, in
             if (pend) {
                 for (int u = 0; u < U; u++) {
                     if (c[u] == kEmpty || old[u] == kEmpty || old[u] == c[u]) continue;
+                    if (__hip_atomic_load(fullflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                        full = true;  // the row is redone elsewhere
+                        continue;
+                    }
                     unsigned s2 = sl[u];
                     bool done = false;
                     for (unsigned probe = 1; probe <= mask; probe++) {
@@ -207,7 +211,10 @@ __device__ __forceinline__ void expand_mixed(const int32_t *lcol, int64_t e0, in
                             break;
                         }
                     }
-                    if (!done) full = true;
+                    if (!done) {
+                        full = true;
+                        __hip_atomic_store(fullflag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
                 }
             }
         }
@@ -259,7 +266,8 @@ __global__ void __launch_bounds__(256) k_direct_mixed(SpOperand R, SpOperand A, 
     p += (size_t)GPB * H2 * 4;
     int *k2 = reinterpret_cast<int *>(p) + g * H2;
     p += (size_t)GPB * H2 * 4;
-    int *ired = reinterpret_cast<int *>(p) + g * (KW + 1);
+    int *ired = reinterpret_cast<int *>(p) + g * (KW + 2);
+    int *fullflag = ired + KW + 1;  // a table of this row filled
 
     const unsigned m1 = (unsigned)H1 - 1, m2 = (unsigned)H2 - 1;
     const int sh1 = 32 - ilog2((unsigned)H1), sh2 = 32 - ilog2((unsigned)H2);
@@ -290,12 +298,15 @@ __global__ void __launch_bounds__(256) k_direct_mixed(SpOperand R, SpOperand A, 
             l2[s] = 0.f;
             a2[s] = 0.f;
         }
-        if (t == 0) *dg = 0.0;
+        if (t == 0) {
+            *dg = 0.0;
+            *fullflag = 0;
+        }
         Group<G>::sync();
         bool full1 = false, full2 = false;
         // stage 1: row r of R*A; every FP32 product is split exactly by TwoProd
         expand_mixed<G, 4>(
-            R.col, rb, re, A.rp32, A.col, slog1, k1, m1, sh1, full1,
+            R.col, rb, re, A.rp32, A.col, slog1, k1, m1, sh1, full1, fullflag,
             [&](int64_t e) { return rv[e]; }, [&](int q) { return av[q]; },
             [&](unsigned sl, float x, float w, int) {
                 const FF pr = two_prod(x, w);
@@ -308,7 +319,8 @@ __global__ void __launch_bounds__(256) k_direct_mixed(SpOperand R, SpOperand A, 
         // stage 2: forward into P; the diagonal and the row sum also in FP64
         double rs = 0.0;
         expand_mixed<G, 2>(
-            k1, 0, full1 ? 0 : nn1, P.rp32, P.col, slog2, k2, m2, sh2, full2,
+            k1, 0, (full1 || __hip_atomic_load(fullflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) ? 0 : nn1,
+            P.rp32, P.col, slog2, k2, m2, sh2, full2, fullflag,
             [&](int64_t e) {
                 Mid m;
                 m.x = two_sum(h1[e], l1[e]);
