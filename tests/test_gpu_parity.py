@@ -285,6 +285,65 @@ def test_long_rows_window_and_spill():
     check(got, want, O.abs_product(R, A, P))
 
 
+def _sampled_rows(out: L.DeviceCSR, rows: np.ndarray) -> CSR:
+    """Rows `rows` of a device product, copied back one segment at a time."""
+    d = out.d
+    rp = np.empty(d.M + 1, dtype=np.int64)
+    L.lib().sp3mm_copy_to_host(rp.ctypes.data, d.rp, (d.M + 1) * 8)
+    irp, cols, vals = [0], [], []
+    for r in rows:
+        a, b = int(rp[r]), int(rp[r + 1])
+        cj = np.empty(b - a, dtype=np.int32)
+        cv = np.empty(b - a, dtype=np.float64)
+        L.lib().sp3mm_copy_to_host(cj.ctypes.data, d.col + 4 * a, (b - a) * 4)
+        L.lib().sp3mm_copy_to_host(cv.ctypes.data, d.val + 8 * a, (b - a) * 8)
+        cols.append(cj.astype(np.int64))
+        vals.append(cv)
+        irp.append(irp[-1] + b - a)
+    return CSR(len(rows), d.N, np.array(irp, dtype=np.int64), np.concatenate(cols), np.concatenate(vals))
+
+
+def _row_subset(R: CSR, rows: np.ndarray) -> CSR:
+    irp, ja, va = [0], [], []
+    for r in rows:
+        a, b = R.irp[r], R.irp[r + 1]
+        ja.append(R.ja[a:b])
+        va.append(R.vals[a:b])
+        irp.append(irp[-1] + b - a)
+    return CSR(len(rows), R.N, np.array(irp, dtype=np.int64), np.concatenate(ja), np.concatenate(va))
+
+
+def _irregular_sampled(n):
+    """Lognormal irregular operator with random aggregation (the irregular
+    workload): full product on the GPU, 40 rows (first, last, random) against
+    the oracle's R[rows]*A*P, patterns exact, values within the error model."""
+    p = problems.irregular_sa(n, seed=4)
+    R, A, P = p["R"], p["A"], p["P"]
+    print(f"irregular {n}: R {R.shape} nnz {R.nnz}", flush=True)
+    d = [L.DeviceCSR.upload(m) for m in (R, A, P)]
+    out, st = L.rap(*d, path=L.DIRECT)
+    print(f"  nnz {st['nnz_out']} off-table rows {st['guarded_entries']} spilled {st['spilled_rows']}", flush=True)
+    rows = np.unique(np.concatenate([[0, R.M - 1], np.random.default_rng(7).choice(R.M, 38, replace=False)]))
+    got = _sampled_rows(out, rows)
+    out.free()
+    for m in d:
+        m.free()
+    Rs = _row_subset(R, rows)
+    want = O.sp3mm_pair(Rs, A, P)
+    check(got, want, None, bound=O.error_bound(Rs, A, P, want))
+
+
+def test_irregular_100k_sampled():
+    _irregular_sampled(100_000)
+
+
+@pytest.mark.slow
+def test_irregular_1m_sampled():
+    """The full-size irregular workload: 1,000,000 fine rows, ~1.1e10 output
+    nonzeros (about 140 GB of HBM for the output and workspaces)."""
+    _irregular_sampled(1_000_000)
+
+
 def test_autotune_mixed():
     R, A, P = (_rounded32(m) for m in triple("stencil27_15"))
     d = [L.DeviceCSR.upload(m, vtype=L.F32) for m in (R, A, P)]
