@@ -335,6 +335,12 @@ def main():
                     "gflops_device": round(flop / (dev_med * 1e-3) / 1e9, 2)},
         "hbm_copy_probe_GBs": round(copy_gbs, 1) if copy_gbs else None,
     }
+    if path == lib.DIRECT:
+        # the same workload on the pair path (R*A)*P, both SpGEMMs one-pass
+        wp, _, devp, _ = time_form(lib, dR, dA, dP, max(5, args.steps // 2), 1, path=lib.PAIR)
+        rec["pair_path"] = {"ms_per_product": round(wp, 4), "device_ms_median": round(devp, 4),
+                            "gflops": round(flop / wp / 1e6, 2),
+                            "roofline_frac_product": round(bytes_alg / (devp * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     if not args.no_mixed:
         log("mixed-precision leg (27-point stencil) ...")
         rec["mixed_precision"] = mixed_leg(lib, problems, max(3, args.steps // 2), 1, not args.no_tune)

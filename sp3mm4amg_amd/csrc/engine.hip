@@ -101,7 +101,7 @@ int tuning_valid(const Tuning &t)
     const bool pow2h = t.subwave_h == 0 || (t.subwave_h >= 64 && t.subwave_h <= 4096 &&
                                             (t.subwave_h & (t.subwave_h - 1)) == 0);
     return (t.u1 == 2 || t.u1 == 4) && (t.u2 >= 2 && t.u2 <= 4) && pow2h && t.slog1_cap >= 0 &&
-           t.slog1_cap <= 8 && t.slog2_cap >= 0 && t.slog2_cap <= 8 && t.h1_shift >= 0 && t.h1_shift <= 2 &&
+           t.slog1_cap <= 8 && t.slog2_cap >= 0 && t.slog2_cap <= 8 && t.h1_shift >= -1 && t.h1_shift <= 2 &&
            (t.h2_div == 1 || t.h2_div == 2 || t.h2_div == 4 || t.h2_div == 8);
 }
 
@@ -964,7 +964,7 @@ static void fused_bins(size_t max_lds, bool mixed, const Tuning &tu, BinThreshol
     thr.nb = 0;
     cfg.clear();
     for (long long v = 64; thr.nb < kMaxBins - 1; v *= 2) {
-        const int H1 = (int)std::max<long long>(64, v >> tu.h1_shift);
+        const int H1 = (int)std::max<long long>(64, tu.h1_shift >= 0 ? v >> tu.h1_shift : v << -tu.h1_shift);
         const int G = H1 <= tu.subwave_h ? 32 : H1 <= 512 ? 64 : 256;
         // both tables hold a whole number of group-widths (the compaction and
         // clearing loops step through them G slots at a time)
@@ -1380,7 +1380,7 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
     const Axis axes[] = {
         {&Tuning::subwave_h, {0, 128, 256, 512}}, {&Tuning::u1, {2, 4}},       {&Tuning::u2, {2, 3, 4}},
         {&Tuning::slog1_cap, {1, 2, 3}},            {&Tuning::slog2_cap, {1, 2}},
-        {&Tuning::h1_shift, {0, 1}},                {&Tuning::h2_div, {2, 4, 8}},
+        {&Tuning::h1_shift, {-1, 0, 1}},            {&Tuning::h2_div, {2, 4, 8}},
     };
     for (const Axis &ax : axes) {
         for (int v : ax.values) {

@@ -87,7 +87,8 @@ struct Tuning {
     int subwave_h = 256;    // largest stage-1 table run by 32-lane groups (0: none)
     int slog1_cap = 3;      // largest lane sub-group (log2) per left entry, stage 1
     int slog2_cap = 1;      // the same, stage 2
-    int h1_shift = 0;       // stage-1 table = bin upper bound >> h1_shift (a full table -> retry)
+    int h1_shift = 0;       // stage-1 table = bin upper bound >> h1_shift, -1: twice the bound
+                            // (a full table -> retry)
     int h2_div = 4;         // stage-2 table = max(64, H1 / h2_div)
 };
 
