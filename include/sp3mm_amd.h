@@ -268,6 +268,17 @@ int sp3mm_batch_result(sp3mm_batch *b, int32_t i, sp3mm_dmat *out);
 int sp3mm_batch_info(sp3mm_batch *b, double *ms_pack, double *ms_build, int64_t *nnz, int32_t *has_graph);
 void sp3mm_batch_free(sp3mm_batch *b);
 
+/* Setup operations of an aggregation AMG level (no counterpart in the
+ * reference, whose caller supplies P and R), so that a hierarchy can be built
+ * on the GPU level after level:
+ *   sp3mm_transpose: AT = A^T (sorted CSR, values of A's type);
+ *   sp3mm_sa_prolongator: P = (I - omega D^-1 A) P0 for square FP64 A, where
+ *     P0 is the 0/1 aggregation matrix of agg (host array of A->M aggregate
+ *     ids in [0, nagg)); D = diag(A) (diag_mode 0) or |diag(A)| + 1 (1). */
+int sp3mm_transpose(const sp3mm_dmat *A, sp3mm_dmat *AT);
+int sp3mm_sa_prolongator(const sp3mm_dmat *A, const int64_t *agg, int64_t nagg, double omega,
+                         int32_t diag_mode, sp3mm_dmat *P);
+
 /* Scheduling of the one-pass direct kernels.  These parameters pick table sizes,
  * group shapes and the spread of work over lanes; results are the same up to
  * FP64 summation order. */

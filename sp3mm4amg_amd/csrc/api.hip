@@ -303,6 +303,29 @@ int sp3mm_batch_info(sp3mm_batch *b, double *ms_pack, double *ms_build, int64_t 
 
 void sp3mm_batch_free(sp3mm_batch *b) { batch_free(b); }
 
+int sp3mm_transpose(const sp3mm_dmat *A, sp3mm_dmat *AT)
+{
+    clear_error();
+    if (!A || !AT) {
+        set_error("transpose: null argument");
+        return -1;
+    }
+    memset(AT, 0, sizeof *AT);
+    return transpose(*A, *AT);
+}
+
+int sp3mm_sa_prolongator(const sp3mm_dmat *A, const int64_t *agg, int64_t nagg, double omega, int32_t diag_mode,
+                         sp3mm_dmat *P)
+{
+    clear_error();
+    if (!A || !P || (!agg && A->M > 0)) {
+        set_error("prolongator: null argument");
+        return -1;
+    }
+    memset(P, 0, sizeof *P);
+    return sa_prolongator(*A, agg, nagg, omega, diag_mode, *P);
+}
+
 static void to_c(const Tuning &t, sp3mm_tuning &o)
 {
     o.u1 = t.u1;

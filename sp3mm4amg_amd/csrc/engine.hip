@@ -1337,6 +1337,104 @@ int rap_direct_mixed(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat 
 }
 
 // --------------------------------------------------------------------------
+// setup operations: transpose, smoothed-aggregation prolongator
+// --------------------------------------------------------------------------
+int transpose(const sp3mm_dmat &A, sp3mm_dmat &T)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    hipStream_t s = c->stream;
+    const bool f32 = A.vtype == SP3MM_F32;
+    if (dmat_alloc(T, A.N, A.M, A.nnz, A.vtype)) return -1;
+    Buf cnt((size_t)(A.N + 1) * 4), cursor((size_t)(A.N + 1) * 4), tiles((size_t)scan_tiles(A.N) * 8),
+        total(8), tcol((size_t)(A.nnz ? A.nnz : 1) * 4), tval((size_t)(A.nnz ? A.nnz : 1) * (f32 ? 4 : 8));
+    if (!cnt.p || !cursor.p || !tiles.p || !total.p || !tcol.p || !tval.p) {
+        set_error("transpose: scratch allocation failed");
+        dmat_release(T);
+        return -1;
+    }
+    launch_fill_i32(cnt.as<int32_t>(), A.N + 1, 0, s);
+    launch_fill_i32(cursor.as<int32_t>(), A.N + 1, 0, s);
+    launch_col_count(A.col, A.nnz, cnt.as<int32_t>(), s);
+    launch_scan(cnt.as<int32_t>(), A.N, T.rp, T.rp32, tiles.as<long long>(), total.as<int64_t>(), s);
+    launch_transpose_scatter(A.rp, A.col, A.val, f32, A.M, T.rp, cursor.as<int32_t>(), tcol.as<int32_t>(), tval.p,
+                             s);
+    launch_sort_rows(T.rp, T.M, tcol.as<int32_t>(), tval.p, f32, T.col, T.val, s);
+    SP3MM_HIP(hipGetLastError());
+    SP3MM_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int sa_prolongator(const sp3mm_dmat &A, const int64_t *agg, int64_t nagg, double omega, int mode, sp3mm_dmat &P)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (A.M != A.N) {
+        set_error("prolongator: A must be square (%lld x %lld)", (long long)A.M, (long long)A.N);
+        return -1;
+    }
+    if (A.vtype != SP3MM_F64) {
+        set_error("prolongator: A must hold FP64 values");
+        return -1;
+    }
+    if (nagg <= 0 || nagg >= INT32_MAX || (mode != 0 && mode != 1)) {
+        set_error("prolongator: bad aggregate count or diagonal mode");
+        return -1;
+    }
+    hipStream_t s = c->stream;
+    const int64_t M = A.M;
+    Buf ctl(sizeof(Ctl)), len((size_t)(M + 1) * 4), tiles((size_t)scan_tiles(M) * 8), dagg((size_t)(M ? M : 1) * 8);
+    if (!ctl.p || !len.p || !tiles.p || !dagg.p) {
+        set_error("prolongator: scratch allocation failed");
+        return -1;
+    }
+    Ctl *d = ctl.as<Ctl>();
+    Ctl h;
+    SP3MM_HIP(hipMemsetAsync(d, 0, sizeof(Ctl), s));
+    if (M) SP3MM_HIP(hipMemcpyAsync(dagg.p, agg, (size_t)M * 8, hipMemcpyHostToDevice, s));
+    // S = I - omega D^-1 A (pattern of A plus any missing diagonal)
+    sp3mm_dmat S{}, P0{};
+    launch_sa_count(A.rp, A.col, M, len.as<int32_t>(), s);
+    if (dmat_alloc(S, M, M, 0, SP3MM_F64)) return -1;
+    launch_scan(len.as<int32_t>(), M, S.rp, S.rp32, tiles.as<long long>(), &d->nnz, s);
+    if (read_ctl(c, d, h)) {
+        dmat_release(S);
+        return -1;
+    }
+    S.nnz = h.nnz;
+    c->pool.release(S.col);
+    c->pool.release(S.val);
+    S.col = static_cast<int32_t *>(c->pool.alloc((size_t)(S.nnz ? S.nnz : 1) * 4));
+    S.val = c->pool.alloc((size_t)(S.nnz ? S.nnz : 1) * 8);
+    if (!S.col || !S.val || S.nnz >= INT32_MAX) {
+        set_error("prolongator: allocation of S (%lld nnz) failed", (long long)S.nnz);
+        dmat_release(S);
+        return -1;
+    }
+    launch_sa_fill(A.rp, A.col, static_cast<const double *>(A.val), M, omega, mode, S.rp, S.col,
+                   static_cast<double *>(S.val), &d->err, s);
+    // P0: one 1.0 per row at its aggregate
+    if (dmat_alloc(P0, M, nagg, M, SP3MM_F64)) {
+        dmat_release(S);
+        return -1;
+    }
+    launch_p0_fill(dagg.as<int64_t>(), M, nagg, P0.rp, P0.rp32, P0.col, static_cast<double *>(P0.val), &d->err,
+                   s);
+    if (read_ctl(c, d, h) || h.err) {
+        if (h.err) set_error("prolongator: %s (zero diagonal, or an aggregate out of range)", err_text(h.err));
+        dmat_release(S);
+        dmat_release(P0);
+        return -1;
+    }
+    const int rc = spgemm(S, P0, P, nullptr, false);
+    dmat_release(S);
+    dmat_release(P0);
+    return rc;
+}
+
+// --------------------------------------------------------------------------
 // autotuner: times the direct product of the caller's operands under candidate
 // schedules and installs the fastest in the context.  Coordinate descent from
 // the current setting, one parameter at a time; `reps` timed products per

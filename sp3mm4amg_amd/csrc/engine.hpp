@@ -87,6 +87,8 @@ void *batch_create(int count, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp
                    double guard, bool graph);
 int batch_run(void *b, bool use_graph, double *ms);
 int tuning_valid(const Tuning &t);
+int transpose(const sp3mm_dmat &A, sp3mm_dmat &T);
+int sa_prolongator(const sp3mm_dmat &A, const int64_t *agg, int64_t nagg, double omega, int mode, sp3mm_dmat &P);
 int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int prec, double guard, int reps,
              Tuning &best, double &ms_best, double &ms_default);
 int batch_result(void *b, int i, sp3mm_dmat &out);

@@ -140,6 +140,17 @@ void launch_rp_shift(const int64_t *rp, int64_t n, int base, int64_t *out, hipSt
 void launch_f64_to_f32(const double *in, int64_t n, float *out, hipStream_t st);
 void launch_f32_to_f64(const float *in, int64_t n, double *out, hipStream_t st);
 void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
+// setup.hip: transpose and smoothed-aggregation prolongator
+void launch_col_count(const int32_t *col, int64_t nnz, int32_t *cnt, hipStream_t st);
+void launch_transpose_scatter(const int64_t *rp, const int32_t *col, const void *val, bool f32, int64_t M,
+                              const int64_t *trp, int32_t *cursor, int32_t *tcol, void *tval, hipStream_t st);
+void launch_sort_rows(const int64_t *rp, int64_t M, const int32_t *icol, const void *ival, bool f32, int32_t *ocol,
+                      void *oval, hipStream_t st);
+void launch_sa_count(const int64_t *rp, const int32_t *col, int64_t M, int32_t *len, hipStream_t st);
+void launch_sa_fill(const int64_t *rp, const int32_t *col, const double *val, int64_t M, double omega, int mode,
+                    const int64_t *srp, int32_t *scol, double *sval, int *err, hipStream_t st);
+void launch_p0_fill(const int64_t *agg, int64_t M, int64_t nagg, int64_t *rp, int32_t *rp32, int32_t *col,
+                    double *val, int *err, hipStream_t st);
 void launch_export_words(const void *src, void *host_dst, int words, hipStream_t st);
 void launch_copy_probe(const void *in, void *out, int64_t bytes, hipStream_t st);
 

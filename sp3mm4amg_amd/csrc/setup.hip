@@ -1,0 +1,200 @@
+// setup.hip -- the step before the Galerkin product in an aggregation AMG
+// setup, on the GPU: the transpose R = P^T and the smoothed-aggregation
+// prolongator P = (I - omega D^-1 A) P0 (P0 = the 0/1 aggregation matrix).
+// The reference receives P and R from its caller; these kernels let a whole
+// hierarchy be built level after level without leaving HBM.
+#include "device_common.hpp"
+#include "kernels.hpp"
+
+namespace sp3mm {
+
+// ---------------------------------------------------------------------------
+// transpose: per-column counts, scan (the caller), scatter, per-row sort
+// ---------------------------------------------------------------------------
+__global__ void k_col_count(const int32_t *__restrict__ col, int64_t nnz, int32_t *__restrict__ cnt)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += stride) atomicAdd(cnt + col[e], 1);
+}
+
+// one thread per row of A: its entries go to the rows of A^T named by their
+// columns, at the next free position of that row (order fixed afterwards)
+template <typename VT>
+__global__ void k_transpose_scatter(const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                    const VT *__restrict__ val, int64_t M, const int64_t *__restrict__ trp,
+                                    int32_t *__restrict__ cursor, int32_t *__restrict__ tcol, VT *__restrict__ tval)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < M; r += stride) {
+        for (int64_t e = rp[r]; e < rp[r + 1]; e++) {
+            const int c = col[e];
+            const int64_t pos = trp[c] + atomicAdd(cursor + c, 1);
+            tcol[pos] = (int32_t)r;
+            tval[pos] = val[e];
+        }
+    }
+}
+
+// one wavefront per row of A^T: rank sort of its (distinct) column indices;
+// keys of rows up to kSortLds entries are staged in LDS, longer rows are
+// ranked from global memory
+constexpr int kSortLds = 1024;
+
+template <typename VT>
+__global__ void __launch_bounds__(256) k_sort_rows(const int64_t *__restrict__ rp, int64_t M,
+                                                    const int32_t *__restrict__ icol, const VT *__restrict__ ival,
+                                                    int32_t *__restrict__ ocol, VT *__restrict__ oval)
+{
+    __shared__ int keys[4][kSortLds];
+    const int w = threadIdx.x / kWave;
+    const int l = lane();
+    int *kw = keys[w];
+    for (int64_t r = (int64_t)blockIdx.x * 4 + w; r < M; r += (int64_t)gridDim.x * 4) {
+        const int64_t b = rp[r];
+        const int n = (int)(rp[r + 1] - b);
+        const bool staged = n <= kSortLds;
+        if (staged)
+            for (int j = l; j < n; j += kWave) kw[j] = icol[b + j];
+        wave_sync();
+        const int *src = staged ? kw : icol + b;
+        for (int j = l; j < n; j += kWave) {
+            const int k = src[j];
+            int rank = 0;
+            for (int i = 0; i < n; i++) rank += src[i] < k;
+            ocol[b + rank] = k;
+            oval[b + rank] = ival[b + j];
+        }
+        wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// S = I - omega D^-1 A on the pattern of A (a missing diagonal is inserted)
+// ---------------------------------------------------------------------------
+// d_i = a_ii (mode 0) or |a_ii| + 1 (mode 1).  Count pass: row lengths of S.
+__global__ void k_sa_count(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int64_t M,
+                           int32_t *__restrict__ len)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < M; r += stride) {
+        bool diag = false;
+        for (int64_t e = rp[r]; e < rp[r + 1]; e++) diag |= col[e] == r;
+        len[r] = (int32_t)(rp[r + 1] - rp[r]) + (diag ? 0 : 1);
+    }
+}
+
+__global__ void k_sa_fill(const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
+                          const double *__restrict__ val, int64_t M, double omega, int mode,
+                          const int64_t *__restrict__ srp, int32_t *__restrict__ scol, double *__restrict__ sval,
+                          int *__restrict__ err)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < M; r += stride) {
+        double d = 0.0;
+        for (int64_t e = rp[r]; e < rp[r + 1]; e++)
+            if (col[e] == r) d = val[e];
+        if (mode == 1) d = fabs(d) + 1.0;
+        if (d == 0.0) {
+            atomicOr(err, kErrBadInput);
+            continue;
+        }
+        const double w = omega / d;
+        int64_t o = srp[r];
+        bool placed = false;
+        for (int64_t e = rp[r]; e < rp[r + 1]; e++) {
+            const int c = col[e];
+            if (!placed && c > r) {  // the missing diagonal goes before the first larger column
+                scol[o] = (int32_t)r;
+                sval[o++] = 1.0;
+                placed = true;
+            }
+            double s = -w * val[e];
+            if (c == r) {
+                s += 1.0;
+                placed = true;
+            }
+            scol[o] = c;
+            sval[o++] = s;
+        }
+        if (!placed) {
+            scol[o] = (int32_t)r;
+            sval[o++] = 1.0;
+        }
+    }
+}
+
+// P0: one entry 1.0 per row at column agg[r]; rejects aggregates out of range
+__global__ void k_p0_fill(const int64_t *__restrict__ agg, int64_t M, int64_t nagg, int64_t *__restrict__ rp,
+                          int32_t *__restrict__ rp32, int32_t *__restrict__ col, double *__restrict__ val,
+                          int *__restrict__ err)
+{
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r <= M; r += stride) {
+        rp[r] = r;
+        if (rp32) rp32[r] = (int32_t)r;
+        if (r < M) {
+            const int64_t a = agg[r];
+            if (a < 0 || a >= nagg) atomicOr(err, kErrBadInput);
+            col[r] = (int32_t)(a < 0 || a >= nagg ? 0 : a);
+            val[r] = 1.0;
+        }
+    }
+}
+
+static unsigned grid1(int64_t n, int threads = 256)
+{
+    int64_t b = (n + threads - 1) / threads;
+    if (b > 65536) b = 65536;
+    return (unsigned)(b < 1 ? 1 : b);
+}
+
+void launch_col_count(const int32_t *col, int64_t nnz, int32_t *cnt, hipStream_t st)
+{
+    if (nnz > 0) hipLaunchKernelGGL(k_col_count, dim3(grid1(nnz)), dim3(256), 0, st, col, nnz, cnt);
+}
+
+void launch_transpose_scatter(const int64_t *rp, const int32_t *col, const void *val, bool f32, int64_t M,
+                              const int64_t *trp, int32_t *cursor, int32_t *tcol, void *tval, hipStream_t st)
+{
+    if (M == 0) return;
+    if (f32)
+        hipLaunchKernelGGL(k_transpose_scatter<float>, dim3(grid1(M)), dim3(256), 0, st, rp, col,
+                           static_cast<const float *>(val), M, trp, cursor, tcol, static_cast<float *>(tval));
+    else
+        hipLaunchKernelGGL(k_transpose_scatter<double>, dim3(grid1(M)), dim3(256), 0, st, rp, col,
+                           static_cast<const double *>(val), M, trp, cursor, tcol, static_cast<double *>(tval));
+}
+
+void launch_sort_rows(const int64_t *rp, int64_t M, const int32_t *icol, const void *ival, bool f32, int32_t *ocol,
+                      void *oval, hipStream_t st)
+{
+    if (M == 0) return;
+    const unsigned g = grid1(M, 4);
+    if (f32)
+        hipLaunchKernelGGL(k_sort_rows<float>, dim3(g), dim3(256), 0, st, rp, M, icol,
+                           static_cast<const float *>(ival), ocol, static_cast<float *>(oval));
+    else
+        hipLaunchKernelGGL(k_sort_rows<double>, dim3(g), dim3(256), 0, st, rp, M, icol,
+                           static_cast<const double *>(ival), ocol, static_cast<double *>(oval));
+}
+
+void launch_sa_count(const int64_t *rp, const int32_t *col, int64_t M, int32_t *len, hipStream_t st)
+{
+    if (M > 0) hipLaunchKernelGGL(k_sa_count, dim3(grid1(M)), dim3(256), 0, st, rp, col, M, len);
+}
+
+void launch_sa_fill(const int64_t *rp, const int32_t *col, const double *val, int64_t M, double omega, int mode,
+                    const int64_t *srp, int32_t *scol, double *sval, int *err, hipStream_t st)
+{
+    if (M > 0)
+        hipLaunchKernelGGL(k_sa_fill, dim3(grid1(M)), dim3(256), 0, st, rp, col, val, M, omega, mode, srp, scol,
+                           sval, err);
+}
+
+void launch_p0_fill(const int64_t *agg, int64_t M, int64_t nagg, int64_t *rp, int32_t *rp32, int32_t *col,
+                    double *val, int *err, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_p0_fill, dim3(grid1(M + 1)), dim3(256), 0, st, agg, M, nagg, rp, rp32, col, val, err);
+}
+
+}  // namespace sp3mm

@@ -137,6 +137,8 @@ def lib():
         L.sp3mm_batch_info.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_int64), P(C.c_int32)]
         L.sp3mm_batch_free.argtypes = [C.c_void_p]
         L.sp3mm_batch_free.restype = None
+        L.sp3mm_transpose.argtypes = [P(DMat), P(DMat)]
+        L.sp3mm_sa_prolongator.argtypes = [P(DMat), C.c_void_p, C.c_int64, C.c_double, C.c_int32, P(DMat)]
         L.sp3mm_get_tuning.argtypes = [P(Tuning)]
         L.sp3mm_set_tuning.argtypes = [P(Tuning)]
         L.sp3mm_autotune.argtypes = [P(DMat), P(DMat), P(DMat), P(Opts), C.c_int32, P(Tuning),
@@ -335,6 +337,42 @@ def rap_mixed_host(R: CSR, A: CSR, P: CSR, guard: float = 0.0, rowsum: bool = Fa
     if rs:
         rs.free()
     return res, st, sums
+
+
+def transpose(A: DeviceCSR) -> DeviceCSR:
+    """A^T on the device (sorted CSR, same value type)."""
+    d = DMat()
+    _check(lib().sp3mm_transpose(C.byref(A.d), C.byref(d)), "sp3mm_transpose")
+    return DeviceCSR(d)
+
+
+def sa_prolongator(A: DeviceCSR, agg: np.ndarray, nagg: int, omega: float = 2.0 / 3.0,
+                   diag_mode: int = 0) -> DeviceCSR:
+    """P = (I - omega D^-1 A) P0 on the device; agg[i] is the aggregate of row i,
+    D = diag(A) (diag_mode 0) or |diag(A)| + 1 (diag_mode 1)."""
+    a = np.ascontiguousarray(agg, dtype=np.int64)
+    if a.shape != (A.d.M,):
+        raise ValueError("one aggregate per row of A")
+    d = DMat()
+    _check(lib().sp3mm_sa_prolongator(C.byref(A.d), a.ctypes.data, int(nagg), float(omega), int(diag_mode),
+                                      C.byref(d)), "sp3mm_sa_prolongator")
+    return DeviceCSR(d)
+
+
+def hierarchy_device(A: DeviceCSR, aggs, omega: float = 2.0 / 3.0, diag_mode: int = 0, path: int = DIRECT):
+    """Builds a smoothed-aggregation hierarchy on the device: for every
+    (agg, nagg) in `aggs`, P = sa_prolongator(A), R = P^T, A <- R*A*P.
+    Returns the list of (R, A, P) levels and the coarsest operator; the caller
+    owns (and frees) every returned matrix except the input A."""
+    levels = []
+    cur = A
+    for agg, nagg in aggs:
+        P = sa_prolongator(cur, agg, nagg, omega, diag_mode)
+        R = transpose(P)
+        nxt, _ = rap(R, cur, P, path=path)
+        levels.append((R, cur, P))
+        cur = nxt
+    return levels, cur
 
 
 def get_tuning() -> dict:

@@ -163,11 +163,12 @@ def random_aggregation(n: int, seed: int, lo: int = 4, hi: int = 12):
 
 
 def smoothed_prolongator(A: CSR, agg: np.ndarray, nagg: int, omega: float = OMEGA,
-                         diag_mode: str = "diag") -> CSR:
+                         diag_mode: str = "diag", absolute: bool = False) -> CSR:
     """P = (I - omega D^-1 A) P0 with P0 the 0/1 aggregation matrix.
 
     diag_mode "diag": d_i = a_ii;  "absplus1": d_i = |a_ii| + 1 (a missing
-    diagonal counts as 0)."""
+    diagonal counts as 0).  absolute=True sums |terms| instead (the magnitude
+    scale of every entry, for rounding-error bounds)."""
     rows = A.row_ids()
     isdiag = A.ja == rows
     d = np.zeros(A.M)
@@ -181,6 +182,8 @@ def smoothed_prolongator(A: CSR, agg: np.ndarray, nagg: int, omega: float = OMEG
     w = omega / d
     s = -(w[rows] * A.vals)
     s[isdiag] += 1.0
+    if absolute:
+        s = np.abs(s)
     r = rows
     c = agg[A.ja]
     v = s

@@ -223,6 +223,112 @@ def test_mixed_precision_requires_fp32_operands():
 
 
 # ---------------------------------------------------------------------------
+# setup operations on the device: transpose, smoothed-aggregation prolongator
+def _rect_with_empties():
+    rng = np.random.default_rng(3)
+    M, N = 300, 170
+    rows = rng.integers(0, M, 2000)
+    rows = rows[rows % 7 != 0]          # empty rows
+    cols = rng.integers(0, N, rows.size)
+    cols = cols[: rows.size]
+    keep = cols % 11 != 0               # empty columns
+    return from_coo(M, N, rows[keep], cols[keep], rng.standard_normal(int(keep.sum())), sum_duplicates=True)
+
+
+@pytest.mark.parametrize("which", ["P_poisson2d64", "P_irregular3000", "A_irregular3000", "rect"])
+def test_transpose_exact(which):
+    if which == "rect":
+        m = _rect_with_empties()
+    else:
+        kind, name = which.split("_", 1)
+        R, A, P = triple(name)
+        m = {"P": P, "A": A}[kind]
+    print(f"transpose {which}: {m.shape} nnz {m.nnz}", flush=True)
+    d = L.DeviceCSR.upload(m)
+    t = L.transpose(d)
+    got = t.download()
+    t.free()
+    d.free()
+    want = m.transpose()
+    got.check()
+    assert got.shape == want.shape
+    assert np.array_equal(got.irp, want.irp) and np.array_equal(got.ja, want.ja)
+    assert np.array_equal(got.vals, want.vals)
+
+
+def _check_prolongator(A, agg, nagg, mode):
+    dA = L.DeviceCSR.upload(A)
+    dP = L.sa_prolongator(dA, agg, nagg, diag_mode=mode)
+    got = dP.download()
+    dP.free()
+    dA.free()
+    name = "diag" if mode == 0 else "absplus1"
+    want = problems.smoothed_prolongator(A, agg, nagg, diag_mode=name)
+    scale = problems.smoothed_prolongator(A, agg, nagg, diag_mode=name, absolute=True)
+    got.check()
+    assert np.array_equal(got.irp, want.irp) and np.array_equal(got.ja, want.ja)
+    n = np.diff(A.irp)[want.row_ids()] + 1.0
+    bound = 1e-14 * np.abs(want.vals) + 4 * n * 2.0 ** -53 * scale.vals
+    assert np.all(np.abs(got.vals - want.vals) <= bound)
+
+
+def test_sa_prolongator_structured():
+    for dims, block in (((64, 64), (2, 2)), ((15, 15, 15), (3, 3, 3))):
+        A = problems.poisson_operator(dims) if len(dims) == 2 else problems.stencil27_random(dims[0], seed=3)
+        agg, cdims = problems.structured_aggregation(dims, block)
+        print(f"prolongator {dims}", flush=True)
+        _check_prolongator(A, agg, int(np.prod(cdims)), 0)
+
+
+def test_sa_prolongator_irregular_and_missing_diagonal():
+    A = problems.lognormal_matrix(3000, 4)
+    agg, nagg = problems.random_aggregation(3000, 5)
+    _check_prolongator(A, agg, nagg, 1)
+    # rows without a stored diagonal get the identity term inserted
+    rows = A.row_ids()
+    keep = ~((A.ja == rows) & (rows % 5 == 0))
+    B = from_coo(A.M, A.N, rows[keep], A.ja[keep], A.vals[keep])
+    _check_prolongator(B, agg, nagg, 1)
+    with pytest.raises(RuntimeError):  # zero diagonal with diag_mode 0
+        _check_prolongator(B, agg, nagg, 0)
+
+
+def test_hierarchy_on_device():
+    """Three smoothed-aggregation levels from the 16^3 7-point operator built on
+    the GPU (prolongator, transpose, R*A*P) against the host construction."""
+    dims = (16, 16, 16)
+    A = problems.poisson_operator(dims)
+    aggs, host = [], []
+    cur = A
+    for lvl in range(3):
+        agg, cdims = problems.structured_aggregation(dims, (2, 2, 2))
+        nagg = int(np.prod(cdims))
+        aggs.append((agg, nagg))
+        R, _, P = problems.galerkin_inputs(cur, agg, nagg)
+        host.append((R, cur, P))
+        cur = O.sp3mm_pair(R, cur, P)
+        dims = cdims
+    dA = L.DeviceCSR.upload(A)
+    levels, coarse = L.hierarchy_device(dA, aggs)
+    for i, (dR, dAl, dP) in enumerate(levels):
+        R, Al, P = host[i]
+        got = dP.download()
+        assert np.array_equal(got.irp, P.irp) and np.array_equal(got.ja, P.ja)
+        assert np.max(np.abs(got.vals - P.vals)) <= 1e-13
+        got = dR.download()
+        assert np.array_equal(got.irp, R.irp) and np.array_equal(got.ja, R.ja)
+        dR.free()
+        dP.free()
+        if i > 0:
+            dAl.free()
+    got = coarse.download()
+    coarse.free()
+    dA.free()
+    print(f"coarsest {got.shape} nnz {got.nnz}")
+    check(got, cur, O.abs_product(*host[-1]), tol=1e-11)
+
+
+# ---------------------------------------------------------------------------
 # schedules: every tuning gives the oracle's pattern and values
 SCHEDULES = [
     {"subwave_h": 0},                                   # wavefront groups only
