@@ -279,6 +279,48 @@ int sp3mm_transpose(const sp3mm_dmat *A, sp3mm_dmat *AT);
 int sp3mm_sa_prolongator(const sp3mm_dmat *A, const int64_t *agg, int64_t nagg, double omega,
                          int32_t diag_mode, sp3mm_dmat *P);
 
+/* Consumers of a level (no counterpart in the reference, whose products feed
+ * the caller's own solver), so that a V-cycle over a device hierarchy runs on
+ * the GPU; FP64 operators, device vectors (x and y distinct):
+ *   sp3mm_spmv: y = alpha A x + beta y (beta == 0: y is not read);
+ *   sp3mm_residual: r = b - A x for square A (r may be b);
+ *   sp3mm_jacobi: `sweeps` damped Jacobi sweeps on A x = b,
+ *     x <- x + omega D^-1 (b - A x), in place; work holds A->M doubles;
+ *   sp3mm_axpby: y = alpha x + beta y on n entries (a zero coefficient skips
+ *     the read of its vector: alpha = beta = 0 clears y);
+ *   sp3mm_dot: *out = x . y, summed in an order that does not depend on the
+ *     launch (the same inputs give the same bits).
+ * spmv, residual, jacobi and axpby are queued on the library's stream in
+ * call order and return without waiting; sp3mm_dot, sp3mm_copy_to_host and
+ * sp3mm_sync wait.  A Jacobi sweep that meets a zero diagonal entry is
+ * reported (once) by the next call that waits. */
+int sp3mm_spmv(const sp3mm_dmat *A, const double *x, double *y, double alpha, double beta);
+int sp3mm_residual(const sp3mm_dmat *A, const double *b, const double *x, double *r);
+int sp3mm_jacobi(const sp3mm_dmat *A, const double *b, double *x, double *work, double omega,
+                 int32_t sweeps);
+int sp3mm_axpby(int64_t n, double alpha, const double *x, double beta, double *y);
+int sp3mm_dot(int64_t n, const double *x, const double *y, double *out);
+
+/* V(nu1, nu2) cycle over a device hierarchy (no counterpart in the
+ * reference): level l (0 = finest, nlevels >= 1 levels above the coarsest)
+ * smooths with nu1 damped Jacobi sweeps, restricts the residual with R[l],
+ * recurses, prolongates with P[l], adds the correction and smooths with nu2
+ * sweeps; the coarsest operator gets coarse_sweeps sweeps from zero.  The
+ * matrices are referenced, not copied: they must outlive the handle.  With
+ * use_graph one cycle is captured as a HIP graph and replayed.
+ * solve: b and x are device vectors of A[0]->M entries; x holds the initial
+ * guess and receives the solution.  Cycles until ||b - A x|| <= rtol ||b||
+ * or maxit cycles; *cycles receives the count, hist (maxit + 1 entries, or
+ * NULL) the relative residual before every cycle and after the last, *ms the
+ * device time of the cycles.  Calls on one handle must not overlap. */
+typedef struct sp3mm_vcycle sp3mm_vcycle;
+int sp3mm_vcycle_create(int32_t nlevels, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P,
+                        const sp3mm_dmat *coarse, int32_t nu1, int32_t nu2, double omega,
+                        int32_t coarse_sweeps, int32_t use_graph, sp3mm_vcycle **out);
+int sp3mm_vcycle_solve(sp3mm_vcycle *v, const double *b, double *x, double rtol, int32_t maxit,
+                       int32_t *cycles, double *hist, double *ms);
+void sp3mm_vcycle_free(sp3mm_vcycle *v);
+
 /* Scheduling of the one-pass direct kernels.  These parameters pick table sizes,
  * group shapes and the spread of work over lanes; results are the same up to
  * FP64 summation order. */
@@ -307,10 +349,11 @@ int sp3mm_autotune(const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P
                    double *ms_best, double *ms_default);
 
 /* Device memory from the library's pool (e.g. for sp3mm_opts.rowsum), its
- * release, and a blocking device-to-host copy. */
+ * release, and blocking copies between host and device. */
 void *sp3mm_device_alloc(int64_t bytes);
 void sp3mm_device_free(void *p);
 int sp3mm_copy_to_host(void *dst, const void *src, int64_t bytes);
+int sp3mm_copy_to_device(void *dst, const void *src, int64_t bytes);
 
 /* Compute units, LDS bytes per workgroup on the device, LDS bytes the bin
  * tables are allowed to use. */

@@ -10,7 +10,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsp3mm_amd.so")
-SOURCES = ["kernels.hip", "mixed.hip", "setup.hip", "engine.hip", "api.hip", "dropin.hip"]
+SOURCES = ["kernels.hip", "mixed.hip", "setup.hip", "engine.hip", "vcycle.hip", "api.hip", "dropin.hip"]
 ARCH = os.environ.get("SP3MM_ARCH", "gfx950")
 
 

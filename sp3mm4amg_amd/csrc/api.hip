@@ -35,8 +35,8 @@ int sp3mm_sync(void)
 {
     Context *c = ctx();
     if (!c) return -1;
-    SP3MM_HIP(hipStreamSynchronize(c->stream));
-    return 0;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    return consumer_wait(c);
 }
 
 int sp3mm_dmat_upload(sp3mm_dmat *out, int64_t M, int64_t N, int64_t nnz, const int64_t *irp,
@@ -326,6 +326,105 @@ int sp3mm_sa_prolongator(const sp3mm_dmat *A, const int64_t *agg, int64_t nagg, 
     return sa_prolongator(*A, agg, nagg, omega, diag_mode, *P);
 }
 
+int sp3mm_spmv(const sp3mm_dmat *A, const double *x, double *y, double alpha, double beta)
+{
+    clear_error();
+    if (!A) {
+        set_error("spmv: null argument");
+        return -1;
+    }
+    return spmv(*A, x, y, y, alpha, beta);
+}
+
+int sp3mm_residual(const sp3mm_dmat *A, const double *b, const double *x, double *r)
+{
+    clear_error();
+    if (!A) {
+        set_error("residual: null argument");
+        return -1;
+    }
+    if (A->M != A->N) {
+        set_error("residual: A must be square");
+        return -1;
+    }
+    return spmv(*A, x, b, r, -1.0, 1.0);
+}
+
+int sp3mm_axpby(int64_t n, double alpha, const double *x, double beta, double *y)
+{
+    clear_error();
+    return axpby(n, alpha, x, beta, y);
+}
+
+int sp3mm_dot(int64_t n, const double *x, const double *y, double *out)
+{
+    clear_error();
+    if (!out) {
+        set_error("dot: null result pointer");
+        return -1;
+    }
+    return dot(n, x, y, *out);
+}
+
+int sp3mm_jacobi(const sp3mm_dmat *A, const double *b, double *x, double *work, double omega, int32_t sweeps)
+{
+    clear_error();
+    if (!A) {
+        set_error("jacobi: null argument");
+        return -1;
+    }
+    return jacobi(*A, b, x, work, omega, sweeps);
+}
+
+int sp3mm_copy_to_device(void *dst, const void *src, int64_t bytes)
+{
+    clear_error();
+    if (bytes < 0 || (bytes && (!dst || !src))) {
+        set_error("copy: bad arguments");
+        return -1;
+    }
+    if (!bytes) return 0;
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    // stream-ordered after earlier calls that may still read dst
+    SP3MM_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyHostToDevice, c->stream));
+    SP3MM_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int sp3mm_vcycle_create(int32_t nlevels, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P,
+                        const sp3mm_dmat *coarse, int32_t nu1, int32_t nu2, double omega, int32_t coarse_sweeps,
+                        int32_t use_graph, sp3mm_vcycle **out)
+{
+    clear_error();
+    if (!out || !coarse || nlevels < 1 || !R || !A || !P) {
+        set_error("vcycle: null argument or no level");
+        return -1;
+    }
+    *out = nullptr;
+    void *v = vcycle_create(nlevels, R, A, P, *coarse, nu1, nu2, omega, coarse_sweeps, use_graph != 0);
+    if (!v) return -1;
+    *out = static_cast<sp3mm_vcycle *>(v);
+    return 0;
+}
+
+int sp3mm_vcycle_solve(sp3mm_vcycle *v, const double *b, double *x, double rtol, int32_t maxit, int32_t *cycles,
+                       double *hist, double *ms)
+{
+    clear_error();
+    if (!v) {
+        set_error("vcycle: null handle");
+        return -1;
+    }
+    int k = 0;
+    const int rc = vcycle_solve(v, b, x, rtol, maxit, &k, hist, ms);
+    if (cycles) *cycles = k;
+    return rc;
+}
+
+void sp3mm_vcycle_free(sp3mm_vcycle *v) { vcycle_free(v); }
+
 static void to_c(const Tuning &t, sp3mm_tuning &o)
 {
     o.u1 = t.u1;
@@ -416,9 +515,9 @@ int sp3mm_copy_to_host(void *dst, const void *src, int64_t bytes)
 {
     Context *c = ctx();
     if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
     SP3MM_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, hipMemcpyDeviceToHost, c->stream));
-    SP3MM_HIP(hipStreamSynchronize(c->stream));
-    return 0;
+    return consumer_wait(c);
 }
 
 double sp3mm_copy_probe(int64_t bytes, int32_t reps)

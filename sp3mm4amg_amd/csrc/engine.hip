@@ -1435,6 +1435,135 @@ int sa_prolongator(const sp3mm_dmat &A, const int64_t *agg, int64_t nagg, double
 }
 
 // --------------------------------------------------------------------------
+// level consumers: SpMV and damped Jacobi on FP64 operators (the operations a
+// V-cycle applies to the operators and transfer matrices built above)
+// --------------------------------------------------------------------------
+// Calls are stream-ordered on the library stream and return without waiting
+// for the device; results are visible to the host after a call that waits
+// (dot, copy to host, sync).  A Jacobi sweep that meets a zero diagonal entry
+// sets a sticky device error word, reported by the next waiting call.
+ConsState *cons_state(Context *c)
+{
+    if (!c->cons) {
+        const size_t bytes = sizeof(ConsState) + (size_t)dot_parts() * 8;
+        if (hipMalloc(&c->cons, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            c->cons = nullptr;
+            set_error("consumer state allocation failed");
+            return nullptr;
+        }
+        if (hipMemsetAsync(c->cons, 0, bytes, c->stream) != hipSuccess) {
+            set_error("consumer state initialisation failed");
+            return nullptr;
+        }
+    }
+    return static_cast<ConsState *>(c->cons);
+}
+
+int consumer_wait(Context *c, double *dot_out)
+{
+    if (!c->cons) {
+        SP3MM_HIP(hipStreamSynchronize(c->stream));
+        return 0;
+    }
+    ConsState h{};
+    SP3MM_HIP(hipMemcpyAsync(&h, c->cons, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    SP3MM_HIP(hipStreamSynchronize(c->stream));
+    if (dot_out) *dot_out = h.dot;
+    if (h.err) {
+        ConsState *d = static_cast<ConsState *>(c->cons);
+        SP3MM_HIP(hipMemsetAsync(&d->err, 0, sizeof(int), c->stream));
+        SP3MM_HIP(hipStreamSynchronize(c->stream));
+        set_error("jacobi: zero diagonal entry (sweep launched before this call)");
+        return -1;
+    }
+    return 0;
+}
+
+int spmv(const sp3mm_dmat &A, const double *x, const double *z, double *y, double alpha, double beta)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (A.vtype != SP3MM_F64) {
+        set_error("spmv: A must hold FP64 values");
+        return -1;
+    }
+    if (A.M && (!x || !y || (beta != 0.0 && !z))) {
+        set_error("spmv: null vector");
+        return -1;
+    }
+    launch_spmv(A.rp, A.col, static_cast<const double *>(A.val), A.M, A.nnz, x, z, y, alpha, beta, c->stream);
+    SP3MM_HIP(hipGetLastError());
+    return 0;
+}
+
+int axpby(int64_t n, double alpha, const double *x, double beta, double *y)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (n < 0 || (n && (!y || (alpha != 0.0 && !x)))) {
+        set_error("axpby: bad length or null vector");
+        return -1;
+    }
+    launch_axpby(n, alpha, x, beta, y, c->stream);
+    SP3MM_HIP(hipGetLastError());
+    return 0;
+}
+
+int dot(int64_t n, const double *x, const double *y, double &out)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (n < 0 || (n && (!x || !y))) {
+        set_error("dot: bad length or null vector");
+        return -1;
+    }
+    ConsState *st = cons_state(c);
+    if (!st) return -1;
+    launch_dot(n, x, y, reinterpret_cast<double *>(st + 1), &st->dot, c->stream);
+    SP3MM_HIP(hipGetLastError());
+    return consumer_wait(c, &out);
+}
+
+int jacobi(const sp3mm_dmat &A, const double *b, double *x, double *work, double omega, int sweeps)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (A.vtype != SP3MM_F64 || A.M != A.N) {
+        set_error("jacobi: A must be square with FP64 values");
+        return -1;
+    }
+    if (sweeps < 0 || (A.M && sweeps > 0 && (!b || !x || !work))) {
+        set_error("jacobi: bad sweep count or null vector");
+        return -1;
+    }
+    if (A.M == 0 || sweeps == 0) return 0;
+    ConsState *st = cons_state(c);
+    if (!st) return -1;
+    queue_jacobi(A, b, x, work, omega, sweeps, &st->err, c->stream);
+    SP3MM_HIP(hipGetLastError());
+    return 0;
+}
+
+void queue_jacobi(const sp3mm_dmat &A, const double *b, double *x, double *work, double omega, int sweeps, int *err,
+                  hipStream_t s)
+{
+    const double *val = static_cast<const double *>(A.val);
+    double *cur = x, *nxt = work;
+    for (int k = 0; k < sweeps; k++) {
+        launch_jacobi(A.rp, A.col, val, A.M, A.nnz, b, cur, nxt, omega, err, s);
+        std::swap(cur, nxt);
+    }
+    // an odd sweep count ends in the work vector: copy back with a kernel
+    // (no copy or fill nodes when the sequence is captured in a graph)
+    if (cur != x) launch_axpby(A.M, 1.0, cur, 0.0, x, s);
+}
+
+// --------------------------------------------------------------------------
 // autotuner: times the direct product of the caller's operands under candidate
 // schedules and installs the fastest in the context.  Coordinate descent from
 // the current setting, one parameter at a time; `reps` timed products per

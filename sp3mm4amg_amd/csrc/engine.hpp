@@ -55,8 +55,29 @@ struct Context {
     // small pinned host mirror for counters read back between phases
     long long *host_small = nullptr;
     long long *host_small_dev = nullptr;  // its device-side address (kernels write it directly)
+    // state of the level consumers (SpMV, Jacobi, dot): a dot result, a sticky
+    // error word set by Jacobi sweeps, and the dot's partial sums
+    void *cons = nullptr;
     std::recursive_mutex mu;
 };
+
+// Device state of the level consumers: the last dot result, a sticky error
+// word that Jacobi sweeps set on a zero diagonal entry, then the dot's
+// partial sums.  Allocated on first use.
+struct ConsState {
+    double dot;
+    int err;
+    int pad;
+};
+ConsState *cons_state(Context *c);
+
+// Waits for the library stream and reports (and clears) an error that a
+// stream-ordered consumer call left in the device error word.
+int consumer_wait(Context *c, double *dot_out = nullptr);
+
+// Queues `sweeps` damped Jacobi sweeps (x in place, work as the other buffer).
+void queue_jacobi(const sp3mm_dmat &A, const double *b, double *x, double *work, double omega, int sweeps, int *err,
+                  hipStream_t s);
 
 Context *ctx();  // initialised on first use (device 0 unless sp3mm_init chose one)
 
@@ -89,6 +110,15 @@ int batch_run(void *b, bool use_graph, double *ms);
 int tuning_valid(const Tuning &t);
 int transpose(const sp3mm_dmat &A, sp3mm_dmat &T);
 int sa_prolongator(const sp3mm_dmat &A, const int64_t *agg, int64_t nagg, double omega, int mode, sp3mm_dmat &P);
+int spmv(const sp3mm_dmat &A, const double *x, const double *z, double *y, double alpha, double beta);
+int axpby(int64_t n, double alpha, const double *x, double beta, double *y);
+int dot(int64_t n, const double *x, const double *y, double &out);
+int jacobi(const sp3mm_dmat &A, const double *b, double *x, double *work, double omega, int sweeps);
+void *vcycle_create(int nlev, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P, const sp3mm_dmat &Ac,
+                    int nu1, int nu2, double omega, int coarse_sweeps, bool graph);
+int vcycle_solve(void *v, const double *b, double *x, double rtol, int maxit, int *cycles, double *hist,
+                 double *ms);
+void vcycle_free(void *v);
 int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int prec, double guard, int reps,
              Tuning &best, double &ms_best, double &ms_default);
 int batch_result(void *b, int i, sp3mm_dmat &out);

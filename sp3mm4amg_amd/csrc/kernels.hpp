@@ -151,6 +151,13 @@ void launch_sa_fill(const int64_t *rp, const int32_t *col, const double *val, in
                     const int64_t *srp, int32_t *scol, double *sval, int *err, hipStream_t st);
 void launch_p0_fill(const int64_t *agg, int64_t M, int64_t nagg, int64_t *rp, int32_t *rp32, int32_t *col,
                     double *val, int *err, hipStream_t st);
+void launch_spmv(const int64_t *rp, const int32_t *col, const double *val, int64_t M, int64_t nnz, const double *x,
+                 const double *z, double *y, double alpha, double beta, hipStream_t st);
+void launch_axpby(int64_t n, double alpha, const double *x, double beta, double *y, hipStream_t st);
+void launch_dot(int64_t n, const double *x, const double *y, double *part, double *out, hipStream_t st);
+int dot_parts();
+void launch_jacobi(const int64_t *rp, const int32_t *col, const double *val, int64_t M, int64_t nnz, const double *b,
+                   const double *xi, double *xo, double omega, int *err, hipStream_t st);
 void launch_export_words(const void *src, void *host_dst, int words, hipStream_t st);
 void launch_copy_probe(const void *in, void *out, int64_t bytes, hipStream_t st);
 

@@ -198,3 +198,150 @@ void launch_p0_fill(const int64_t *agg, int64_t M, int64_t nagg, int64_t *rp, in
 }
 
 }  // namespace sp3mm
+
+namespace sp3mm {
+
+// ---------------------------------------------------------------------------
+// consumers of a level: y = alpha A x + beta y, and damped Jacobi sweeps
+// ---------------------------------------------------------------------------
+// y = alpha A x + beta z (z may be y; with beta == 0 z is not read).
+// L lanes per row (L a power of two up to 64), partial sums reduced by shuffles
+template <int L>
+__global__ void __launch_bounds__(256) k_spmv(const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                               const double *__restrict__ val, int64_t M,
+                                               const double *__restrict__ x, const double *z, double *y,
+                                               double alpha, double beta)
+{
+    const int sub = threadIdx.x & (L - 1);
+    const int64_t stride = (int64_t)gridDim.x * (blockDim.x / L);
+    for (int64_t r = (int64_t)blockIdx.x * (blockDim.x / L) + threadIdx.x / L; r < M; r += stride) {
+        double s = 0.0;
+        for (int64_t e = rp[r] + sub; e < rp[r + 1]; e += L) s += val[e] * x[col[e]];
+        for (int o = L / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        if (sub == 0) y[r] = alpha * s + (beta != 0.0 ? beta * z[r] : 0.0);
+    }
+}
+
+// y = alpha x + beta y; a zero coefficient skips the read of its vector
+__global__ void __launch_bounds__(256) k_axpby(int64_t n, double alpha, const double *x, double beta, double *y)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = (alpha != 0.0 ? alpha * x[i] : 0.0) + (beta != 0.0 ? beta * y[i] : 0.0);
+}
+
+// Dot product in two launches with a fixed summation order (the result does
+// not depend on scheduling): kDotBlocks partial sums, then one block adds them.
+constexpr int kDotBlocks = 1024;
+__global__ void __launch_bounds__(256) k_dot_partial(int64_t n, const double *__restrict__ x,
+                                                      const double *__restrict__ y, double *__restrict__ part)
+{
+    __shared__ double red[256 / kWave];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        s += x[i] * y[i];
+    for (int o = kWave / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane() == 0) red[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < 256 / kWave; w++) t += red[w];
+        part[blockIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_dot_final(const double *__restrict__ part, int nb, double *__restrict__ out)
+{
+    __shared__ double red[256 / kWave];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += blockDim.x) s += part[i];
+    for (int o = kWave / 2; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane() == 0) red[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < 256 / kWave; w++) t += red[w];
+        *out = t;
+    }
+}
+
+// xo = xi + omega (b - A xi) / a_ii
+template <int L>
+__global__ void __launch_bounds__(256) k_jacobi(const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                 const double *__restrict__ val, int64_t M,
+                                                 const double *__restrict__ b, const double *__restrict__ xi,
+                                                 double *__restrict__ xo, double omega, int *__restrict__ err)
+{
+    const int sub = threadIdx.x & (L - 1);
+    const int64_t stride = (int64_t)gridDim.x * (blockDim.x / L);
+    for (int64_t r = (int64_t)blockIdx.x * (blockDim.x / L) + threadIdx.x / L; r < M; r += stride) {
+        double s = 0.0, d = 0.0;
+        for (int64_t e = rp[r] + sub; e < rp[r + 1]; e += L) {
+            const int c = col[e];
+            s += val[e] * xi[c];
+            if (c == r) d = val[e];
+        }
+        for (int o = L / 2; o > 0; o >>= 1) {
+            s += __shfl_xor(s, o);
+            d += __shfl_xor(d, o);
+        }
+        if (sub == 0) {
+            if (d == 0.0) atomicOr(err, kErrBadInput);
+            else xo[r] = xi[r] + omega * (b[r] - s) / d;
+        }
+    }
+}
+
+static int lanes_for(int64_t M, int64_t nnz)
+{
+    const double avg = M ? (double)nnz / (double)M : 1.0;
+    int L = 1;
+    while (L < 64 && L < avg) L *= 2;
+    return L;
+}
+
+#define SP3MM_BY_LANES(K, ...)                                                                   \
+    switch (L) {                                                                                 \
+    case 1: hipLaunchKernelGGL(K<1>, __VA_ARGS__); break;                                        \
+    case 2: hipLaunchKernelGGL(K<2>, __VA_ARGS__); break;                                        \
+    case 4: hipLaunchKernelGGL(K<4>, __VA_ARGS__); break;                                        \
+    case 8: hipLaunchKernelGGL(K<8>, __VA_ARGS__); break;                                        \
+    case 16: hipLaunchKernelGGL(K<16>, __VA_ARGS__); break;                                      \
+    case 32: hipLaunchKernelGGL(K<32>, __VA_ARGS__); break;                                      \
+    default: hipLaunchKernelGGL(K<64>, __VA_ARGS__); break;                                      \
+    }
+
+void launch_spmv(const int64_t *rp, const int32_t *col, const double *val, int64_t M, int64_t nnz, const double *x,
+                 const double *z, double *y, double alpha, double beta, hipStream_t st)
+{
+    if (M == 0) return;
+    const int L = lanes_for(M, nnz);
+    const unsigned g = grid1(M, 256 / L);
+    SP3MM_BY_LANES(k_spmv, dim3(g), dim3(256), 0, st, rp, col, val, M, x, z, y, alpha, beta)
+}
+
+void launch_axpby(int64_t n, double alpha, const double *x, double beta, double *y, hipStream_t st)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_axpby, dim3(grid1(n, 256)), dim3(256), 0, st, n, alpha, x, beta, y);
+}
+
+void launch_dot(int64_t n, const double *x, const double *y, double *part, double *out, hipStream_t st)
+{
+    const unsigned nb = n ? (unsigned)std::min<int64_t>((n + 255) / 256, kDotBlocks) : 1;
+    hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(256), 0, st, n, x, y, part);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(256), 0, st, part, (int)nb, out);
+}
+
+int dot_parts() { return kDotBlocks; }
+
+void launch_jacobi(const int64_t *rp, const int32_t *col, const double *val, int64_t M, int64_t nnz, const double *b,
+                   const double *xi, double *xo, double omega, int *err, hipStream_t st)
+{
+    if (M == 0) return;
+    const int L = lanes_for(M, nnz);
+    const unsigned g = grid1(M, 256 / L);
+    SP3MM_BY_LANES(k_jacobi, dim3(g), dim3(256), 0, st, rp, col, val, M, b, xi, xo, omega, err)
+}
+#undef SP3MM_BY_LANES
+
+}  // namespace sp3mm

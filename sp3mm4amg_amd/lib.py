@@ -148,6 +148,18 @@ def lib():
         L.sp3mm_device_free.argtypes = [C.c_void_p]
         L.sp3mm_device_free.restype = None
         L.sp3mm_copy_to_host.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+        L.sp3mm_copy_to_device.argtypes = [C.c_void_p, C.c_void_p, C.c_int64]
+        L.sp3mm_spmv.argtypes = [P(DMat), C.c_void_p, C.c_void_p, C.c_double, C.c_double]
+        L.sp3mm_residual.argtypes = [P(DMat), C.c_void_p, C.c_void_p, C.c_void_p]
+        L.sp3mm_jacobi.argtypes = [P(DMat), C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int32]
+        L.sp3mm_axpby.argtypes = [C.c_int64, C.c_double, C.c_void_p, C.c_double, C.c_void_p]
+        L.sp3mm_dot.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, P(C.c_double)]
+        L.sp3mm_vcycle_create.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, P(DMat), C.c_int32,
+                                          C.c_int32, C.c_double, C.c_int32, C.c_int32, P(C.c_void_p)]
+        L.sp3mm_vcycle_solve.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int32,
+                                         P(C.c_int32), C.c_void_p, P(C.c_double)]
+        L.sp3mm_vcycle_free.argtypes = [C.c_void_p]
+        L.sp3mm_vcycle_free.restype = None
         L.sp3mm_sync.restype = C.c_int
         L.sp3mm_copy_probe.argtypes = [C.c_int64, C.c_int32]
         L.sp3mm_copy_probe.restype = C.c_double
@@ -259,6 +271,13 @@ class DeviceArray:
         self.ptr = lib().sp3mm_device_alloc(self.n * self.dtype.itemsize)
         if not self.ptr:
             raise RuntimeError("sp3mm_device_alloc failed")
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype)
+        _check(lib().sp3mm_copy_to_device(d.ptr, a.ctypes.data, a.nbytes), "sp3mm_copy_to_device")
+        return d
 
     def numpy(self) -> np.ndarray:
         out = np.empty(self.n, dtype=self.dtype)
@@ -373,6 +392,53 @@ def hierarchy_device(A: DeviceCSR, aggs, omega: float = 2.0 / 3.0, diag_mode: in
         levels.append((R, cur, P))
         cur = nxt
     return levels, cur
+
+
+# ---------------------------------------------------------------------------
+# level consumers (FP64 operators, device vectors as DeviceArray); queued on
+# the library's stream: dot(), DeviceArray.numpy() and sync() wait for them
+# ---------------------------------------------------------------------------
+def spmv(A: DeviceCSR, x: DeviceArray, y: DeviceArray, alpha: float = 1.0, beta: float = 0.0) -> None:
+    """y = alpha A x + beta y."""
+    if x.n != A.d.N or y.n != A.d.M:
+        raise ValueError("spmv: vector lengths do not match A")
+    _check(lib().sp3mm_spmv(C.byref(A.d), x.ptr, y.ptr, float(alpha), float(beta)), "sp3mm_spmv")
+
+
+def residual(A: DeviceCSR, b: DeviceArray, x: DeviceArray, r: DeviceArray) -> None:
+    """r = b - A x."""
+    if not (b.n == x.n == r.n == A.d.M):
+        raise ValueError("residual: vector lengths do not match A")
+    _check(lib().sp3mm_residual(C.byref(A.d), b.ptr, x.ptr, r.ptr), "sp3mm_residual")
+
+
+def jacobi(A: DeviceCSR, b: DeviceArray, x: DeviceArray, work: DeviceArray, omega: float = 2.0 / 3.0,
+           sweeps: int = 1) -> None:
+    """`sweeps` damped Jacobi sweeps on A x = b, in place on x."""
+    if not (b.n == x.n == A.d.M and work.n >= A.d.M):
+        raise ValueError("jacobi: vector lengths do not match A")
+    _check(lib().sp3mm_jacobi(C.byref(A.d), b.ptr, x.ptr, work.ptr, float(omega), int(sweeps)), "sp3mm_jacobi")
+
+
+def axpby(alpha: float, x: DeviceArray | None, beta: float, y: DeviceArray) -> None:
+    """y = alpha x + beta y (alpha = beta = 0 clears y)."""
+    if x is not None and x.n != y.n:
+        raise ValueError("axpby: vector lengths differ")
+    _check(lib().sp3mm_axpby(y.n, float(alpha), x.ptr if x is not None else None, float(beta), y.ptr),
+           "sp3mm_axpby")
+
+
+def sync() -> None:
+    """Waits for the library's stream; raises on an error a queued call left."""
+    _check(lib().sp3mm_sync(), "sp3mm_sync")
+
+
+def dot(x: DeviceArray, y: DeviceArray) -> float:
+    if x.n != y.n:
+        raise ValueError("dot: vector lengths differ")
+    out = C.c_double()
+    _check(lib().sp3mm_dot(x.n, x.ptr, y.ptr, C.byref(out)), "sp3mm_dot")
+    return out.value
 
 
 def get_tuning() -> dict:

@@ -1,0 +1,270 @@
+"""GPU: the consumers of a hierarchy (SpMV, residual, Jacobi, axpby, dot) and a
+V-cycle over a device-built hierarchy.
+
+Bar: SpMV, residual and Jacobi within 1e-13 relative (per entry, against the
+row's sum of absolute terms) of the same operation in scipy; axpby exact; dot
+identical bits on repeated calls and within 1e-13 of numpy.  The V-cycle's
+residual history matches a scipy V-cycle on the oracle's hierarchy (the same
+operations in the same order) to 1e-6 relative per cycle, plus 1e-12 for the
+rounding floor, and the residual drops by 1e-10 within 25 cycles.
+"""
+import functools
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import oracle as O
+from sp3mm4amg_amd import amg
+from sp3mm4amg_amd import lib as L
+from sp3mm4amg_amd import problems
+from sp3mm4amg_amd.csr import CSR
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu_lib():
+    L.init(0)
+    yield
+
+
+def _sp(m: CSR):
+    return sp.csr_matrix((m.vals, m.ja, m.irp), shape=m.shape)
+
+
+def _wide_rows(n, per_row, seed):
+    rng = np.random.default_rng(seed)
+    irp = np.arange(n + 1, dtype=np.int64) * per_row
+    ja = np.sort(np.stack([rng.choice(n, per_row, replace=False) for _ in range(n)]), axis=1).ravel()
+    return CSR(n, n, irp, ja.astype(np.int64), rng.standard_normal(n * per_row))
+
+
+@functools.lru_cache(maxsize=None)
+def _matrix(name):
+    if name == "poisson16":
+        return problems.poisson3d_sa(16)["A"]        # 7 per row: 8 lanes a row
+    if name == "P_poisson16":
+        return problems.poisson3d_sa(16)["P"]        # rectangular
+    if name == "irregular3000":
+        return problems.irregular_sa(3000, seed=4)["A"]
+    if name == "wide":
+        return _wide_rows(700, 130, 7)               # 64 lanes a row
+    raise KeyError(name)
+
+
+NAMES = ["poisson16", "P_poisson16", "irregular3000", "wide"]
+
+
+def _rel_close(got, want, absrow, tol=1e-13):
+    err = np.abs(got - want) / np.maximum(absrow, 1e-300)
+    assert err.max(initial=0.0) <= tol, f"max rel err {err.max()}"
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_spmv_and_residual(name):
+    m = _matrix(name)
+    rng = np.random.default_rng(11)
+    x, y0 = rng.standard_normal(m.N), rng.standard_normal(m.M)
+    S, Sa = _sp(m), abs(_sp(m))
+    dA = L.DeviceCSR.upload(m)
+    dx, dy = L.DeviceArray.from_numpy(x), L.DeviceArray.from_numpy(y0)
+    L.spmv(dA, dx, dy, 2.0, -0.5)
+    _rel_close(dy.numpy(), 2.0 * (S @ x) - 0.5 * y0, 2.0 * (Sa @ np.abs(x)) + 0.5 * np.abs(y0))
+    L.spmv(dA, dx, dy)   # beta 0: y is overwritten
+    _rel_close(dy.numpy(), S @ x, Sa @ np.abs(x))
+    if m.M == m.N:
+        db, dr = L.DeviceArray.from_numpy(y0), L.DeviceArray(m.M)
+        L.residual(dA, db, dx, dr)
+        _rel_close(dr.numpy(), y0 - S @ x, np.abs(y0) + Sa @ np.abs(x))
+        L.residual(dA, db, dx, db)   # in place on b
+        _rel_close(db.numpy(), y0 - S @ x, np.abs(y0) + Sa @ np.abs(x))
+        db.free()
+        dr.free()
+    for v in (dx, dy):
+        v.free()
+    dA.free()
+
+
+@pytest.mark.parametrize("sweeps", [1, 2, 5])
+def test_jacobi_sweeps(sweeps):
+    m = _matrix("irregular3000")
+    S = _sp(m)
+    d = S.diagonal()
+    rng = np.random.default_rng(3)
+    b, x = rng.standard_normal(m.M), rng.standard_normal(m.M)
+    want = x.copy()
+    for _ in range(sweeps):
+        want = want + 0.7 * (b - S @ want) / d
+    dA = L.DeviceCSR.upload(m)
+    db, dx, dw = L.DeviceArray.from_numpy(b), L.DeviceArray.from_numpy(x), L.DeviceArray(m.M)
+    L.jacobi(dA, db, dx, dw, 0.7, sweeps)
+    got = dx.numpy()
+    assert np.max(np.abs(got - want) / np.maximum(np.abs(want), 1.0)) <= 1e-11
+    for v in (db, dx, dw):
+        v.free()
+    dA.free()
+
+
+def test_jacobi_zero_diagonal_is_an_error():
+    n = 5
+    irp = np.arange(n + 1, dtype=np.int64)
+    ja = np.array([1, 0, 3, 2, 4], dtype=np.int64)   # rows 0-3 hold no diagonal entry
+    m = CSR(n, n, irp, ja, np.ones(n))
+    dA = L.DeviceCSR.upload(m)
+    db, dx, dw = L.DeviceArray.from_numpy(np.ones(n)), L.DeviceArray.from_numpy(np.zeros(n)), L.DeviceArray(n)
+    L.jacobi(dA, db, dx, dw, 1.0, 1)          # queued: reported by the next wait
+    with pytest.raises(RuntimeError, match="zero diagonal"):
+        L.sync()
+    L.sync()                                   # reported once, then cleared
+    for v in (db, dx, dw):
+        v.free()
+    dA.free()
+
+
+@pytest.mark.parametrize("n", [1, 255, 256 * 1024 + 17, 3_000_001])
+def test_axpby_and_dot(n):
+    rng = np.random.default_rng(n)
+    x, y = rng.standard_normal(n), rng.standard_normal(n)
+    dx, dy = L.DeviceArray.from_numpy(x), L.DeviceArray.from_numpy(y)
+    d1, d2 = L.dot(dx, dy), L.dot(dx, dy)
+    assert d1 == d2, "dot is not reproducible"
+    assert abs(d1 - float(np.dot(x, y))) <= 1e-13 * float(np.dot(np.abs(x), np.abs(y)))
+    L.axpby(2.0, dx, 0.25, dy)
+    assert np.array_equal(dy.numpy(), 2.0 * x + 0.25 * y)
+    L.axpby(0.0, None, 0.0, dy)
+    assert not np.any(dy.numpy())
+    dx.free()
+    dy.free()
+
+
+# ---------------------------------------------------------------------------
+# V-cycle over a device-built hierarchy
+def _host_vcycle(levels, Ac, b, nu=2, omega=2.0 / 3.0, cs=50, cycles=25):
+    """The same V-cycle as amg.VCycle, in scipy, on the oracle's hierarchy."""
+    def jac(A, rhs, x, k):
+        d = A.diagonal()
+        for _ in range(k):
+            x = x + omega * (rhs - A @ x) / d
+        return x
+
+    def cyc(lvl, rhs, x):
+        if lvl == len(levels):
+            return jac(Ac, rhs, np.zeros_like(rhs), cs)
+        R, A, P = levels[lvl]
+        x = jac(A, rhs, x, nu)
+        rc = R @ (rhs - A @ x)
+        x = x + P @ cyc(lvl + 1, rc, np.zeros_like(rc))
+        return jac(A, rhs, x, nu)
+
+    A0 = levels[0][1]
+    x = np.zeros_like(b)
+    bn = np.linalg.norm(b)
+    hist = [np.linalg.norm(b - A0 @ x) / bn]
+    for _ in range(cycles):
+        if hist[-1] <= 1e-10:
+            break
+        x = cyc(0, b, x)
+        hist.append(np.linalg.norm(b - A0 @ x) / bn)
+    return x, hist
+
+
+@pytest.mark.parametrize("n", [24, 32])
+def test_vcycle_residual_drop(n):
+    dims = (n, n, n)
+    A = problems.poisson_operator(dims)
+    aggs, host = [], []
+    cur = A
+    while dims[0] > 4:
+        agg, cdims = problems.structured_aggregation(dims, (2, 2, 2))
+        nagg = int(np.prod(cdims))
+        aggs.append((agg, nagg))
+        R, _, P = problems.galerkin_inputs(cur, agg, nagg)
+        host.append((_sp(R), _sp(cur), _sp(P)))
+        cur = O.sp3mm_pair(R, cur, P)
+        dims = cdims
+    dA = L.DeviceCSR.upload(A)
+    levels, coarse = L.hierarchy_device(dA, aggs)
+    rng = np.random.default_rng(n)
+    b = rng.standard_normal(A.M)
+    vc = amg.VCycle(levels, coarse)
+    x, hist = vc.solve(b, rtol=1e-10, maxit=25)
+    xh, hh = _host_vcycle(host, _sp(cur), b)
+    print(f"{n}^3: {len(levels)} levels, coarse {coarse.d.M}; cycles {len(hist) - 1}; "
+          f"factor {(hist[-1] / hist[0]) ** (1.0 / max(len(hist) - 1, 1)):.3f}")
+    assert hist[-1] <= 1e-10, f"no convergence: {hist}"
+    # the stopping test may trip one cycle apart when a residual sits at 1e-10
+    assert abs(len(hist) - len(hh)) <= 1, (hist, hh)
+    for g, w in zip(hist, hh):
+        assert abs(g - w) <= 1e-6 * w + 1e-12, (g, w)
+    if len(hist) == len(hh):
+        assert np.max(np.abs(x - xh)) <= 1e-8 * np.max(np.abs(xh))
+    vc.free()
+    for i, (dR, dAl, dP) in enumerate(levels):
+        dR.free()
+        dP.free()
+        if i > 0:
+            dAl.free()
+    coarse.free()
+    dA.free()
+
+
+def _device_hierarchy(n):
+    dims = (n, n, n)
+    A = problems.poisson_operator(dims)
+    aggs = []
+    while dims[0] > 4:
+        agg, cdims = problems.structured_aggregation(dims, (2, 2, 2))
+        aggs.append((agg, int(np.prod(cdims))))
+        dims = cdims
+    dA = L.DeviceCSR.upload(A)
+    levels, coarse = L.hierarchy_device(dA, aggs)
+    return A, dA, levels, coarse
+
+
+def _free_hierarchy(dA, levels, coarse):
+    for i, (dR, dAl, dP) in enumerate(levels):
+        dR.free()
+        dP.free()
+        if i > 0:
+            dAl.free()
+    coarse.free()
+    dA.free()
+
+
+def test_graph_vcycle_matches_call_by_call_cycle():
+    """The captured cycle replays the call-by-call cycle bit for bit, on
+    repeated solves with other device work in between and with a new
+    right-hand side and initial guess."""
+    A, dA, levels, coarse = _device_hierarchy(32)
+    rng = np.random.default_rng(5)
+    b1, b2, x2 = rng.standard_normal(A.M), rng.standard_normal(A.M), rng.standard_normal(A.M)
+    ref = amg.VCycle(levels, coarse)
+    want1 = ref.solve(b1, rtol=1e-10, maxit=30)
+    want2 = ref.solve(b2, x0=x2, rtol=1e-9, maxit=30)
+    ref.free()
+    for use_graph in (True, False):
+        g = amg.GraphVCycle(levels, coarse, use_graph=use_graph)
+        for rep in range(3):
+            x, hist, ms = g.solve(b1, rtol=1e-10, maxit=30)
+            assert hist == want1[1], (use_graph, rep)
+            assert np.array_equal(x, want1[0])
+            # eager work on the same stream between replays
+            out, _ = L.rap(levels[0][0], levels[0][1], levels[0][2])
+            out.free()
+            x, hist, ms = g.solve(b2, x0=x2, rtol=1e-9, maxit=30)
+            assert hist == want2[1], (use_graph, rep)
+            assert np.array_equal(x, want2[0])
+        print(f"graph={use_graph}: {len(hist) - 1} cycles, {ms / max(len(hist) - 1, 1):.3f} ms per cycle")
+        g.free()
+    _free_hierarchy(dA, levels, coarse)
+
+
+def test_graph_vcycle_rejects_unchained_levels():
+    A, dA, levels, coarse = _device_hierarchy(16)
+    R, Al, P = levels[0]
+    with pytest.raises(RuntimeError, match="do not chain"):
+        amg.GraphVCycle([(R, Al, P)], Al)
+    with pytest.raises(RuntimeError, match="do not chain"):
+        amg.GraphVCycle([(P, Al, R)], levels[1][1])
+    _free_hierarchy(dA, levels, coarse)
