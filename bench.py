@@ -344,6 +344,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
     ap.add_argument("--no-consumer", action="store_true", help="skip the V-cycle / SpMV leg")
+    ap.add_argument("--schedule", default=None,
+                    help="install this schedule instead of autotuning, e.g. u1=4,u2=3,subwave_h=0 "
+                         "(fields of sp3mm_tuning; the others keep their defaults)")
     ap.add_argument("--no-tune", action="store_true",
                     help="use the default schedule instead of autotuning each workload (untimed setup)")
     args = ap.parse_args()
@@ -370,7 +373,11 @@ def main():
     mac1, mac2, nnz_ra = st_pair["mac1"], st_pair["mac2"], st_pair["nnz_mid"]
     flop = 2 * (mac1 + mac2)
     schedule, tune_s = None, 0.0
-    if not args.no_tune and path == lib.DIRECT:
+    if args.schedule:
+        lib.set_tuning({k: int(v) for k, v in (f.split("=") for f in args.schedule.split(","))})
+        schedule = lib.get_tuning()
+        log(f"schedule installed: {schedule}")
+    elif not args.no_tune and path == lib.DIRECT:
         t0 = time.perf_counter()
         schedule, ms_best, ms_default = lib.autotune(dR, dA, dP, reps=3)
         tune_s = time.perf_counter() - t0

@@ -336,6 +336,9 @@ typedef struct {
                             the bound; a row whose table fills is redone by the
                             long-row kernels) */
     int32_t h2_div;      /* stage-2 table = max(64, stage-1 table / h2_div): 1, 2, 4, 8 */
+    int32_t xcd_map;     /* 1: the workgroups of each XCD (workgroup index mod 8) take a
+                            contiguous part of a bin's rows, 0: rows interleaved over all
+                            workgroups */
 } sp3mm_tuning;
 int sp3mm_get_tuning(sp3mm_tuning *t);
 /* NULL restores the defaults. */

@@ -434,6 +434,7 @@ static void to_c(const Tuning &t, sp3mm_tuning &o)
     o.slog2_cap = t.slog2_cap;
     o.h1_shift = t.h1_shift;
     o.h2_div = t.h2_div;
+    o.xcd_map = t.xcd_map;
 }
 
 int sp3mm_get_tuning(sp3mm_tuning *t)
@@ -461,6 +462,7 @@ int sp3mm_set_tuning(const sp3mm_tuning *t)
         n.slog2_cap = t->slog2_cap;
         n.h1_shift = t->h1_shift;
         n.h2_div = t->h2_div;
+        n.xcd_map = t->xcd_map;
         if (!tuning_valid(n)) {
             set_error("set_tuning: parameter out of range");
             return -1;

@@ -10,6 +10,9 @@
 namespace sp3mm {
 
 constexpr int kWave = 64;
+// MI355X (gfx950) has 8 XCDs, each with its own L2; the workgroups of a
+// launch are dispatched to them round-robin (workgroup b runs on XCD b % 8)
+constexpr int kXcds = 8;
 constexpr int kEmpty = -1;
 
 // Error bits raised by kernels instead of hanging or writing out of bounds.
@@ -217,6 +220,37 @@ __device__ __forceinline__ long long block_exclusive_scan(long long v, long long
     }
     __syncthreads();
     return before + x - v;
+}
+
+// Walk of a list of n rows by the groups of a grid (GPB groups per
+// workgroup): group g of workgroup b starts at list index i and steps by
+// `stride` while i < end.  nxcd == 1: workgroup b takes rows b*GPB.. and the
+// grid strides over the whole list.  nxcd > 1 (needs gridDim.x >= nxcd): the
+// list is cut into nxcd contiguous segments and the workgroups of XCD
+// x = b % nxcd stride through segment x only, so the rows one L2 serves at a
+// time are neighbours in the list (and in a mesh-ordered matrix share most of
+// the rows of the operands they read).  Every index in [0, n) is visited once.
+struct RowWalk {
+    int i, stride, end;
+};
+template <int GPB>
+__device__ __forceinline__ RowWalk row_walk(int n, int g, int nxcd)
+{
+    RowWalk w;
+    if (nxcd > 1) {
+        const int x = blockIdx.x % nxcd, j = blockIdx.x / nxcd;
+        const int nper = ((int)gridDim.x - x + nxcd - 1) / nxcd;
+        const int seg = (n + nxcd - 1) / nxcd;
+        const int s0 = min(n, x * seg);
+        w.end = min(n, s0 + seg);
+        w.i = s0 + j * GPB + g;
+        w.stride = nper * GPB;
+    } else {
+        w.i = blockIdx.x * GPB + g;
+        w.stride = gridDim.x * GPB;
+        w.end = n;
+    }
+    return w;
 }
 
 }  // namespace sp3mm

@@ -102,12 +102,12 @@ int tuning_valid(const Tuning &t)
                                             (t.subwave_h & (t.subwave_h - 1)) == 0);
     return (t.u1 == 2 || t.u1 == 4) && (t.u2 >= 2 && t.u2 <= 4) && pow2h && t.slog1_cap >= 0 &&
            t.slog1_cap <= 8 && t.slog2_cap >= 0 && t.slog2_cap <= 8 && t.h1_shift >= -1 && t.h1_shift <= 2 &&
-           (t.h2_div == 1 || t.h2_div == 2 || t.h2_div == 4 || t.h2_div == 8);
+           (t.h2_div == 1 || t.h2_div == 2 || t.h2_div == 4 || t.h2_div == 8) && (t.xcd_map == 0 || t.xcd_map == 1);
 }
 
 // initial scheduling: defaults, overridden by SP3MM_FUSED_U ("ab", a and b in
 // {2,4}), SP3MM_SUBWAVE_H, SP3MM_SLOG_MAX (both stages), SP3MM_H1_SHIFT,
-// SP3MM_H2_DIV
+// SP3MM_H2_DIV, SP3MM_XCD_MAP
 static void tuning_from_env(Tuning &t)
 {
     Tuning e = t;
@@ -121,6 +121,7 @@ static void tuning_from_env(Tuning &t)
     if (const char *v = getenv("SP3MM_SLOG_MAX")) e.slog1_cap = e.slog2_cap = atoi(v);
     if (const char *v = getenv("SP3MM_H1_SHIFT")) e.h1_shift = atoi(v);
     if (const char *v = getenv("SP3MM_H2_DIV")) e.h2_div = atoi(v);
+    if (const char *v = getenv("SP3MM_XCD_MAP")) e.xcd_map = atoi(v);
     if (tuning_valid(e)) t = e;
 }
 
@@ -637,7 +638,8 @@ static int spgemm_onepass(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &
         if (h.counts[i] == 0) continue;
         launch_spgemm_fused(cfg[i].G, cfg[i].H, sub_log(B, cfg[i].G), a, b, lists.as<int32_t>() + (int64_t)i * M,
                             d->counts + i, h.counts[i], rown.as<int32_t>(), slotn.as<int32_t>() + nbase[i],
-                            scol.as<int32_t>() + soff[i], sval.as<double>() + soff[i], &d->err, max_blocks(c), s);
+                            scol.as<int32_t>() + soff[i], sval.as<double>() + soff[i], &d->err, max_blocks(c), s,
+                            c->tune.xcd_map);
     }
     const int t1 = tm ? tm->mark() : -1;
     const int nlong = h.counts[thr.nb];
@@ -1553,6 +1555,9 @@ void queue_jacobi(const sp3mm_dmat &A, const double *b, double *x, double *work,
                   hipStream_t s)
 {
     const double *val = static_cast<const double *>(A.val);
+    if (sweeps > 1 && launch_jacobi_small(A.rp, A.col, val, A.M, A.nnz, b, x, omega, sweeps, err,
+                                          ctx()->lds_device, s))
+        return;
     double *cur = x, *nxt = work;
     for (int k = 0; k < sweeps; k++) {
         launch_jacobi(A.rp, A.col, val, A.M, A.nnz, b, cur, nxt, omega, err, s);
@@ -1608,6 +1613,7 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
         {&Tuning::subwave_h, {0, 128, 256, 512}}, {&Tuning::u1, {2, 4}},       {&Tuning::u2, {2, 3, 4}},
         {&Tuning::slog1_cap, {1, 2, 3}},            {&Tuning::slog2_cap, {1, 2}},
         {&Tuning::h1_shift, {-1, 0, 1}},            {&Tuning::h2_div, {2, 4, 8}},
+        {&Tuning::xcd_map, {1, 0}},
     };
     for (const Axis &ax : axes) {
         for (int v : ax.values) {

@@ -520,7 +520,7 @@ __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand 
                                                        int32_t *__restrict__ slotn,
                                                        int32_t *__restrict__ scol, double *__restrict__ sval,
                                                        int32_t *__restrict__ retry, int *__restrict__ nretry,
-                                                       int *__restrict__ err)
+                                                       int *__restrict__ err, int nxcd)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int GPB = 256 / G;
@@ -547,18 +547,19 @@ __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand 
     const VT *rval = static_cast<const VT *>(R.val);
     const VT *aval = static_cast<const VT *>(A.val);
     const VT *pval = static_cast<const VT *>(P.val);
+    const RowWalk w = row_walk<GPB>(n, g, nxcd);
+    const int stride = w.stride, end = w.end;
+    int i = w.i;
     // The row index and R's row bounds of the group's next row are loaded while
     // the current row is processed (two dependent loads off the critical path).
-    const int stride = gridDim.x * GPB;
-    int i = blockIdx.x * GPB + g;
-    int r = i < n ? rows[i] : 0;
+    int r = i < end ? rows[i] : 0;
     int64_t rb = 0, re = 0;
-    if (i < n) {
+    if (i < end) {
         rb = R.rp[r];
         re = R.rp[r + 1];
     }
-    for (; i < n; i += stride) {
-        const int rn = i + stride < n ? rows[i + stride] : 0;
+    for (; i < end; i += stride) {
+        const int rn = i + stride < end ? rows[i + stride] : 0;
         for (int s = t; s < W; s += G) {
             k1[s] = kEmpty;
             v1[s] = 0.0;
@@ -616,7 +617,7 @@ __global__ void __launch_bounds__(256) k_spgemm_fused(SpOperand A, SpOperand B, 
                                                        const int *__restrict__ count, int H, int slog,
                                                        int32_t *__restrict__ rownnz, int32_t *__restrict__ slotn,
                                                        int32_t *__restrict__ scol, double *__restrict__ sval,
-                                                       int *__restrict__ err)
+                                                       int *__restrict__ err, int nxcd)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int GPB = 256 / G;
@@ -635,16 +636,17 @@ __global__ void __launch_bounds__(256) k_spgemm_fused(SpOperand A, SpOperand B, 
     const int n = *count;
     const double *aval = static_cast<const double *>(A.val);
     const double *bval = static_cast<const double *>(B.val);
-    const int stride = gridDim.x * GPB;
-    int i = blockIdx.x * GPB + g;
-    int r = i < n ? rows[i] : 0;
+    const RowWalk w = row_walk<GPB>(n, g, nxcd);
+    const int stride = w.stride, end = w.end;
+    int i = w.i;
+    int r = i < end ? rows[i] : 0;
     int64_t rb = 0, re = 0;
-    if (i < n) {
+    if (i < end) {
         rb = A.rp[r];
         re = A.rp[r + 1];
     }
-    for (; i < n; i += stride) {
-        const int rn = i + stride < n ? rows[i + stride] : 0;
+    for (; i < end; i += stride) {
+        const int rn = i + stride < end ? rows[i + stride] : 0;
         for (int s = t; s < W; s += G) {
             k[s] = kEmpty;
             v[s] = 0.0;
@@ -686,12 +688,13 @@ static unsigned grid_for(int rows, int G, int max_blocks);
 
 void launch_spgemm_fused(int G, int H, int slog, const SpOperand &A, const SpOperand &B, const int32_t *rows,
                          const int *count, int rows_host, int32_t *rownnz, int32_t *slotn, int32_t *scol,
-                         double *sval, int *err, int max_blocks, hipStream_t st)
+                         double *sval, int *err, int max_blocks, hipStream_t st, int xcd_map)
 {
     const unsigned grid = grid_for(rows_host, G, max_blocks);
     const size_t lds = spgemm_fused_lds(G, H);
     slog = std::min(slog, ilog2((unsigned)G));
-#define SP3MM_SPG_ARGS A, B, rows, count, H, slog, rownnz, slotn, scol, sval, err
+    const int nxcd = xcd_map && grid >= (unsigned)kXcds ? kXcds : 1;
+#define SP3MM_SPG_ARGS A, B, rows, count, H, slog, rownnz, slotn, scol, sval, err, nxcd
     if (G == 32) hipLaunchKernelGGL((k_spgemm_fused<32, 4>), dim3(grid), dim3(256), lds, st, SP3MM_SPG_ARGS);
     else if (G == 64) hipLaunchKernelGGL((k_spgemm_fused<64, 4>), dim3(grid), dim3(256), lds, st, SP3MM_SPG_ARGS);
     else hipLaunchKernelGGL((k_spgemm_fused<256, 4>), dim3(grid), dim3(256), lds, st, SP3MM_SPG_ARGS);
@@ -1476,7 +1479,7 @@ size_t fused_lds(int G, int H1, int H2)
 }
 
 using FusedFn = void (*)(SpOperand, SpOperand, SpOperand, const int32_t *, const int *, int, int, int, int,
-                        int32_t *, int32_t *, int32_t *, double *, int32_t *, int *, int *);
+                        int32_t *, int32_t *, int32_t *, double *, int32_t *, int *, int *, int);
 
 // left entries held per lane: stage 1 U1 in {2, 4}, stage 2 U2 in {2, 3, 4}
 template <int G, typename VT>
@@ -1507,8 +1510,9 @@ void launch_direct_fused(int G, int H1, int H2, int slog1, int slog2, const SpOp
                              : (G == 32   ? fused_variant<32, double>(u1, u2)
                                 : G == 64 ? fused_variant<64, double>(u1, u2)
                                           : fused_variant<256, double>(u1, u2));
+    const int nxcd = tu.xcd_map && grid >= (unsigned)kXcds ? kXcds : 1;
     hipLaunchKernelGGL(fn, dim3(grid), dim3(256), lds, st, R, A, P, rows, count, H1, H2, slog1, slog2, rownnz,
-                       slotn, scol, sval, retry, nretry, err);
+                       slotn, scol, sval, retry, nretry, err, nxcd);
 }
 
 void launch_gather_rows(const int32_t *rows, const int *count, int rows_host, const int32_t *slotn, int cap,

@@ -90,6 +90,8 @@ struct Tuning {
     int h1_shift = 0;       // stage-1 table = bin upper bound >> h1_shift, -1: twice the bound
                             // (a full table -> retry)
     int h2_div = 4;         // stage-2 table = max(64, H1 / h2_div)
+    int xcd_map = 1;        // 1: each XCD (block % 8) works through its own contiguous part of
+                            // a bin's row list (rows that share A and P rows share one L2)
 };
 
 void launch_direct_fused(int G, int H1, int H2, int slog1, int slog2, const SpOperand &R, const SpOperand &A,
@@ -101,7 +103,7 @@ void launch_direct_fused(int G, int H1, int H2, int slog1, int slog2, const SpOp
 size_t spgemm_fused_lds(int G, int H);
 void launch_spgemm_fused(int G, int H, int slog, const SpOperand &A, const SpOperand &B, const int32_t *rows,
                          const int *count, int rows_host, int32_t *rownnz, int32_t *slotn, int32_t *scol,
-                         double *sval, int *err, int max_blocks, hipStream_t st);
+                         double *sval, int *err, int max_blocks, hipStream_t st, int xcd_map);
 void launch_gather_rows(const int32_t *rows, const int *count, int rows_host, const int32_t *slotn, int cap,
                         const int32_t *scol, const double *sval, const int64_t *crp, int32_t *ccol, double *cval,
                         int64_t out_cap, int *err, int max_blocks, hipStream_t st);
@@ -153,6 +155,11 @@ void launch_p0_fill(const int64_t *agg, int64_t M, int64_t nagg, int64_t *rp, in
                     double *val, int *err, hipStream_t st);
 void launch_spmv(const int64_t *rp, const int32_t *col, const double *val, int64_t M, int64_t nnz, const double *x,
                  const double *z, double *y, double alpha, double beta, hipStream_t st);
+// all sweeps in one workgroup when the system fits in max_lds bytes of LDS
+// (returns false, launching nothing, otherwise)
+bool launch_jacobi_small(const int64_t *rp, const int32_t *col, const double *val, int64_t M, int64_t nnz,
+                         const double *b, double *x, double omega, int sweeps, int *err, size_t max_lds,
+                         hipStream_t st);
 void launch_axpby(int64_t n, double alpha, const double *x, double beta, double *y, hipStream_t st);
 void launch_dot(int64_t n, const double *x, const double *y, double *part, double *out, hipStream_t st);
 int dot_parts();

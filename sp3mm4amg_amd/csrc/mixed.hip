@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(256) k_direct_mixed(SpOperand R, SpOperand A, 
                                                        double *__restrict__ rowsum,
                                                        int32_t *__restrict__ retry, int *__restrict__ nretry,
                                                        int32_t *__restrict__ fix, int *__restrict__ nfix,
-                                                       int *__restrict__ err)
+                                                       int *__restrict__ err, int nxcd)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int GPB = 256 / G;
@@ -276,16 +276,17 @@ __global__ void __launch_bounds__(256) k_direct_mixed(SpOperand R, SpOperand A, 
     const float *av = static_cast<const float *>(A.val);
     const float *pv = static_cast<const float *>(P.val);
     // the next row's index and bounds are loaded while this row is processed
-    const int stride = gridDim.x * GPB;
-    int i = blockIdx.x * GPB + g;
-    int r = i < n ? rows[i] : 0;
+    const RowWalk w = row_walk<GPB>(n, g, nxcd);
+    const int stride = w.stride, end = w.end;
+    int i = w.i;
+    int r = i < end ? rows[i] : 0;
     int64_t rb = 0, re = 0;
-    if (i < n) {
+    if (i < end) {
         rb = R.rp[r];
         re = R.rp[r + 1];
     }
-    for (; i < n; i += stride) {
-        const int rn = i + stride < n ? rows[i + stride] : 0;
+    for (; i < end; i += stride) {
+        const int rn = i + stride < end ? rows[i + stride] : 0;
         for (int s = t; s < H1; s += G) {
             k1[s] = kEmpty;
             h1[s] = 0.f;
@@ -407,8 +408,9 @@ void launch_direct_mixed(int G, int H1, int H2, int slog1, int slog2, float guar
     slog1 = std::min(slog1, std::min(cap, std::max(0, tu.slog1_cap)));
     slog2 = std::min(slog2, std::min(cap, std::max(0, tu.slog2_cap)));
     auto fn = G == 32 ? k_direct_mixed<32> : G == 64 ? k_direct_mixed<64> : k_direct_mixed<256>;
+    const int nxcd = tu.xcd_map && grid >= (unsigned)kXcds ? kXcds : 1;
     hipLaunchKernelGGL(fn, dim3(grid), dim3(256), lds, st, R, A, P, rows, count, H1, H2, slog1, slog2, guard,
-                       rownnz, slotn, scol, sval, rowsum, retry, nretry, fix, nfix, err);
+                       rownnz, slotn, scol, sval, rowsum, retry, nretry, fix, nfix, err, nxcd);
 }
 
 }  // namespace sp3mm

@@ -3,6 +3,8 @@
 // prolongator P = (I - omega D^-1 A) P0 (P0 = the 0/1 aggregation matrix).
 // The reference receives P and R from its caller; these kernels let a whole
 // hierarchy be built level after level without leaving HBM.
+#include <cstdlib>
+
 #include "device_common.hpp"
 #include "kernels.hpp"
 
@@ -291,9 +293,75 @@ __global__ void __launch_bounds__(256) k_jacobi(const int64_t *__restrict__ rp, 
     }
 }
 
+// All `sweeps` sweeps of a small system in one workgroup, with the matrix,
+// b and both x vectors staged in LDS once (a coarsest-level solve is a few
+// kilobytes; a launch per sweep would cost more than the sweep).  Rows are
+// split over L-lane groups as in k_jacobi, with the same per-row summation
+// order, so the result equals `sweeps` launches of k_jacobi.
+constexpr int kSmallRows = 2048;
+template <int L>
+__global__ void __launch_bounds__(1024) k_jacobi_small(const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                        const double *__restrict__ val, int M, int nnz,
+                                                        const double *__restrict__ b, double *__restrict__ x,
+                                                        double omega, int sweeps, int *__restrict__ err)
+{
+    extern __shared__ double lds[];
+    double *sv = lds;                   // nnz values
+    double *sb = sv + nnz;              // M
+    double *xs0 = sb + M, *xs1 = xs0 + M;
+    int *sc = reinterpret_cast<int *>(xs1 + M);  // nnz columns
+    int *sr = sc + nnz;                 // M + 1 offsets from rp[0]
+    const int64_t base = rp[0];
+    for (int i = threadIdx.x; i < nnz; i += blockDim.x) {
+        sv[i] = val[base + i];
+        sc[i] = col[base + i];
+    }
+    for (int i = threadIdx.x; i <= M; i += blockDim.x) sr[i] = (int)(rp[i] - base);
+    for (int i = threadIdx.x; i < M; i += blockDim.x) {
+        sb[i] = b[i];
+        xs0[i] = x[i];
+    }
+    __syncthreads();
+    const int sub = threadIdx.x & (L - 1);
+    double *xi = xs0, *xo = xs1;
+    for (int k = 0; k < sweeps; k++) {
+        for (int r = threadIdx.x / L; r < M; r += blockDim.x / L) {
+            double s = 0.0, d = 0.0;
+            for (int e = sr[r] + sub; e < sr[r + 1]; e += L) {
+                const int c = sc[e];
+                s += sv[e] * xi[c];
+                if (c == r) d = sv[e];
+            }
+            for (int o = L / 2; o > 0; o >>= 1) {
+                s += __shfl_xor(s, o);
+                d += __shfl_xor(d, o);
+            }
+            if (sub == 0) {
+                if (d == 0.0) atomicOr(err, kErrBadInput);
+                else xo[r] = xi[r] + omega * (sb[r] - s) / d;
+            }
+        }
+        __syncthreads();
+        double *t = xi;
+        xi = xo;
+        xo = t;
+    }
+    for (int i = threadIdx.x; i < M; i += blockDim.x) x[i] = xi[i];
+}
+
+static size_t jacobi_small_lds(int64_t M, int64_t nnz) { return (size_t)nnz * 12 + (size_t)M * 24 + (size_t)(M + 1) * 4; }
+
+// Lanes per row: the smallest power of two (at most 64) not below the mean
+// row length divided by SP3MM_SPMV_ROW_DIV (default 4: about four entries
+// per lane, measured best on SA hierarchies of 7-point operators).
 static int lanes_for(int64_t M, int64_t nnz)
 {
-    const double avg = M ? (double)nnz / (double)M : 1.0;
+    static const double div = [] {
+        const char *e = getenv("SP3MM_SPMV_ROW_DIV");
+        const double v = e ? atof(e) : 4.0;
+        return v >= 1.0 && v <= 64.0 ? v : 4.0;
+    }();
+    const double avg = M ? (double)nnz / (double)M / div : 1.0;
     int L = 1;
     while (L < 64 && L < avg) L *= 2;
     return L;
@@ -333,6 +401,18 @@ void launch_dot(int64_t n, const double *x, const double *y, double *part, doubl
 }
 
 int dot_parts() { return kDotBlocks; }
+
+bool launch_jacobi_small(const int64_t *rp, const int32_t *col, const double *val, int64_t M, int64_t nnz,
+                         const double *b, double *x, double omega, int sweeps, int *err, size_t max_lds,
+                         hipStream_t st)
+{
+    if (M <= 0 || M > kSmallRows || jacobi_small_lds(M, nnz) > max_lds) return false;
+    const int L = lanes_for(M, nnz);
+    const size_t lds = jacobi_small_lds(M, nnz);
+    SP3MM_BY_LANES(k_jacobi_small, dim3(1), dim3(1024), lds, st, rp, col, val, (int)M, (int)nnz, b, x, omega,
+                   sweeps, err)
+    return true;
+}
 
 void launch_jacobi(const int64_t *rp, const int32_t *col, const double *val, int64_t M, int64_t nnz, const double *b,
                    const double *xi, double *xo, double omega, int *err, hipStream_t st)

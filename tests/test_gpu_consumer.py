@@ -268,3 +268,43 @@ def test_graph_vcycle_rejects_unchained_levels():
     with pytest.raises(RuntimeError, match="do not chain"):
         amg.GraphVCycle([(P, Al, R)], levels[1][1])
     _free_hierarchy(dA, levels, coarse)
+
+
+SMALL = {"irr64": lambda: problems.irregular_sa(64, seed=9)["A"],           # staged in LDS
+         "p2d32": lambda: problems.poisson_operator((32, 32, 1)),          # staged in LDS
+         "p1d2048": lambda: problems.poisson_operator((2048, 1, 1)),       # staged in LDS (128 KB)
+         "p1d2049": lambda: problems.poisson_operator((2049, 1, 1)),       # too many rows: a launch per sweep
+         "irr1500": lambda: problems.irregular_sa(1500, seed=9)["A"]}      # too many entries
+
+
+@pytest.mark.parametrize("name", list(SMALL))
+def test_jacobi_many_sweeps_one_launch(name):
+    """Many sweeps on a small system run in one workgroup; the result equals
+    the same sweeps issued one call at a time."""
+    m = SMALL[name]()
+    M = m.M
+    # a diagonally dominant copy, so that 37 sweeps stay bounded
+    rows = np.repeat(np.arange(M), np.diff(m.irp))
+    absrow = np.bincount(rows, weights=np.abs(m.vals), minlength=M)
+    vals = np.where(m.ja == rows, absrow[rows] + 1.0, m.vals)
+    m = CSR(M, M, m.irp, m.ja, vals)
+    rng = np.random.default_rng(M)
+    b, x = rng.standard_normal(M), rng.standard_normal(M)
+    dA = L.DeviceCSR.upload(m)
+    db, dx, dw = L.DeviceArray.from_numpy(b), L.DeviceArray.from_numpy(x), L.DeviceArray(M)
+    L.jacobi(dA, db, dx, dw, 0.6, 37)
+    many = dx.numpy()
+    dx.free()
+    dx = L.DeviceArray.from_numpy(x)
+    for _ in range(37):
+        L.jacobi(dA, db, dx, dw, 0.6, 1)
+    one = dx.numpy()
+    assert np.array_equal(many, one)
+    S = _sp(m)
+    want = x.copy()
+    for _ in range(37):
+        want = want + 0.6 * (b - S @ want) / S.diagonal()
+    assert np.max(np.abs(many - want) / np.maximum(np.abs(want), 1.0)) <= 1e-10
+    for v in (db, dx, dw):
+        v.free()
+    dA.free()

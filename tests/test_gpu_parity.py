@@ -337,6 +337,8 @@ SCHEDULES = [
     {"subwave_h": 512, "u1": 2, "u2": 2, "slog1_cap": 1, "slog2_cap": 0},
     {"h1_shift": 1, "h2_div": 8},                       # tables that fill -> retry path
     {"h1_shift": 2, "h2_div": 1, "u1": 2, "u2": 4},
+    {"xcd_map": 0},                                     # rows interleaved over all workgroups
+    {"xcd_map": 0, "subwave_h": 0, "u2": 3},
 ]
 
 
