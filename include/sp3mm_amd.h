@@ -234,6 +234,16 @@ int sp3mm_dmat_upload(sp3mm_dmat *out, int64_t M, int64_t N, int64_t nnz,
 /* Download to host arrays of M+1, nnz, nnz entries (values as stored). */
 int sp3mm_dmat_download(const sp3mm_dmat *m, int64_t *irp, int64_t *ja, void *val,
                         int32_t base);
+/* The same for CSR already in device memory (device pointers; e.g. a
+ * PSBLAS-side matrix in the reference's spmat layout, 1-based with base 1):
+ * import validates and converts on the GPU into a library-owned matrix,
+ * export writes a matrix back into caller-owned device arrays (M+1, nnz, nnz
+ * entries).  No host copy of the arrays is made. */
+int sp3mm_dmat_import(sp3mm_dmat *out, int64_t M, int64_t N, int64_t nnz,
+                      const int64_t *d_irp, const int64_t *d_ja, const void *d_val,
+                      int32_t vtype, int32_t base);
+int sp3mm_dmat_export(const sp3mm_dmat *m, int64_t *d_irp, int64_t *d_ja, void *d_val,
+                      int32_t base);
 /* Convert values to another type in place (allocates a new value array). */
 int sp3mm_dmat_convert(sp3mm_dmat *m, int32_t vtype);
 void sp3mm_dmat_free(sp3mm_dmat *m);

@@ -118,6 +118,92 @@ int sp3mm_dmat_upload(sp3mm_dmat *out, int64_t M, int64_t N, int64_t nnz, const 
     return 0;
 }
 
+int sp3mm_dmat_import(sp3mm_dmat *out, int64_t M, int64_t N, int64_t nnz, const int64_t *d_irp,
+                      const int64_t *d_ja, const void *d_val, int32_t vtype, int32_t base)
+{
+    clear_error();
+    if (!out || !d_irp || (nnz && (!d_ja || !d_val))) {
+        set_error("import: null argument");
+        return -1;
+    }
+    if (M < 0 || N < 0 || nnz < 0 || N >= INT32_MAX || M >= INT32_MAX) {
+        set_error("import: unsupported shape %lld x %lld (%lld nnz)", (long long)M, (long long)N, (long long)nnz);
+        return -1;
+    }
+    if ((vtype != SP3MM_F64 && vtype != SP3MM_F32) || (base != 0 && base != 1)) {
+        set_error("import: bad value type or index base");
+        return -1;
+    }
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    hipStream_t s = c->stream;
+    sp3mm_dmat m;
+    if (dmat_alloc(m, M, N, nnz, vtype)) return -1;
+    Buf err(64);
+    if (!err.p) {
+        dmat_release(m);
+        set_error("import: allocation failed");
+        return -1;
+    }
+    // The row offsets are validated on the device before anything walks the
+    // columns (the column kernel trusts them for its bounds): one wait here.
+    int herr = 0;
+    int rc = 0;
+    do {
+        if (hipMemsetAsync(err.p, 0, 64, s) != hipSuccess) { rc = -1; break; }
+        launch_rp_convert(d_irp, M, nnz, base, m.rp, m.rp32, err.as<int>(), s);
+        if (hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) { rc = -1; break; }
+        if (herr) {
+            set_error("import: row offsets must be monotone and run from %d to nnz+%d", base, base);
+            rc = -2;
+            break;
+        }
+        if (nnz) {
+            launch_col_convert(d_ja, m.rp, M, N, base, m.col, err.as<int>(), s);
+            if (hipMemcpyAsync(m.val, d_val, (size_t)nnz * (vtype == SP3MM_F32 ? 4 : 8), hipMemcpyDeviceToDevice,
+                               s) != hipSuccess) { rc = -1; break; }
+        }
+        if (hipMemcpyAsync(&herr, err.p, 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess) { rc = -1; break; }
+        if (herr) {
+            set_error("import: columns must lie in [%d, N+%d) and increase strictly inside each row", base, base);
+            rc = -2;
+        }
+    } while (0);
+    if (rc) {
+        if (rc == -1) set_error("import: HIP error %s", hipGetErrorString(hipGetLastError()));
+        dmat_release(m);
+        return -1;
+    }
+    *out = m;
+    return 0;
+}
+
+int sp3mm_dmat_export(const sp3mm_dmat *m, int64_t *d_irp, int64_t *d_ja, void *d_val, int32_t base)
+{
+    clear_error();
+    if (!m || !d_irp || (base != 0 && base != 1)) {
+        set_error("export: null argument or bad index base");
+        return -1;
+    }
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    hipStream_t s = c->stream;
+    launch_rp_shift(m->rp, m->M + 1, base, d_irp, s);
+    if (m->nnz) {
+        if (d_ja) launch_col_widen(m->col, m->nnz, base, d_ja, s);
+        if (d_val)
+            SP3MM_HIP(hipMemcpyAsync(d_val, m->val, (size_t)m->nnz * (m->vtype == SP3MM_F32 ? 4 : 8),
+                                     hipMemcpyDeviceToDevice, s));
+    }
+    SP3MM_HIP(hipGetLastError());
+    SP3MM_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
 int sp3mm_dmat_download(const sp3mm_dmat *m, int64_t *irp, int64_t *ja, void *val, int32_t base)
 {
     clear_error();

@@ -124,6 +124,9 @@ def lib():
                                         C.c_void_p, C.c_void_p, C.c_int32, C.c_int32]
         L.sp3mm_dmat_download.argtypes = [P(DMat), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
         L.sp3mm_dmat_convert.argtypes = [P(DMat), C.c_int32]
+        L.sp3mm_dmat_import.argtypes = [P(DMat), C.c_int64, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, C.c_int32, C.c_int32]
+        L.sp3mm_dmat_export.argtypes = [P(DMat), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32]
         L.sp3mm_dmat_free.argtypes = [P(DMat)]
         L.sp3mm_dmat_free.restype = None
         L.sp3mm_rap.argtypes = [P(DMat), P(DMat), P(DMat), P(DMat), P(Opts), P(Stats)]
@@ -216,6 +219,25 @@ class DeviceCSR:
         _check(lib().sp3mm_dmat_download(C.byref(self.d), irp.ctypes.data, ja.ctypes.data,
                                          vals.ctypes.data, base), "sp3mm_dmat_download")
         return CSR(M, self.d.N, irp - base, ja[:nz] - base, vals[:nz])
+
+    @classmethod
+    def import_device(cls, M: int, N: int, nnz: int, irp: "DeviceArray", ja: "DeviceArray",
+                      vals: "DeviceArray", vtype: int = F64, base: int = 0) -> "DeviceCSR":
+        """A matrix whose int64 offsets/columns (index base `base`) and values
+        are already in device memory; converted on the GPU."""
+        d = DMat()
+        _check(lib().sp3mm_dmat_import(C.byref(d), M, N, nnz, irp.ptr, ja.ptr if ja else None,
+                                       vals.ptr if vals else None, vtype, base), "sp3mm_dmat_import")
+        return cls(d)
+
+    def export_device(self, irp: "DeviceArray", ja: "DeviceArray", vals: "DeviceArray", base: int = 0) -> None:
+        """Writes int64 offsets/columns (index base `base`) and values into
+        caller-owned device arrays of M+1, nnz and nnz entries."""
+        if irp.n < self.d.M + 1 or (ja is not None and ja.n < self.d.nnz) or \
+                (vals is not None and vals.n < self.d.nnz):
+            raise ValueError("export: arrays too short")
+        _check(lib().sp3mm_dmat_export(C.byref(self.d), irp.ptr, ja.ptr if ja is not None else None,
+                                       vals.ptr if vals is not None else None, base), "sp3mm_dmat_export")
 
     def convert(self, vtype: int) -> None:
         _check(lib().sp3mm_dmat_convert(C.byref(self.d), vtype), "sp3mm_dmat_convert")

@@ -629,3 +629,48 @@ def test_poisson3d128_full_size():
         assert st["nnz_out"] == 8382712 and st["mac1"] == 57740288
     for m in (dR, dA, dP):
         m.free()
+
+
+# ---------------------------------------------------------------------------
+# device-pointer import/export (CSR already in HBM, reference spmat layout)
+@pytest.mark.parametrize("base", [0, 1])
+def test_device_import_export_round_trip(poisson2d64, base):
+    R, A, P = poisson2d64
+    dev = []
+    for m in (R, A, P):
+        irp = L.DeviceArray.from_numpy(m.irp.astype(np.int64) + base)
+        ja = L.DeviceArray.from_numpy(m.ja.astype(np.int64) + base)
+        va = L.DeviceArray.from_numpy(m.vals.astype(np.float64))
+        dev.append(L.DeviceCSR.import_device(m.M, m.N, m.nnz, irp, ja, va, base=base))
+        for v in (irp, ja, va):
+            v.free()
+    out, _ = L.rap(*dev, path=L.DIRECT)
+    want = out.download()
+    oirp, oja, ova = L.DeviceArray(out.d.M + 1, np.int64), L.DeviceArray(out.d.nnz, np.int64), L.DeviceArray(out.d.nnz)
+    out.export_device(oirp, oja, ova, base=base)
+    assert np.array_equal(oirp.numpy() - base, want.irp)
+    assert np.array_equal(oja.numpy() - base, want.ja)
+    assert np.array_equal(ova.numpy(), want.vals)
+    check(want, O.sp3mm_merged(R, A, P), O.abs_product(R, A, P))
+    for v in (oirp, oja, ova):
+        v.free()
+    for m in dev + [out]:
+        m.free()
+
+
+def test_device_import_rejects_bad_csr():
+    n = 4
+    good_irp = np.array([0, 1, 2, 3, 4], dtype=np.int64)
+    cases = {
+        "decreasing offsets": (np.array([0, 2, 1, 3, 4], dtype=np.int64), np.array([0, 1, 2, 3])),
+        "offsets past nnz": (np.array([0, 1, 2, 3, 9], dtype=np.int64), np.array([0, 1, 2, 3])),
+        "column out of range": (good_irp, np.array([0, 1, 2, 4])),
+    }
+    for what, (irp, ja) in cases.items():
+        di = L.DeviceArray.from_numpy(irp)
+        dj = L.DeviceArray.from_numpy(ja.astype(np.int64))
+        dv = L.DeviceArray.from_numpy(np.ones(4))
+        with pytest.raises(RuntimeError, match="import"):
+            L.DeviceCSR.import_device(n, n, 4, di, dj, dv)
+        for v in (di, dj, dv):
+            v.free()
