@@ -845,7 +845,8 @@ static int rap_direct_twopass(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp
         if (dw.alloc(dense_blocks(c, h.counts[thr.nb]), A.N, P.N, false, s)) return -1;
         launch_dense_direct(false, dense_blocks(c, h.counts[thr.nb]), r, a, p,
                             lists.as<int32_t>() + (int64_t)thr.nb * M, d->counts + thr.nb, dw.ws, dlog1, dlog2,
-                            n1.as<int32_t>(), rown.as<int32_t>(), nullptr, nullptr, nullptr, &d->err, s);
+                            n1.as<int32_t>(), rown.as<int32_t>(), nullptr, nullptr, nullptr, &d->err, s, false,
+                            nullptr, c->max_lds);
     }
     const int es1 = tm.mark();
     if (dmat_alloc(C, M, P.N, 0, SP3MM_F64)) return -1;
@@ -864,7 +865,7 @@ static int rap_direct_twopass(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp
         }
         launch_dense_direct(false, dense_blocks(c, h.nretry), r, a, p, retry.as<int32_t>(), &d->nretry, dw.ws,
                             dlog1, dlog2, n1.as<int32_t>(), rown.as<int32_t>(), nullptr, nullptr, nullptr,
-                            &d->err, s);
+                            &d->err, s, false, nullptr, c->max_lds);
         SP3MM_HIP(hipMemsetAsync(d->counts2, 0, sizeof(d->counts2), s));
         launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d->nnz, s);
         launch_rebin(n1.as<int32_t>(), rown.as<int32_t>(), M, lists.as<int32_t>(), d->counts2, thr2, binscr.as<int>(), s);
@@ -915,7 +916,7 @@ static int rap_direct_twopass(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp
         }
         launch_dense_direct(true, wb, r, a, p, lists.as<int32_t>() + (int64_t)thr2.nb * M, d->counts2 + thr2.nb,
                             dwn.ws, dlog1, dlog2, nullptr, nullptr, C.rp, C.col, static_cast<double *>(C.val),
-                            &d->err, s);
+                            &d->err, s, false, nullptr, c->max_lds);
         launches++;
     }
     const int en1 = tm.mark();
@@ -1078,7 +1079,7 @@ struct FusedPlan {
         if (nspill > 0)
             launch_dense_direct(num, dense_blocks(c, nspill), r(), a(), p(), spill.as<int32_t>(), &d()->nspill,
                                 sws.ws, dlog1, dlog2, nullptr, rn, num ? C.rp : nullptr, num ? C.col : nullptr, cv,
-                                &d()->err, s, f32, rs);
+                                &d()->err, s, f32, rs, c->max_lds);
     }
 
     void launch_fixups(Context *c, hipStream_t s, bool all_bins)
@@ -1191,7 +1192,7 @@ struct FusedPlan {
             if (sws.alloc(dense_blocks(c, nspill), A.N, P.N, true, s)) return -1;
             launch_dense_direct(false, dense_blocks(c, nspill), r(), a(), p(), spill.as<int32_t>(), &d()->nspill,
                                 sws.ws, dlog1, dlog2, nullptr, rown.as<int32_t>(), nullptr, nullptr, nullptr,
-                                &d()->err, s, f32);
+                                &d()->err, s, f32, nullptr, c->max_lds);
             launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d()->nnz, s);
             if (read_ctl(c, d(), h)) return -1;
         }

@@ -64,14 +64,19 @@ __device__ __forceinline__ long long row_measure(int64_t r, const int64_t *__res
     }
     const int64_t a0 = arp[r], a1 = arp[r + 1];
     long long u = 0;
-    int64_t e = a0;
-    for (; e + 4 <= a1; e += 4) {  // four independent gathers in flight
-        const int k0 = acol[e], k1 = acol[e + 1], k2 = acol[e + 2], k3 = acol[e + 3];
-        u += (brp[k0 + 1] - brp[k0]) + (brp[k1 + 1] - brp[k1]) + (brp[k2 + 1] - brp[k2]) + (brp[k3 + 1] - brp[k3]);
-    }
-    for (; e < a1; e++) {
-        const int k = acol[e];
-        u += brp[k + 1] - brp[k];
+    // batches of kB entries: all their columns, then all their row bounds, are
+    // loaded back to back (two memory round trips per batch); entries past
+    // the row read its first entry and count nothing
+    constexpr int kB = 16;
+    for (int64_t e = a0; e < a1; e += kB) {
+        int k[kB];
+#pragma unroll
+        for (int j = 0; j < kB; j++) k[j] = acol[e + j < a1 ? e + j : a0];
+        int len[kB];
+#pragma unroll
+        for (int j = 0; j < kB; j++) len[j] = brp[k[j] + 1] - brp[k[j]];
+#pragma unroll
+        for (int j = 0; j < kB; j++) u += e + j < a1 ? len[j] : 0;
     }
     return u;
 }
@@ -860,8 +865,8 @@ __device__ __forceinline__ void dense_expand(const int32_t *lcol, const LV *lval
 
 // sorted emission of the set bits (and values) of a workspace; returns the
 // count.  Entries beyond `cap` are counted but not written.
-__device__ int dense_emit(unsigned *bits, double *vals, int nw, int32_t *outc, double *outv, long long cap,
-                          long long *sh)
+__device__ __forceinline__ int dense_emit(unsigned *bits, double *vals, int nw, int32_t *outc, double *outv,
+                                          long long cap, long long *sh)
 {
     const int t = threadIdx.x;
     long long base = 0;
@@ -937,8 +942,12 @@ __global__ void __launch_bounds__(kDense) k_dense_spgemm(SpOperand A, SpOperand 
 
 // direct triple product rows that are too long for the LDS tables:
 // row of R*A densely over A's columns, compacted into a sorted list, then
-// forwarded densely over P's columns.
-template <bool NUM, typename VT = double>
+// forwarded densely over P's columns.  L2: P's column space fits in LDS, and
+// the second stage accumulates there (bitmap + values) instead of in global
+// memory; L1: the first stage's bitmap over A's columns lives in LDS (its
+// values stay in global memory).  LDS workspaces are zeroed once per
+// workgroup, and emission clears what it emits.
+template <bool NUM, typename VT = double, bool L1 = false, bool L2 = false>
 __global__ void __launch_bounds__(kDense) k_dense_direct(SpOperand R, SpOperand A, SpOperand P,
                                                           const int32_t *__restrict__ rows,
                                                           const int *__restrict__ count, DenseWs ws,
@@ -951,12 +960,30 @@ __global__ void __launch_bounds__(kDense) k_dense_direct(SpOperand R, SpOperand 
     __shared__ long long sh[kDense / 64];
     __shared__ double dsh[kDense / 64];
     const size_t b = blockIdx.x;
+    extern __shared__ __align__(16) unsigned char dsm[];
+    // LDS layout: [stage-2 values | stage-2 bitmap | stage-1 bitmap]
+    const size_t l2bytes = L2 ? (NUM ? (size_t)ws.nwP * 32 * 8 : 0) + (size_t)ws.nwP * 4 : 0;
     unsigned *bitsA = ws.bitsA + b * ws.nwA;
+    if constexpr (L1) {
+        bitsA = reinterpret_cast<unsigned *>(dsm + l2bytes);
+        for (int w = threadIdx.x; w < ws.nwA; w += kDense) bitsA[w] = 0u;
+    }
     double *valsA = NUM ? ws.valsA + b * ((size_t)ws.nwA * 32) : nullptr;
     int32_t *listc = ws.listc + b * ((size_t)ws.nwA * 32);
     double *listv = NUM ? ws.listv + b * ((size_t)ws.nwA * 32) : nullptr;
-    unsigned *bitsP = ws.bitsP + b * ws.nwP;
-    double *valsP = NUM ? ws.valsP + b * ((size_t)ws.nwP * 32) : nullptr;
+    unsigned *bitsP;
+    double *valsP;
+    if constexpr (L2) {
+        valsP = NUM ? reinterpret_cast<double *>(dsm) : nullptr;
+        bitsP = reinterpret_cast<unsigned *>(dsm + (NUM ? (size_t)ws.nwP * 32 * 8 : 0));
+        for (int w = threadIdx.x; w < ws.nwP; w += kDense) bitsP[w] = 0u;
+        if constexpr (NUM)
+            for (int s = threadIdx.x; s < ws.nwP * 32; s += kDense) valsP[s] = 0.0;
+    } else {
+        bitsP = ws.bitsP + b * ws.nwP;
+        valsP = NUM ? ws.valsP + b * ((size_t)ws.nwP * 32) : nullptr;
+    }
+    if constexpr (L1 || L2) __syncthreads();
     const VT *rval = static_cast<const VT *>(R.val);
     const VT *aval = static_cast<const VT *>(A.val);
     const VT *pval = static_cast<const VT *>(P.val);
@@ -1453,23 +1480,41 @@ void launch_window_direct(bool num, int blocks, size_t lds, const SpOperand &R, 
 void launch_dense_direct(bool num, int blocks, const SpOperand &R, const SpOperand &A, const SpOperand &P,
                          const int32_t *rows, const int *count, const DenseWs &ws, int slog1, int slog2,
                          int32_t *n1out, int32_t *rownnz, const int64_t *crp, int32_t *ccol, double *cval,
-                         int *err, hipStream_t st, bool f32in, double *rowsum)
+                         int *err, hipStream_t st, bool f32in, double *rowsum, size_t max_lds)
 {
+    // LDS workspaces within max_lds bytes: stage 2 (P's column space: bitmap,
+    // and values in the numeric pass) first, as it takes most of the
+    // accumulations, then stage 1's bitmap over A's columns
+    const size_t lds2 = (size_t)ws.nwP * 4 + (num ? (size_t)ws.nwP * 32 * 8 : 0);
+    const size_t lds1 = (size_t)ws.nwA * 4;
+    const bool l2 = lds2 <= max_lds;
+    const bool l1 = (l2 ? lds2 : 0) + lds1 <= max_lds;
+    const size_t lds = (l2 ? lds2 : 0) + (l1 ? lds1 : 0);
+#define SP3MM_DENSE_ARGS R, A, P, rows, count, ws, slog1, slog2, n1out, rownnz, crp, ccol, cval, rowsum, err
+#define SP3MM_DENSE_LAUNCH(NUMV, VTT)                                                                           \
+    do {                                                                                                         \
+        if (l1 && l2)                                                                                            \
+            hipLaunchKernelGGL((k_dense_direct<NUMV, VTT, true, true>), dim3(blocks), dim3(kDense), lds, st,    \
+                               SP3MM_DENSE_ARGS);                                                                \
+        else if (l2)                                                                                             \
+            hipLaunchKernelGGL((k_dense_direct<NUMV, VTT, false, true>), dim3(blocks), dim3(kDense), lds, st,   \
+                               SP3MM_DENSE_ARGS);                                                                \
+        else if (l1)                                                                                             \
+            hipLaunchKernelGGL((k_dense_direct<NUMV, VTT, true, false>), dim3(blocks), dim3(kDense), lds, st,   \
+                               SP3MM_DENSE_ARGS);                                                                \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_dense_direct<NUMV, VTT, false, false>), dim3(blocks), dim3(kDense), 0, st,    \
+                               SP3MM_DENSE_ARGS);                                                                \
+    } while (0)
     if (f32in) {
-        if (num)
-            hipLaunchKernelGGL((k_dense_direct<true, float>), dim3(blocks), dim3(kDense), 0, st, R, A, P, rows, count,
-                               ws, slog1, slog2, n1out, rownnz, crp, ccol, cval, rowsum, err);
-        else
-            hipLaunchKernelGGL((k_dense_direct<false, float>), dim3(blocks), dim3(kDense), 0, st, R, A, P, rows,
-                               count, ws, slog1, slog2, n1out, rownnz, crp, ccol, cval, rowsum, err);
-        return;
+        if (num) SP3MM_DENSE_LAUNCH(true, float);
+        else SP3MM_DENSE_LAUNCH(false, float);
+    } else {
+        if (num) SP3MM_DENSE_LAUNCH(true, double);
+        else SP3MM_DENSE_LAUNCH(false, double);
     }
-    if (num)
-        hipLaunchKernelGGL(k_dense_direct<true>, dim3(blocks), dim3(kDense), 0, st, R, A, P, rows, count, ws,
-                           slog1, slog2, n1out, rownnz, crp, ccol, cval, rowsum, err);
-    else
-        hipLaunchKernelGGL(k_dense_direct<false>, dim3(blocks), dim3(kDense), 0, st, R, A, P, rows, count, ws,
-                           slog1, slog2, n1out, rownnz, crp, ccol, cval, rowsum, err);
+#undef SP3MM_DENSE_LAUNCH
+#undef SP3MM_DENSE_ARGS
 }
 
 size_t fused_lds(int G, int H1, int H2)

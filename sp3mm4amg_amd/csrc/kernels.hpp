@@ -76,7 +76,8 @@ void launch_window_direct(bool num, int blocks, size_t lds, const SpOperand &R, 
 void launch_dense_direct(bool num, int blocks, const SpOperand &R, const SpOperand &A, const SpOperand &P,
                          const int32_t *rows, const int *count, const DenseWs &ws, int slog1, int slog2,
                          int32_t *n1out, int32_t *rownnz, const int64_t *crp, int32_t *ccol, double *cval,
-                         int *err, hipStream_t st, bool f32in = false, double *rowsum = nullptr);
+                         int *err, hipStream_t st, bool f32in = false, double *rowsum = nullptr,
+                         size_t max_lds = 0);
 
 size_t fused_lds(int G, int H1, int H2);
 // Scheduling parameters of the one-pass direct kernels.  They decide table
