@@ -344,7 +344,7 @@ struct Ctl {
 
 static int max_blocks(const Context *c) { return c->num_cu * 16; }
 
-constexpr int kDenseThreads = 256;
+constexpr int kDenseThreads = 1024;
 // persistent workgroups (each with its own dense workspace) for the long-row kernels
 static int dense_blocks(const Context *c, int rows)
 {

@@ -816,10 +816,12 @@ __global__ void k_scan_add(int64_t *__restrict__ out, int64_t n, const long long
 // Each persistent workgroup owns one workspace: a presence bitmap of 32-bit
 // words and (numeric) a dense FP64 value array over the output columns.
 // Products set bits with global atomicOr and add values with global FP64
-// atomics; sorted emission walks the bitmap word by word (256 words per step,
+// atomics; sorted emission walks the bitmap word by word (1024 words per step,
 // one per thread, a workgroup scan giving output positions) and clears every
-// bit and value it emits, so the workspace is clean for the next row.
-constexpr int kDense = 256;
+// bit and value it emits, so the workspace is clean for the next row.  1024
+// threads (16 wavefronts) per workgroup: a workgroup whose LDS workspace
+// fills a CU still has the waves to hide the latency of its row gathers.
+constexpr int kDense = 1024;
 
 __device__ __forceinline__ unsigned ld_word(const unsigned *p)
 {
