@@ -43,13 +43,23 @@ def log(msg):
 
 def alg_bytes(R, A, P, mac1, mac2, nnz_ra, nnz_out, vbytes=8):
     """Row-gather traffic of the product (stream R, gather A rows by R, gather P
-    rows by RA, write RAP), 4-byte column indices and row offsets."""
-    i = p = 4
+    rows by RA, write RAP), 4-byte column indices; row offsets of 4 bytes, or
+    8 for a matrix with 2^31 or more entries."""
+    i = 4
+    po = lambda nnz: 8 if nnz >= 2 ** 31 else 4
     m = R.M
-    return (R.nnz * (vbytes + i) + (m + 1) * p
-            + 2 * p * R.nnz + (vbytes + i) * mac1
-            + 2 * p * nnz_ra + (vbytes + i) * mac2
-            + nnz_out * (vbytes + i) + (m + 1) * p)
+    return (R.nnz * (vbytes + i) + (m + 1) * po(R.nnz)
+            + 2 * po(A.nnz) * R.nnz + (vbytes + i) * mac1
+            + 2 * po(P.nnz) * nnz_ra + (vbytes + i) * mac2
+            + nnz_out * (vbytes + i) + (m + 1) * po(nnz_out))
+
+
+def compulsory_bytes(R, A, P, nnz_out, vbytes=8):
+    """Each operand read once and the product written once (values, 4-byte
+    columns, row offsets)."""
+    po = lambda nnz: 8 if nnz >= 2 ** 31 else 4
+    return sum(m.nnz * (vbytes + 4) + (m.M + 1) * po(m.nnz) for m in (R, A, P)) + \
+        nnz_out * (vbytes + 4) + (R.M + 1) * po(nnz_out)
 
 
 def cpu_baseline(R, A, P, flop_per_row_fn, budget_s=20.0):
@@ -176,6 +186,7 @@ def batch_leg(lib, problems, reps):
                        "(sides U{16..256}, PCG64 seed 5, 2x2 SA), direct path",
            "products": len(items)}
     flop = 0
+    b_alg = b_comp = 0
     lib.set_tuning(None)
     for form, vt, prec in (("fp64", lib.F64, lib.FP64), ("mixed", lib.F32, lib.MIXED),
                            ("mixed64", lib.F32, lib.MIXED64)):
@@ -186,10 +197,12 @@ def batch_leg(lib, problems, reps):
             mats.append(tuple(lib.DeviceCSR.upload(m, vtype=vt) for m in (R, A, P)))
         kw = {"path": lib.DIRECT, "precision": prec}
         if form == "fp64":
-            for (dR, dA, dP) in mats:
+            for (R, A, P), (dR, dA, dP) in zip(items, mats):
                 out, st = lib.rap(dR, dA, dP, path=lib.PAIR)
                 out.free()
                 flop += 2 * (st["mac1"] + st["mac2"])
+                b_alg += alg_bytes(R, A, P, st["mac1"], st["mac2"], st["nnz_mid"], st["nnz_out"])
+                b_comp += compulsory_bytes(R, A, P, st["nnz_out"])
         # per-product loop
         for (dR, dA, dP) in mats:
             out, _ = lib.rap(dR, dA, dP, **kw)
@@ -235,6 +248,11 @@ def batch_leg(lib, problems, reps):
                      "graph_build_ms": round(info["ms_build"], 3),
                      "max_rel_diff_batched_vs_single": worst}
     rec["flop"] = flop
+    rec["bytes_alg_fp64"] = b_alg
+    rec["bytes_compulsory_fp64"] = b_comp
+    g = rec["fp64"]["graph_device_ms_median"] * 1e-3
+    rec["fp64"]["roofline_frac"] = round(b_alg / g / 1e9 / HBM_PEAK_GBS, 4)
+    rec["fp64"]["roofline_frac_compulsory"] = round(b_comp / g / 1e9 / HBM_PEAK_GBS, 4)
     # the same 5-level hierarchy built entirely on the device: prolongator,
     # transpose and R*A*P per level (the operators above were built on the host)
     dims = (96, 96, 96)
@@ -333,6 +351,49 @@ def consumer_leg(lib, problems, dA, n, reps=50):
                           "roofline_frac": round(spmv_bytes / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
 
 
+def irregular_leg(lib, problems, reps=2, n=1_000_000):
+    """Irregular rows at full size: lognormal row lengths on 1M x 1M, random
+    aggregation (PCG64 seed 4), FP64, direct path.  MAC2 (the products of the
+    second stage) comes from the row upper bounds of (R*A)*P, with R*A formed
+    once outside the timed region."""
+    import numpy as np
+    t0 = time.perf_counter()
+    p = problems.irregular_sa(n, seed=4)
+    R, A, P = p["R"], p["A"], p["P"]
+    log(f"irregular inputs generated in {time.perf_counter() - t0:.1f}s: nnz R/A/P {R.nnz}/{A.nnz}/{P.nnz}")
+    dR, dA, dP = (lib.DeviceCSR.upload(m) for m in (R, A, P))
+    RA, _ = lib.spgemm(dR, dA)
+    nnz_ra = RA.nnz
+    mac2 = int(lib.ub(RA, dP)[-1])
+    RA.free()
+    lib.set_tuning(None)
+    out, st = lib.rap(dR, dA, dP, path=lib.DIRECT)
+    out.free()
+    lib.sync()
+    dev, wall = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out, st = lib.rap(dR, dA, dP, path=lib.DIRECT)
+        lib.sync()
+        wall.append((time.perf_counter() - t0) * 1e3)
+        dev.append(st["ms_total"])
+        nnz_out = st["nnz_out"]
+        out.free()
+    mac1 = st["mac1"]
+    for m in (dR, dA, dP):
+        m.free()
+    ms = float(np.median(dev))
+    ba = alg_bytes(R, A, P, mac1, mac2, nnz_ra, nnz_out)
+    bc = compulsory_bytes(R, A, P, nnz_out)
+    return {"workload": f"irregular_lognormal_{n}_seed4 Galerkin R*A*P, FP64, direct path (default schedule)",
+            "rows_coarse": R.M, "nnz_A": A.nnz, "nnz_RA": nnz_ra, "nnz_RAP": nnz_out, "mac1": mac1, "mac2": mac2,
+            "ms_per_product": round(ms, 1), "wall_ms": round(float(np.median(wall)), 1),
+            "gflops": round(2 * (mac1 + mac2) / (ms * 1e-3) / 1e9, 1),
+            "rows_off_table": st["guarded_entries"], "rows_spilled": st["spilled_rows"],
+            "bytes_alg": ba, "roofline_frac": round(ba / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "bytes_compulsory": bc, "roofline_frac_compulsory": round(bc / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -344,6 +405,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
     ap.add_argument("--no-consumer", action="store_true", help="skip the V-cycle / SpMV leg")
+    ap.add_argument("--no-irregular", action="store_true", help="skip the 1M irregular-rows leg")
     ap.add_argument("--schedule", default=None,
                     help="install this schedule instead of autotuning, e.g. u1=4,u2=3,subwave_h=0 "
                          "(fields of sp3mm_tuning; the others keep their defaults)")
@@ -432,7 +494,10 @@ def main():
                      "bytes_alg": bytes_alg},
         "product": {"device_ms_median": round(dev_med, 4),
                     "roofline_frac": round(bytes_alg / (dev_med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                    "gflops_device": round(flop / (dev_med * 1e-3) / 1e9, 2)},
+                    "gflops_device": round(flop / (dev_med * 1e-3) / 1e9, 2),
+                    "bytes_compulsory": compulsory_bytes(R, A, P, nnz_out),
+                    "roofline_frac_compulsory": round(compulsory_bytes(R, A, P, nnz_out) / (dev_med * 1e-3) / 1e9
+                                                      / HBM_PEAK_GBS, 4)},
         "hbm_copy_probe_GBs": round(copy_gbs, 1) if copy_gbs else None,
     }
     if path == lib.DIRECT:
@@ -450,6 +515,9 @@ def main():
     if not args.no_consumer:
         log("consumer leg (device hierarchy + V-cycle + SpMV) ...")
         rec["consumer"] = consumer_leg(lib, problems, dA, args.n)
+    if not args.no_irregular:
+        log("irregular leg (1M lognormal rows) ...")
+        rec["irregular"] = irregular_leg(lib, problems)
     if not args.no_cpu:
         log("timing the reference CPU path on a sample ...")
         rec["cpu_baseline"] = cpu_baseline(R, A, P, None)

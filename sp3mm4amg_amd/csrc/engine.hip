@@ -356,7 +356,7 @@ static int dense_blocks(const Context *c, int rows)
 // dense workspaces of the direct long-row kernels: bitmap + list over A's
 // columns, bitmap over P's columns, and (numeric) the value arrays
 struct DenseDirectWs {
-    Buf bitsA, valsA, listc, listv, bitsP, valsP;
+    Buf bitsA, valsA, listc, listv, bitsP, valsP, cur;
     DenseWs ws{};
     int alloc(int wb, int64_t ncolA, int64_t ncolP, bool num, hipStream_t s)
     {
@@ -369,8 +369,9 @@ struct DenseDirectWs {
             valsA = Buf(slotsA * 8);
             listv = Buf(slotsA * 8);
             valsP = Buf(slotsP * 8);
+            cur = Buf(slotsA * 4);
         }
-        if (!bitsA.p || !listc.p || !bitsP.p || (num && (!valsA.p || !listv.p || !valsP.p))) {
+        if (!bitsA.p || !listc.p || !bitsP.p || (num && (!valsA.p || !listv.p || !valsP.p || !cur.p))) {
             set_error("dense workspace allocation failed");
             return -1;
         }
@@ -386,6 +387,7 @@ struct DenseDirectWs {
         ws.listv = listv.as<double>();
         ws.bitsP = bitsP.as<unsigned>();
         ws.valsP = valsP.as<double>();
+        ws.cur = cur.as<int32_t>();
         ws.nwA = nwA;
         ws.nwP = nwP;
         return 0;

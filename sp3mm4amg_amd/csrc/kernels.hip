@@ -868,7 +868,7 @@ __device__ __forceinline__ void dense_expand(const int32_t *lcol, const LV *lval
 // sorted emission of the set bits (and values) of a workspace; returns the
 // count.  Entries beyond `cap` are counted but not written.
 __device__ __forceinline__ int dense_emit(unsigned *bits, double *vals, int nw, int32_t *outc, double *outv,
-                                          long long cap, long long *sh)
+                                          long long cap, long long *sh, int c0 = 0)
 {
     const int t = threadIdx.x;
     long long base = 0;
@@ -883,7 +883,7 @@ __device__ __forceinline__ int dense_emit(unsigned *bits, double *vals, int nw, 
             x &= x - 1;
             const int c = w * 32 + b;
             const bool in = pos < cap;
-            if (outc && in) outc[pos] = c;
+            if (outc && in) outc[pos] = c0 + c;
             if (vals) {
                 const double v = ld_val(vals + c);
                 vals[c] = 0.0;
@@ -1026,6 +1026,95 @@ __global__ void __launch_bounds__(kDense) k_dense_direct(SpOperand R, SpOperand 
             const int got = dense_emit(bitsP, valsP, ws.nwP, ccol + off, cval + off, crp[r + 1] - off, sh);
             if (threadIdx.x == 0 && got != crp[r + 1] - off) atomicOr(err, kErrOverflow);
         }
+    }
+}
+
+// Numeric pass of the direct long-row kernel when P's column space exceeds
+// LDS: the first stage as in k_dense_direct (its bitmap in LDS when L1), then
+// the second stage one window of `wcols` of P's columns at a time, accumulated
+// in LDS.  Every first-stage entry keeps a cursor into its (sorted) row of P,
+// so each window consumes only its slice and P's rows are read once in all;
+// each window's slice of the output row is emitted in column order straight
+// after the previous one.  The window region reuses the first stage's bitmap
+// space (clear once that stage is emitted); emission keeps it clear.
+template <typename VT, bool L1>
+__global__ void __launch_bounds__(kDense) k_dense_window(SpOperand R, SpOperand A, SpOperand P,
+                                                          const int32_t *__restrict__ rows,
+                                                          const int *__restrict__ count, DenseWs ws, int wcols,
+                                                          int slog1, const int64_t *__restrict__ crp,
+                                                          int32_t *__restrict__ ccol, double *__restrict__ cval,
+                                                          double *__restrict__ rowsum, int *__restrict__ err)
+{
+    extern __shared__ __align__(16) unsigned char dsm[];
+    __shared__ long long sh[kDense / 64];
+    __shared__ double dsh[kDense / 64];
+    const size_t b = blockIdx.x;
+    double *wv = reinterpret_cast<double *>(dsm);                        // wcols values
+    unsigned *wbits = reinterpret_cast<unsigned *>(dsm + (size_t)wcols * 8);  // wcols / 32 words
+    unsigned *bitsA = L1 ? reinterpret_cast<unsigned *>(dsm) : ws.bitsA + b * ws.nwA;
+    double *valsA = ws.valsA + b * ((size_t)ws.nwA * 32);
+    int32_t *listc = ws.listc + b * ((size_t)ws.nwA * 32);
+    double *listv = ws.listv + b * ((size_t)ws.nwA * 32);
+    int32_t *cur = ws.cur + b * ((size_t)ws.nwA * 32);
+    {
+        // clear the LDS once: the larger of the two uses of the region
+        const int nw = max(L1 ? ws.nwA : 0, wcols * 2 + wcols / 32);
+        unsigned *z = reinterpret_cast<unsigned *>(dsm);
+        for (int w = threadIdx.x; w < nw; w += kDense) z[w] = 0u;
+        __syncthreads();
+    }
+    const VT *rval = static_cast<const VT *>(R.val);
+    const VT *aval = static_cast<const VT *>(A.val);
+    const VT *pval = static_cast<const VT *>(P.val);
+    const int n = *count;
+    const int ncol = (int)P.N;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int r = rows[i];
+        long long f1 = 0;
+        dense_expand<true, VT, VT>(R.col, rval, R.rp[r], R.rp[r + 1], A.rp32, A.col, aval, slog1, bitsA, valsA, f1);
+        __threadfence();
+        __syncthreads();
+        const int n1 = dense_emit(bitsA, valsA, ws.nwA, listc, listv, (long long)ws.nwA * 32, sh);
+        for (int e = threadIdx.x; e < n1; e += kDense) cur[e] = P.rp32[listc[e]];
+        __threadfence();
+        __syncthreads();
+        const int64_t off = crp[r];
+        const long long cap = crp[r + 1] - off;
+        long long base = 0;
+        double ps = 0.0;
+        for (int w0 = 0; w0 < ncol; w0 += wcols) {
+            const int w1 = min(ncol, w0 + wcols);
+            for (int e = threadIdx.x; e < n1; e += kDense) {
+                int q = cur[e];
+                const int qe = P.rp32[listc[e] + 1];
+                const double lv = listv[e];
+                for (; q < qe; q++) {
+                    const int c = P.col[q];
+                    if (c >= w1) break;
+                    const int cw = c - w0;
+                    atomicOr(wbits + (cw >> 5), 1u << (cw & 31));
+                    const double pr = lv * (double)pval[q];
+                    atomicAdd(wv + cw, pr);
+                    ps += pr;
+                }
+                cur[e] = q;
+            }
+            __syncthreads();
+            base += dense_emit(wbits, wv, (w1 - w0 + 31) / 32, ccol + off + base, cval + off + base,
+                               cap - base, sh, w0);
+        }
+        if (rowsum) {
+            for (int o = 32; o > 0; o >>= 1) ps += __shfl_xor(ps, o);
+            if (lane() == 0) dsh[threadIdx.x / 64] = ps;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double tot = 0.0;
+                for (int w = 0; w < kDense / 64; w++) tot += dsh[w];
+                rowsum[r] = tot;
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0 && base != cap) atomicOr(err, kErrOverflow);
     }
 }
 
@@ -1492,6 +1581,31 @@ void launch_dense_direct(bool num, int blocks, const SpOperand &R, const SpOpera
     const bool l2 = lds2 <= max_lds;
     const bool l1 = (l2 ? lds2 : 0) + lds1 <= max_lds;
     const size_t lds = (l2 ? lds2 : 0) + (l1 ? lds1 : 0);
+    if (num && !l2 && ws.cur) {
+        // the second stage in LDS windows of P's columns (k_dense_window): the
+        // widest power-of-two window whose values and bitmap fit, sharing the
+        // region with the first stage's bitmap when that fits too
+        int wcols = 1 << 20;
+        while (wcols >= 1024 && (size_t)wcols * 8 + (size_t)wcols / 8 > max_lds) wcols >>= 1;
+        if (wcols >= 1024) {
+            const size_t wbytes = (size_t)wcols * 8 + (size_t)wcols / 8;
+            const bool w1 = lds1 <= max_lds;
+            const size_t wl = std::max(w1 ? lds1 : 0, wbytes);
+#define SP3MM_WIN_LAUNCH(VTT)                                                                                   \
+    do {                                                                                                         \
+        if (w1)                                                                                                  \
+            hipLaunchKernelGGL((k_dense_window<VTT, true>), dim3(blocks), dim3(kDense), wl, st, R, A, P, rows,   \
+                               count, ws, wcols, slog1, crp, ccol, cval, rowsum, err);                           \
+        else                                                                                                     \
+            hipLaunchKernelGGL((k_dense_window<VTT, false>), dim3(blocks), dim3(kDense), wl, st, R, A, P, rows,  \
+                               count, ws, wcols, slog1, crp, ccol, cval, rowsum, err);                           \
+    } while (0)
+            if (f32in) SP3MM_WIN_LAUNCH(float);
+            else SP3MM_WIN_LAUNCH(double);
+#undef SP3MM_WIN_LAUNCH
+            return;
+        }
+    }
 #define SP3MM_DENSE_ARGS R, A, P, rows, count, ws, slog1, slog2, n1out, rownnz, crp, ccol, cval, rowsum, err
 #define SP3MM_DENSE_LAUNCH(NUMV, VTT)                                                                           \
     do {                                                                                                         \

@@ -59,6 +59,7 @@ struct DenseWs {
     double *listv;
     unsigned *bitsP;
     double *valsP;
+    int32_t *cur;  // numeric pass, windowed second stage: one cursor per first-stage entry
     int nwA, nwP;
 };
 

@@ -442,7 +442,14 @@ def _irregular_sampled(n):
 
 
 def test_irregular_100k_sampled():
+    """P's ~12.5K columns fit in LDS: the dense rows' second stage runs there."""
     _irregular_sampled(100_000)
+
+
+def test_irregular_200k_sampled():
+    """P's ~25K columns exceed LDS: the dense rows' second stage runs in two
+    column windows (k_dense_window)."""
+    _irregular_sampled(200_000)
 
 
 @pytest.mark.slow
