@@ -92,15 +92,38 @@ def cpu_baseline(R, A, P, flop_per_row_fn, budget_s=20.0):
     frac = min(1.0, (1 / 64) * budget_s / max(t, 1e-3))
     Rs = sample(frac)
     t = run(Rs)
-    mac1 = int(O.ub(Rs, A)[-1])
-    RAs = O.spmm(Rs, A)
-    mac2 = int(O.ub(RAs, P)[-1])
-    flop = 2 * (mac1 + mac2)
-    return {"value": round(flop / t / 1e9, 4), "unit": "GFLOP/s", "cores": threads, "kind": kind,
-            "sample": f"rows 0..{Rs.M - 1} of R ({Rs.M / R.M:.3%}) times A times P, "
-                      f"sp3mmRowByRowPair_0 + spmmRowByRow_SymbNum_0 (IDXMAP), "
-                      f"{t:.2f} s for {flop} FLOP",
-            "seconds": round(t, 3)}
+    def flop_of(Rs):
+        mac1 = int(O.ub(Rs, A)[-1])
+        mac2 = int(O.ub(O.spmm(Rs, A), P)[-1])
+        return 2 * (mac1 + mac2)
+
+    flop = flop_of(Rs)
+    rec = {"value": round(flop / t / 1e9, 4), "unit": "GFLOP/s", "cores": threads, "kind": kind,
+           "sample": f"rows 0..{Rs.M - 1} of R ({Rs.M / R.M:.3%}) times A times P, "
+                     f"sp3mmRowByRowPair_0 + spmmRowByRow_SymbNum_0 (IDXMAP), "
+                     f"{t:.2f} s for {flop} FLOP",
+           "seconds": round(t, 3)}
+    if kind == "reference":
+        # the reference's other two forms, on smaller samples (budget / 3 each):
+        # the pair product with its default SpMM (spmmRowByRow2DBlocks, the test
+        # driver's default 20 x 2 grid) and the direct (merged) product
+        others = {}
+        for name, fn in (("pair_2dblocks", lambda S: ref.sp3mm_pair(S, A, P, spmm="spmmRowByRow2DBlocks_",
+                                                                     threads=threads, grid=(20, 2))),
+                         ("merged", lambda S: ref.sp3mm_merged(S, A, P, threads=threads))):
+            S = sample(1 / 256)
+            t0 = time.perf_counter()
+            fn(S)
+            t1 = time.perf_counter() - t0
+            S = sample(min(1.0, (1 / 256) * budget_s / 3 / max(t1, 1e-3)))
+            t0 = time.perf_counter()
+            fn(S)
+            t1 = time.perf_counter() - t0
+            f1 = flop_of(S)
+            others[name] = {"value": round(f1 / t1 / 1e9, 4), "unit": "GFLOP/s",
+                            "sample": f"rows 0..{S.M - 1} of R ({S.M / R.M:.3%}), {t1:.2f} s for {f1} FLOP"}
+        rec["other_forms"] = others
+    return rec
 
 
 def time_form(lib, dR, dA, dP, steps, warmup, **kw):
