@@ -511,7 +511,7 @@ int spgemm_impl(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &C, sp3mm_s
         SP3MM_HIP(hipMemsetAsync(bits.p, 0, (size_t)wb * nwB * 4, s));
         launch_dense_spgemm(false, wb, a, b, lists.as<int32_t>() + (int64_t)thr.nb * M, d->counts + thr.nb,
                             bits.as<unsigned>(), nullptr, nwB, sub_log(B, kDenseThreads), rown.as<int32_t>(),
-                            nullptr, nullptr, nullptr, &d->err, s);
+                            nullptr, nullptr, nullptr, &d->err, s, c->max_lds);
     }
     if (dmat_alloc(C, M, B.N, 0, SP3MM_F64)) return -1;
     // C.rp32 was sized for nnz 0; it is re-checked once the total is known
@@ -561,7 +561,7 @@ int spgemm_impl(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &C, sp3mm_s
         SP3MM_HIP(hipMemsetAsync(dvals.p, 0, (size_t)wb * nwB * 32 * 8, s));
         launch_dense_spgemm(true, wb, a, b, lists.as<int32_t>() + (int64_t)thr2.nb * M, d->counts2 + thr2.nb,
                             dbits.as<unsigned>(), dvals.as<double>(), nwB, sub_log(B, kDenseThreads), nullptr,
-                            C.rp, C.col, static_cast<double *>(C.val), &d->err, s);
+                            C.rp, C.col, static_cast<double *>(C.val), &d->err, s, c->max_lds);
     }
     const int t1 = tm ? tm->mark() : -1;
     if (read_ctl(c, d, h)) return -1;
@@ -659,7 +659,7 @@ static int spgemm_onepass(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &
         SP3MM_HIP(hipMemsetAsync(dvals.p, 0, dvals.bytes, s));
         launch_dense_spgemm(false, wb, a, b, lists.as<int32_t>() + (int64_t)thr.nb * M, d->counts + thr.nb,
                             dbits.as<unsigned>(), nullptr, nwB, sub_log(B, kDenseThreads), rown.as<int32_t>(),
-                            nullptr, nullptr, nullptr, &d->err, s);
+                            nullptr, nullptr, nullptr, &d->err, s, c->max_lds);
     }
     if (dmat_alloc(C, M, B.N, 0, SP3MM_F64)) return -1;
     launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d->nnz, s);
@@ -697,7 +697,7 @@ static int spgemm_onepass(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &
     if (nlong > 0)
         launch_dense_spgemm(true, wb, a, b, lists.as<int32_t>() + (int64_t)thr.nb * M, d->counts + thr.nb,
                             dbits.as<unsigned>(), dvals.as<double>(), nwB, sub_log(B, kDenseThreads), nullptr, C.rp,
-                            C.col, static_cast<double *>(C.val), &d->err, s);
+                            C.col, static_cast<double *>(C.val), &d->err, s, c->max_lds);
     if (read_ctl(c, d, h)) return -1;
     if (h.err) {
         set_error("SpGEMM: %s", err_text(h.err));

@@ -906,7 +906,9 @@ __device__ __forceinline__ long long block_sum_dense(long long v, long long *sh)
 }
 
 // pairwise SpMM rows that are too long for the LDS tables
-template <bool NUM>
+// LDS: B's column space fits in LDS (bitmap, and values in the numeric pass);
+// the workspace is zeroed once per workgroup and kept clear as above.
+template <bool NUM, bool LDS = false>
 __global__ void __launch_bounds__(kDense) k_dense_spgemm(SpOperand A, SpOperand B, const int32_t *__restrict__ rows,
                                                           const int *__restrict__ count, unsigned *ws_bits,
                                                           double *ws_vals, int nw, int slog,
@@ -916,8 +918,20 @@ __global__ void __launch_bounds__(kDense) k_dense_spgemm(SpOperand A, SpOperand 
                                                           int *__restrict__ err)
 {
     __shared__ long long sh[kDense / 64];
-    unsigned *bits = ws_bits + (size_t)blockIdx.x * nw;
-    double *vals = NUM ? ws_vals + (size_t)blockIdx.x * ((size_t)nw * 32) : nullptr;
+    unsigned *bits;
+    double *vals;
+    if constexpr (LDS) {
+        extern __shared__ __align__(16) unsigned char dsm[];
+        vals = NUM ? reinterpret_cast<double *>(dsm) : nullptr;
+        bits = reinterpret_cast<unsigned *>(dsm + (NUM ? (size_t)nw * 32 * 8 : 0));
+        for (int w = threadIdx.x; w < nw; w += kDense) bits[w] = 0u;
+        if constexpr (NUM)
+            for (int s = threadIdx.x; s < nw * 32; s += kDense) vals[s] = 0.0;
+        __syncthreads();
+    } else {
+        bits = ws_bits + (size_t)blockIdx.x * nw;
+        vals = NUM ? ws_vals + (size_t)blockIdx.x * ((size_t)nw * 32) : nullptr;
+    }
     const double *aval = static_cast<const double *>(A.val);
     const double *bval = static_cast<const double *>(B.val);
     const int n = *count;
@@ -1326,14 +1340,21 @@ int dense_words(int64_t ncols) { return (int)((ncols + 31) / 32) + 1; }
 
 void launch_dense_spgemm(bool num, int blocks, const SpOperand &A, const SpOperand &B, const int32_t *rows,
                          const int *count, unsigned *bits, double *vals, int nw, int slog, int32_t *rownnz,
-                         const int64_t *crp, int32_t *ccol, double *cval, int *err, hipStream_t st)
+                         const int64_t *crp, int32_t *ccol, double *cval, int *err, hipStream_t st,
+                         size_t max_lds)
 {
-    if (num)
-        hipLaunchKernelGGL(k_dense_spgemm<true>, dim3(blocks), dim3(kDense), 0, st, A, B, rows, count, bits, vals,
-                           nw, slog, rownnz, crp, ccol, cval, err);
+    const size_t lds = (size_t)nw * 4 + (num ? (size_t)nw * 32 * 8 : 0);
+    const bool in_lds = lds <= max_lds;
+#define SP3MM_DSPG_ARGS A, B, rows, count, bits, vals, nw, slog, rownnz, crp, ccol, cval, err
+    if (num && in_lds)
+        hipLaunchKernelGGL((k_dense_spgemm<true, true>), dim3(blocks), dim3(kDense), lds, st, SP3MM_DSPG_ARGS);
+    else if (num)
+        hipLaunchKernelGGL((k_dense_spgemm<true, false>), dim3(blocks), dim3(kDense), 0, st, SP3MM_DSPG_ARGS);
+    else if (in_lds)
+        hipLaunchKernelGGL((k_dense_spgemm<false, true>), dim3(blocks), dim3(kDense), lds, st, SP3MM_DSPG_ARGS);
     else
-        hipLaunchKernelGGL(k_dense_spgemm<false>, dim3(blocks), dim3(kDense), 0, st, A, B, rows, count, bits,
-                           vals, nw, slog, rownnz, crp, ccol, cval, err);
+        hipLaunchKernelGGL((k_dense_spgemm<false, false>), dim3(blocks), dim3(kDense), 0, st, SP3MM_DSPG_ARGS);
+#undef SP3MM_DSPG_ARGS
 }
 
 // ---------------------------------------------------------------------------

@@ -66,7 +66,8 @@ struct DenseWs {
 int dense_words(int64_t ncols);
 void launch_dense_spgemm(bool num, int blocks, const SpOperand &A, const SpOperand &B, const int32_t *rows,
                          const int *count, unsigned *bits, double *vals, int nw, int slog, int32_t *rownnz,
-                         const int64_t *crp, int32_t *ccol, double *cval, int *err, hipStream_t st);
+                         const int64_t *crp, int32_t *ccol, double *cval, int *err, hipStream_t st,
+                         size_t max_lds = 0);
 // long rows whose column windows fit in `lds` bytes of LDS (see k_window_direct);
 // the count pass (num = false) lists the rows that do not fit in `spill`
 int window_cap(size_t lds);

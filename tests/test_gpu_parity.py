@@ -393,6 +393,18 @@ def test_long_rows_window_and_spill():
     check(got, want, O.abs_product(R, A, P))
 
 
+def test_pair_path_long_rows():
+    """The pair form on the 40K irregular workload: R*A has rows beyond the
+    hash tables whose columns (A's 40K) exceed LDS (global dense kernel), and
+    (R*A)*P has long rows over P's ~5K columns (LDS dense kernel)."""
+    p = problems.irregular_sa(40000, seed=4)
+    R, A, P = p["R"], p["A"], p["P"]
+    want = O.sp3mm_pair(R, A, P)
+    got, st = L.rap_host(R, A, P, path=L.PAIR)
+    print(f"  pair: nnz RA {st['nnz_mid']} RAP {st['nnz_out']}")
+    check(got, want, None, bound=O.error_bound(R, A, P, want))
+
+
 def _sampled_rows(out: L.DeviceCSR, rows: np.ndarray) -> CSR:
     """Rows `rows` of a device product, copied back one segment at a time."""
     d = out.d
