@@ -225,6 +225,13 @@ int sp3mm_init(int device);
 const char *sp3mm_last_error(void);
 /* Block until all queued device work of the library has finished. */
 int sp3mm_sync(void);
+/* Queue the library's device work on the caller's HIP stream (a hipStream_t
+ * created by the caller, not the null stream; NULL restores the library's own
+ * stream).  Work already queued on the previous stream finishes first.  Calls
+ * that must read sizes back still wait for that stream; sp3mm_batch_create
+ * and sp3mm_vcycle_create capture on it. */
+int sp3mm_set_stream(void *stream);
+void *sp3mm_get_stream(void);
 
 /* Upload host CSR (int64 row offsets and columns in index base `base`,
  * values of type vtype) after validating shape, sortedness and ranges. */

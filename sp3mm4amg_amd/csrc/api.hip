@@ -31,6 +31,18 @@ int sp3mm_init(int device) { return init_device(device); }
 
 const char *sp3mm_last_error(void) { return last_error_text(); }
 
+int sp3mm_set_stream(void *stream)
+{
+    clear_error();
+    return set_stream(static_cast<hipStream_t>(stream));
+}
+
+void *sp3mm_get_stream(void)
+{
+    Context *c = ctx();
+    return c ? static_cast<void *>(c->stream) : nullptr;
+}
+
 int sp3mm_sync(void)
 {
     Context *c = ctx();

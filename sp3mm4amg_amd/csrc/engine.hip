@@ -156,7 +156,8 @@ int init_device(int dev)
         if (v >= 16384 && v < c->max_lds) c->max_lds = v;
     }
     tuning_from_env(c->tune);
-    SP3MM_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    SP3MM_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+    c->stream = c->own_stream;
     SP3MM_HIP(hipHostMalloc((void **)&c->host_small, 4096, hipHostMallocDefault));
     if (hipHostGetDevicePointer((void **)&c->host_small_dev, c->host_small, 0) != hipSuccess) {
         (void)hipGetLastError();
@@ -1447,6 +1448,16 @@ int sa_prolongator(const sp3mm_dmat &A, const int64_t *agg, int64_t nagg, double
 // for the device; results are visible to the host after a call that waits
 // (dot, copy to host, sync).  A Jacobi sweep that meets a zero diagonal entry
 // sets a sticky device error word, reported by the next waiting call.
+int set_stream(hipStream_t s)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    SP3MM_HIP(hipStreamSynchronize(c->stream));
+    c->stream = s ? s : c->own_stream;
+    return 0;
+}
+
 ConsState *cons_state(Context *c)
 {
     if (!c->cons) {

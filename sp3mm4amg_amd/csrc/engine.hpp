@@ -45,7 +45,8 @@ class Pool {
 
 struct Context {
     int device = -1;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // the stream every call is queued on
+    hipStream_t own_stream = nullptr;  // the library's own (non-blocking) stream
     int num_cu = 0;
     size_t max_lds = 65536;
     size_t lds_device = 65536;
@@ -110,6 +111,9 @@ int batch_run(void *b, bool use_graph, double *ms);
 int tuning_valid(const Tuning &t);
 int transpose(const sp3mm_dmat &A, sp3mm_dmat &T);
 int sa_prolongator(const sp3mm_dmat &A, const int64_t *agg, int64_t nagg, double omega, int mode, sp3mm_dmat &P);
+// Queue later calls on `s` (nullptr: the library's own stream), after work
+// already queued on the current stream has finished.
+int set_stream(hipStream_t s);
 int spmv(const sp3mm_dmat &A, const double *x, const double *z, double *y, double alpha, double beta);
 int axpby(int64_t n, double alpha, const double *x, double beta, double *y);
 int dot(int64_t n, const double *x, const double *y, double &out);

@@ -164,6 +164,8 @@ def lib():
         L.sp3mm_vcycle_free.argtypes = [C.c_void_p]
         L.sp3mm_vcycle_free.restype = None
         L.sp3mm_sync.restype = C.c_int
+        L.sp3mm_set_stream.argtypes = [C.c_void_p]
+        L.sp3mm_get_stream.restype = C.c_void_p
         L.sp3mm_copy_probe.argtypes = [C.c_int64, C.c_int32]
         L.sp3mm_copy_probe.restype = C.c_double
         _LIB = L
@@ -448,6 +450,16 @@ def axpby(alpha: float, x: DeviceArray | None, beta: float, y: DeviceArray) -> N
         raise ValueError("axpby: vector lengths differ")
     _check(lib().sp3mm_axpby(y.n, float(alpha), x.ptr if x is not None else None, float(beta), y.ptr),
            "sp3mm_axpby")
+
+
+def set_stream(handle: int | None) -> None:
+    """Queue the library's work on a caller's HIP stream (e.g. a torch stream's
+    cuda_stream handle); None restores the library's own stream."""
+    _check(lib().sp3mm_set_stream(handle), "sp3mm_set_stream")
+
+
+def get_stream() -> int | None:
+    return lib().sp3mm_get_stream()
 
 
 def sync() -> None:

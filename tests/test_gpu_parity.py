@@ -693,3 +693,38 @@ def test_device_import_rejects_bad_csr():
             L.DeviceCSR.import_device(n, n, 4, di, dj, dv)
         for v in (di, dj, dv):
             v.free()
+
+
+# ---------------------------------------------------------------------------
+# the caller's stream
+def test_caller_stream():
+    """Products on both paths and a batch graph queued on a caller's HIP
+    stream (created, default priority, through the HIP runtime the library
+    links), then the library's own stream restored."""
+    hip = C.CDLL("libamdhip64.so.7")
+    handle = C.c_void_p()
+    assert hip.hipStreamCreate(C.byref(handle)) == 0
+    R, A, P = triple("poisson3d32")
+    want = O.sp3mm_merged(R, A, P)
+    scale = O.abs_product(R, A, P)
+    own = L.get_stream()
+    L.set_stream(handle.value)
+    try:
+        assert L.get_stream() == handle.value
+        for path in (L.DIRECT, L.PAIR):
+            got, _ = L.rap_host(R, A, P, path=path)
+            check(got, want, scale)
+        d = [L.DeviceCSR.upload(m) for m in (R, A, P)]
+        b = L.Batch([d[0]], [d[1]], [d[2]], graph=True)
+        for _ in range(3):
+            b.run(True)
+            out = b.result(0)
+            check(out.download(), want, scale)
+            out.free()
+        b.free()
+        for m in d:
+            m.free()
+    finally:
+        L.set_stream(None)
+        hip.hipStreamDestroy(handle)
+    assert L.get_stream() == own
