@@ -367,6 +367,11 @@ int sp3mm_set_tuning(const sp3mm_tuning *t);
 int sp3mm_autotune(const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P,
                    const sp3mm_opts *opts, int32_t reps, sp3mm_tuning *best,
                    double *ms_best, double *ms_default);
+/* The same on a batch's packed product (with the batch's precision); the
+ * schedule found is installed and the batch (and its graph) rebuilt under
+ * it. */
+int sp3mm_batch_autotune(sp3mm_batch *b, int32_t reps, sp3mm_tuning *best,
+                         double *ms_best, double *ms_default);
 
 /* Device memory from the library's pool (e.g. for sp3mm_opts.rowsum), its
  * release, and blocking copies between host and device. */

@@ -401,6 +401,24 @@ int sp3mm_batch_info(sp3mm_batch *b, double *ms_pack, double *ms_build, int64_t 
 
 void sp3mm_batch_free(sp3mm_batch *b) { batch_free(b); }
 
+static void to_c(const Tuning &t, sp3mm_tuning &o);
+
+int sp3mm_batch_autotune(sp3mm_batch *b, int32_t reps, sp3mm_tuning *best, double *ms_best, double *ms_default)
+{
+    clear_error();
+    if (!b) {
+        set_error("batch: null handle");
+        return -1;
+    }
+    Tuning t{};
+    double mb = 0, md = 0;
+    if (batch_autotune(b, reps, t, mb, md)) return -1;
+    if (best) to_c(t, *best);
+    if (ms_best) *ms_best = mb;
+    if (ms_default) *ms_default = md;
+    return 0;
+}
+
 int sp3mm_transpose(const sp3mm_dmat *A, sp3mm_dmat *AT)
 {
     clear_error();

@@ -1658,6 +1658,8 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
 struct Batch {
     int count = 0;
     bool mixed = false;
+    int prec = SP3MM_PREC_FP64;
+    double guard = 1e-2;
     std::vector<int64_t> roff, coff, M;
     sp3mm_dmat Rb{}, Ab{}, Pb{};
     FusedPlan plan;
@@ -1665,6 +1667,41 @@ struct Batch {
     hipGraphExec_t exec = nullptr;
     double ms_pack = 0, ms_build = 0;
 };
+
+// builds the batch's plan under the context's current schedule and, with
+// `graph`, captures its replay into a HIP graph
+static int batch_build(Context *c, Batch *b, bool graph)
+{
+    Timer tm(c->stream, true);
+    const int e1 = tm.mark();
+    b->plan.R = b->Rb;
+    b->plan.A = b->Ab;
+    b->plan.P = b->Pb;
+    b->plan.mixed = b->prec == SP3MM_PREC_MIXED;
+    b->plan.f32 = b->mixed;
+    b->plan.tu = c->tune;
+    b->plan.guard = b->guard;
+    if (b->plan.build(c, tm)) return -1;
+    const int e2 = tm.mark();
+    (void)hipEventSynchronize(tm.ev[e2]);
+    b->ms_build = tm.ms(e1, e2);
+    if (graph) {
+        hipStream_t s = c->stream;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            return 0;  // usable without a graph
+        }
+        b->plan.record(c, s);
+        if (hipStreamEndCapture(s, &b->graph) != hipSuccess ||
+            hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            if (b->graph) (void)hipGraphDestroy(b->graph);
+            b->graph = nullptr;
+            b->exec = nullptr;
+        }
+    }
+    return 0;
+}
 
 static int pack(Context *c, int count, const sp3mm_dmat *const *ms, int vtype, sp3mm_dmat &out,
                 const std::vector<int64_t> &coloff)
@@ -1750,14 +1787,11 @@ void *batch_create(int count, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp
         return nullptr;
     }
     const int e1 = tm.mark();
-    b->plan.R = b->Rb;
-    b->plan.A = b->Ab;
-    b->plan.P = b->Pb;
-    b->plan.mixed = prec == SP3MM_PREC_MIXED;
-    b->plan.f32 = mixed;
-    b->plan.tu = c->tune;
-    b->plan.guard = guard > 0 ? guard : 1e-2;
-    if (b->plan.build(c, tm)) {
+    (void)hipEventSynchronize(tm.ev[e1]);
+    b->ms_pack = tm.ms(e0, e1);
+    b->prec = prec;
+    b->guard = guard > 0 ? guard : 1e-2;
+    if (batch_build(c, b, graph)) {
         b->plan.release_all();
         dmat_release(b->Rb);
         dmat_release(b->Ab);
@@ -1765,26 +1799,26 @@ void *batch_create(int count, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp
         delete b;
         return nullptr;
     }
-    const int e2 = tm.mark();
-    (void)hipEventSynchronize(tm.ev[e2]);
-    b->ms_pack = tm.ms(e0, e1);
-    b->ms_build = tm.ms(e1, e2);
-    if (graph) {
-        hipStream_t s = c->stream;
-        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-            set_error("batch: stream capture failed to start");
-            return b;  // usable without a graph
-        }
-        b->plan.record(c, s);
-        if (hipStreamEndCapture(s, &b->graph) != hipSuccess ||
-            hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
-            (void)hipGetLastError();
-            if (b->graph) (void)hipGraphDestroy(b->graph);
-            b->graph = nullptr;
-            b->exec = nullptr;
-        }
-    }
     return b;
+}
+
+int batch_autotune(void *h, int reps, Tuning &best, double &ms_best, double &ms_default)
+{
+    Context *c = ctx();
+    if (!c || !h) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    Batch *b = static_cast<Batch *>(h);
+    if (autotune(b->Rb, b->Ab, b->Pb, b->prec, b->guard, reps, best, ms_best, ms_default)) return -1;
+    // rebuild the plan (and the graph, if there was one) for the schedule found
+    const bool graph = b->exec != nullptr;
+    SP3MM_HIP(hipStreamSynchronize(c->stream));
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->plan.release_all();
+    b->plan = FusedPlan();
+    return batch_build(c, b, graph);
 }
 
 int batch_run(void *h, bool use_graph, double *ms)

@@ -108,6 +108,9 @@ void dmat_release(sp3mm_dmat &m);
 void *batch_create(int count, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P, int prec,
                    double guard, bool graph);
 int batch_run(void *b, bool use_graph, double *ms);
+// times candidate schedules on the batch's packed product, installs the
+// fastest and rebuilds the batch (plan and graph) with it
+int batch_autotune(void *b, int reps, Tuning &best, double &ms_best, double &ms_default);
 int tuning_valid(const Tuning &t);
 int transpose(const sp3mm_dmat &A, sp3mm_dmat &T);
 int sa_prolongator(const sp3mm_dmat &A, const int64_t *agg, int64_t nagg, double omega, int mode, sp3mm_dmat &P);

@@ -139,6 +139,7 @@ def lib():
         L.sp3mm_batch_result.argtypes = [C.c_void_p, C.c_int32, P(DMat)]
         L.sp3mm_batch_info.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_int64), P(C.c_int32)]
         L.sp3mm_batch_free.argtypes = [C.c_void_p]
+        L.sp3mm_batch_autotune.argtypes = [C.c_void_p, C.c_int32, P(Tuning), P(C.c_double), P(C.c_double)]
         L.sp3mm_batch_free.restype = None
         L.sp3mm_transpose.argtypes = [P(DMat), P(DMat)]
         L.sp3mm_sa_prolongator.argtypes = [P(DMat), C.c_void_p, C.c_int64, C.c_double, C.c_int32, P(DMat)]
@@ -347,6 +348,16 @@ class Batch:
         d = DMat()
         _check(lib().sp3mm_batch_result(self.h, i, C.byref(d)), "sp3mm_batch_result")
         return DeviceCSR(d)
+
+    def autotune(self, reps: int = 3):
+        """Tune the schedule on the packed product and rebuild the batch (and
+        its graph) under it; returns (schedule, ms best, ms of the starting
+        schedule)."""
+        t = Tuning()
+        mb, md = C.c_double(), C.c_double()
+        _check(lib().sp3mm_batch_autotune(self.h, int(reps), C.byref(t), C.byref(mb), C.byref(md)),
+               "sp3mm_batch_autotune")
+        return t.as_dict(), mb.value, md.value
 
     def info(self):
         a, b, n, g = C.c_double(), C.c_double(), C.c_int64(), C.c_int32()

@@ -552,6 +552,38 @@ def test_batch_graph_replays(prec):
             m.free()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed64"])
+def test_batch_autotune_rebuilds(prec):
+    """Tuning a batch installs a valid schedule and rebuilds its plan and
+    graph; replays under it match the oracle."""
+    probs = _batch_problems()
+    mixed = prec != "fp64"
+    vt = L.F32 if mixed else L.F64
+    rnd = (lambda m: m.with_vals(m.vals.astype(np.float32).astype(np.float64))) if mixed else (lambda m: m)
+    dev = [[L.DeviceCSR.upload(rnd(m), vtype=vt) for m in t] for t in probs]
+    wants = [O.sp3mm_pair(*[rnd(m) for m in t]) for t in probs]
+    scales = [O.abs_product(*[rnd(m) for m in t]) for t in probs]
+    b = L.Batch([d[0] for d in dev], [d[1] for d in dev], [d[2] for d in dev], graph=True,
+                precision=L.FP64 if not mixed else L.MIXED64)
+    try:
+        best, ms_best, ms_default = b.autotune(reps=2)
+        print(f"batch autotune {prec}: {best} {ms_best:.3f} ms (start {ms_default:.3f})")
+        assert best == L.get_tuning()
+        assert b.info()["graph"]
+        for rep in range(3):
+            b.run(graph=True)
+            for i, want in enumerate(wants):
+                out = b.result(i)
+                check(out.download(), want, scales[i])
+                out.free()
+    finally:
+        b.free()
+        L.set_tuning(None)
+        for t in dev:
+            for m in t:
+                m.free()
+
+
 def test_batch_rejects_changed_pattern():
     """A replay whose pattern no longer matches the plan is reported, not silently wrong."""
     R, A, P = triple("poisson2d64")
