@@ -1020,6 +1020,11 @@ struct FusedPlan {
     long long mac1 = 0;
     int slog1 = 0, slog2 = 0, dlog1 = 0, dlog2 = 0;
     Buf ctl, ub, rown, lists, retry, slotn, fix, tiles, scol, sval, binscr, spill;
+    // long rows in one pass (k_window_direct slot mode): a scratch slot of
+    // lcap entries per row of the long bin, gathered after the scan
+    Buf lslotn, lcol, lval;
+    int lcap = 0;
+    bool lslots = false;
     DenseDirectWs sws;  // global dense workspace of the spilled rows
     sp3mm_dmat C{};
     double ms_ub = 0, ms_num = 0, ms_scan = 0;
@@ -1070,11 +1075,23 @@ struct FusedPlan {
         int32_t *rn = num ? nullptr : rown.as<int32_t>();
         double *rs = num && mixed ? rowsum : nullptr;
         const int64_t cap = num ? ccap : 0;
-        if (nlong > 0)
-            launch_window_direct(num, std::min(nlong, c->num_cu), wl, r(), a(), p(),
-                                 lists.as<int32_t>() + (int64_t)thr.nb * M, d()->counts + thr.nb, dlog1,
-                                 spill.as<int32_t>(), &d()->nspill, rn, num ? C.rp : nullptr, num ? C.col : nullptr,
-                                 cv, rs, cap, &d()->err, s, f32);
+        int32_t *llist = lists.as<int32_t>() + (int64_t)thr.nb * M;
+        if (nlong > 0 && lslots) {
+            // one pass into the slots at count time, a gather in the tail
+            if (!num)
+                launch_window_direct(true, std::min(nlong, c->num_cu), wl, r(), a(), p(), llist,
+                                     d()->counts + thr.nb, dlog1, spill.as<int32_t>(), &d()->nspill,
+                                     rown.as<int32_t>(), nullptr, nullptr, nullptr, mixed ? rowsum : nullptr, 0,
+                                     &d()->err, s, f32, lslotn.as<int32_t>(), lcol.as<int32_t>(), lval.as<double>());
+            else
+                launch_gather_rows(llist, d()->counts + thr.nb, nlong, lslotn.as<int32_t>(), lcap,
+                                   lcol.as<int32_t>(), lval.as<double>(), C.rp, C.col, static_cast<double *>(C.val),
+                                   ccap, &d()->err, max_blocks(c), s);
+        } else if (nlong > 0) {
+            launch_window_direct(num, std::min(nlong, c->num_cu), wl, r(), a(), p(), llist, d()->counts + thr.nb,
+                                 dlog1, spill.as<int32_t>(), &d()->nspill, rn, num ? C.rp : nullptr,
+                                 num ? C.col : nullptr, cv, rs, cap, &d()->err, s, f32);
+        }
         if (nretry > 0 || (!num && !built) || (num && spec))
             launch_window_direct(num, c->num_cu, wl, r(), a(), p(), retry.as<int32_t>(), &d()->nretry, dlog1,
                                  spill.as<int32_t>(), &d()->nspill, rn, num ? C.rp : nullptr, num ? C.col : nullptr,
@@ -1143,6 +1160,15 @@ struct FusedPlan {
         if (!scol.p || !sval.p) {
             set_error("scratch allocation (%lld entries) failed", (long long)stot);
             return -1;
+        }
+        // long rows in one pass when their slots stay within 1 GiB
+        lcap = window_cap(c->max_lds - 1024);
+        lslots = false;
+        if (nlong > 0 && (size_t)nlong * lcap * 12 <= ((size_t)1 << 30)) {
+            lslotn = Buf((size_t)nlong * 4);
+            lcol = Buf((size_t)nlong * lcap * 4);
+            lval = Buf((size_t)nlong * lcap * 8);
+            lslots = lslotn.p && lcol.p && lval.p;
         }
         slog1 = sub_log(A, 64);
         slog2 = sub_log(P, 64);

@@ -1368,9 +1368,15 @@ void launch_dense_spgemm(bool num, int blocks, const SpOperand &A, const SpOpera
 // native ds_add_f64; stage 2 finds the window of the forwarded row in P and
 // does the same in a second LDS region.  The presence bits of stage 2, read in
 // order, are the sorted output row.  A row whose windows exceed the LDS
-// capacity `cap` (values) goes to `spill` (count pass only), for the global
-// dense kernel.  Workgroups of kWin lanes: with one row in LDS per workgroup,
-// 16 wavefronts per CU keep enough loads in flight.
+// capacity `cap` (values) goes to `spill` (count pass, or slot mode), for the
+// global dense kernel.  Workgroups of kWin lanes: with one row in LDS per
+// workgroup, 16 wavefronts per CU keep enough loads in flight.
+//
+// Slot mode (numeric pass with `scol` set): one pass instead of count +
+// numeric.  Row i of the list is emitted to its scratch slot scol/sval + i*cap
+// (a row that fits the windows has at most cap entries) and its length goes
+// to rownnz[r] and slotn[i]; k_gather_rows later moves the slots to their
+// final place.
 constexpr int kWin = 1024;
 
 __device__ __forceinline__ void block_minmax(int &lo, int &hi, int *sh)
@@ -1403,9 +1409,11 @@ __global__ void __launch_bounds__(kWin) k_window_direct(SpOperand R, SpOperand A
                                                            const int64_t *__restrict__ crp,
                                                            int32_t *__restrict__ ccol, double *__restrict__ cval,
                                                            double *__restrict__ rowsum, int64_t out_cap,
-                                                           int *__restrict__ err)
+                                                           int *__restrict__ err, int32_t *__restrict__ slotn,
+                                                           int32_t *__restrict__ scol, double *__restrict__ sval)
 {
     extern __shared__ __align__(16) unsigned char smem[];
+    const bool slots = NUM && scol != nullptr;
     double *V = reinterpret_cast<double *>(smem);                  // cap values
     unsigned *B = reinterpret_cast<unsigned *>(smem + (size_t)cap * 8);  // cap/32 words
     __shared__ int ish[2 * (kWin / 64)];
@@ -1434,9 +1442,10 @@ __global__ void __launch_bounds__(kWin) k_window_direct(SpOperand R, SpOperand A
         const int span1 = hi1 >= lo1 ? hi1 - lo1 + 1 : 0;
         const int nw1 = (span1 + 31) / 32;
         if (nw1 * 32 > cap) {
-            if (!NUM && t == 0) {
+            if ((!NUM || slots) && t == 0) {
                 spill[atomicAdd(nspill, 1)] = r;
                 rownnz[r] = 0;
+                if (slots) slotn[i] = 0;
             }
             continue;
         }
@@ -1479,9 +1488,10 @@ __global__ void __launch_bounds__(kWin) k_window_direct(SpOperand R, SpOperand A
         const int nw2 = (span2 + 31) / 32;
         const int off2 = nw1 * 32;  // region 2 starts on a word boundary
         if (off2 + nw2 * 32 > cap) {
-            if (!NUM && t == 0) {
+            if ((!NUM || slots) && t == 0) {
                 spill[atomicAdd(nspill, 1)] = r;
                 rownnz[r] = 0;
+                if (slots) slotn[i] = 0;
             }
             continue;
         }
@@ -1532,12 +1542,24 @@ __global__ void __launch_bounds__(kWin) k_window_direct(SpOperand R, SpOperand A
                     rowsum[r] = tot;
                 }
             }
-            // sorted emission of region 2
-            const int64_t off = crp[r], len = crp[r + 1] - off;
-            if (off + len > out_cap) {  // output allocated smaller than this row needs
-                if (t == 0) atomicOr(err, kErrOverflow);
-                __syncthreads();
-                continue;
+            // sorted emission of region 2 (to the output row, or to the slot)
+            int32_t *oc;
+            double *ov;
+            int64_t len;
+            if (slots) {
+                oc = scol + (size_t)i * cap;
+                ov = sval + (size_t)i * cap;
+                len = cap;
+            } else {
+                const int64_t off = crp[r];
+                len = crp[r + 1] - off;
+                if (off + len > out_cap) {  // output allocated smaller than this row needs
+                    if (t == 0) atomicOr(err, kErrOverflow);
+                    __syncthreads();
+                    continue;
+                }
+                oc = ccol + off;
+                ov = cval + off;
             }
             long long base = 0;
             for (int w0 = 0; w0 < nw2; w0 += kWin) {
@@ -1550,14 +1572,21 @@ __global__ void __launch_bounds__(kWin) k_window_direct(SpOperand R, SpOperand A
                     x &= x - 1;
                     const int j2 = w * 32 + b;
                     if (pos < len) {
-                        ccol[off + pos] = lo2 + j2;
-                        cval[off + pos] = V2[j2];
+                        oc[pos] = lo2 + j2;
+                        ov[pos] = V2[j2];
                     }
                     pos++;
                 }
                 base += total;
             }
-            if (t == 0 && base != len) atomicOr(err, kErrOverflow);
+            if (slots) {
+                if (t == 0) {
+                    rownnz[r] = (int32_t)base;
+                    slotn[i] = (int32_t)base;
+                }
+            } else if (t == 0 && base != len) {
+                atomicOr(err, kErrOverflow);
+            }
         }
         __syncthreads();
     }
@@ -1573,12 +1602,14 @@ int window_cap(size_t lds)
 void launch_window_direct(bool num, int blocks, size_t lds, const SpOperand &R, const SpOperand &A,
                           const SpOperand &P, const int32_t *rows, const int *count, int slog1, int32_t *spill,
                           int *nspill, int32_t *rownnz, const int64_t *crp, int32_t *ccol, double *cval,
-                          double *rowsum, int64_t out_cap, int *err, hipStream_t st, bool f32in)
+                          double *rowsum, int64_t out_cap, int *err, hipStream_t st, bool f32in,
+                          int32_t *slotn, int32_t *scol, double *sval)
 {
     const int cap = window_cap(lds);
     const size_t bytes = (size_t)cap * 8 + (size_t)cap / 32 * 4;
     if (slog1 > 6) slog1 = 6;
-#define SP3MM_WIN_ARGS R, A, P, rows, count, cap, slog1, spill, nspill, rownnz, crp, ccol, cval, rowsum, out_cap, err
+#define SP3MM_WIN_ARGS R, A, P, rows, count, cap, slog1, spill, nspill, rownnz, crp, ccol, cval, rowsum, out_cap, \
+                       err, slotn, scol, sval
     if (f32in) {
         if (num) hipLaunchKernelGGL((k_window_direct<true, float>), dim3(blocks), dim3(kWin), bytes, st, SP3MM_WIN_ARGS);
         else hipLaunchKernelGGL((k_window_direct<false, float>), dim3(blocks), dim3(kWin), bytes, st, SP3MM_WIN_ARGS);
@@ -1701,11 +1732,14 @@ void launch_gather_rows(const int32_t *rows, const int *count, int rows_host, co
                         const int32_t *scol, const double *sval, const int64_t *crp, int32_t *ccol, double *cval,
                         int64_t out_cap, int *err, int max_blocks, hipStream_t st)
 {
-    const int L = cap <= 128 ? 16 : 32;
+    const int L = cap <= 128 ? 16 : cap <= 4096 ? 32 : 256;
     unsigned grid = (unsigned)((rows_host + 256 / L - 1) / (256 / L));
     if (grid > (unsigned)max_blocks) grid = (unsigned)max_blocks;
     if (grid < 1) grid = 1;
-    if (L == 16)
+    if (L == 256)
+        hipLaunchKernelGGL(k_gather_rows<256>, dim3(grid), dim3(256), 0, st, rows, count, slotn, cap, scol, sval,
+                           crp, ccol, cval, out_cap, err);
+    else if (L == 16)
         hipLaunchKernelGGL(k_gather_rows<16>, dim3(grid), dim3(256), 0, st, rows, count, slotn, cap, scol, sval, crp,
                            ccol, cval, out_cap, err);
     else
