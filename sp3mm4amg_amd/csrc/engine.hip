@@ -1115,6 +1115,28 @@ struct FusedPlan {
         }
     }
 
+    // Bins of at most kMergeRows rows are folded into the next larger non-empty
+    // bin (its tables hold their upper bounds too; chains of small bins move up
+    // together): a launch of a handful of rows costs a whole row's latency,
+    // while the appended rows fill groups of the larger launch that hold one
+    // row fewer than the rest.  Updates cnt; the device lists and counts follow
+    // with launch_merge_bin.
+    std::vector<std::pair<int, int>> merges;
+    void plan_merges()
+    {
+        merges.clear();
+        for (int b = 0; b < thr.nb; b++) {
+            if (cnt[b] == 0 || cnt[b] > kMergeRows) continue;
+            int to = b + 1;
+            while (to < thr.nb && cnt[to] == 0) to++;
+            if (to >= thr.nb) continue;
+            merges.push_back({b, to});
+            cnt[to] += cnt[b];
+            cnt[b] = 0;
+        }
+    }
+    static constexpr int kMergeRows = 1024;
+
     int build(Context *c, Timer &tm)
     {
         hipStream_t s = c->stream;
@@ -1145,6 +1167,8 @@ struct FusedPlan {
         mac1 = (long long)h.totals[0];
         cnt.assign(h.counts, h.counts + thr.nb + 1);
         nlong = cnt[thr.nb];
+        plan_merges();
+        for (const auto &m : merges) launch_merge_bin(lists.as<int32_t>(), d()->counts, m.first, m.second, M, s);
         soff.assign(thr.nb + 1, 0);
         nbase.assign(thr.nb + 1, 0);
         int64_t ntot = 0;
@@ -1258,6 +1282,7 @@ struct FusedPlan {
         launch_fill_i32(reinterpret_cast<int32_t *>(d()), (int64_t)(sizeof(Ctl) / 4), 0, s);
         launch_ub_bin(r(), a(), ub.as<int64_t>(), rown.as<int32_t>(), lists.as<int32_t>(), d()->counts, thr,
                       d()->totals, binscr.as<int>(), s);
+        for (const auto &m : merges) launch_merge_bin(lists.as<int32_t>(), d()->counts, m.first, m.second, M, s);
         for (int i = 0; i < thr.nb; i++)
             if (cnt[i]) launch_main(c, s, i);
         launch_long(c, s, false);

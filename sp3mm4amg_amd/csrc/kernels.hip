@@ -1761,6 +1761,26 @@ void launch_check_nnz(const int64_t *nnz, int64_t want, const int *nretry, int w
                        err);
 }
 
+// Appends bin `from`'s row list to bin `to`'s (lists of M entries per bin) and
+// moves its count: the rows then run in `to`'s launch, whose tables are larger.
+__global__ void k_merge_bin(int32_t *__restrict__ lists, int *__restrict__ counts, int from, int to, int64_t M)
+{
+    const int n = counts[from], base = counts[to];
+    const int32_t *src = lists + (int64_t)from * M;
+    int32_t *dst = lists + (int64_t)to * M + base;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) dst[j] = src[j];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        counts[to] = base + n;
+        counts[from] = 0;
+    }
+}
+
+void launch_merge_bin(int32_t *lists, int *counts, int from, int to, int64_t M, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_merge_bin, dim3(1), dim3(1024), 0, st, lists, counts, from, to, M);
+}
+
 // ---------------------------------------------------------------------------
 // batches: block-diagonal packing of many operands, and extraction of one block
 // ---------------------------------------------------------------------------

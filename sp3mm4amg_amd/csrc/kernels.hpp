@@ -146,6 +146,8 @@ void launch_rp_shift(const int64_t *rp, int64_t n, int base, int64_t *out, hipSt
 void launch_f64_to_f32(const double *in, int64_t n, float *out, hipStream_t st);
 void launch_f32_to_f64(const float *in, int64_t n, double *out, hipStream_t st);
 void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
+// appends bin `from`'s row list to bin `to`'s and moves its count (one workgroup)
+void launch_merge_bin(int32_t *lists, int *counts, int from, int to, int64_t M, hipStream_t st);
 // setup.hip: transpose and smoothed-aggregation prolongator
 void launch_col_count(const int32_t *col, int64_t nnz, int32_t *cnt, hipStream_t st);
 void launch_transpose_scatter(const int64_t *rp, const int32_t *col, const void *val, bool f32, int64_t M,
