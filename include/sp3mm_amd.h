@@ -296,6 +296,24 @@ int sp3mm_transpose(const sp3mm_dmat *A, sp3mm_dmat *AT);
 int sp3mm_sa_prolongator(const sp3mm_dmat *A, const int64_t *agg, int64_t nagg, double omega,
                          int32_t diag_mode, sp3mm_dmat *P);
 
+/* A whole smoothed-aggregation hierarchy as one replayable plan (no
+ * counterpart in the reference, whose caller builds the hierarchy): level l
+ * takes its operator A_l and the host array aggs[l] of A_l->M aggregate ids in
+ * [0, naggs[l]), and forms P_l = (I - omega D^-1 A_l) P0_l, R_l = P_l^T and
+ * A_{l+1} = R_l A_l P_l, all in HBM.  Creation builds every level once and,
+ * with use_graph, captures the chain as one HIP graph.  sp3mm_hier_update
+ * recomputes every level from A's current values (A must keep its pattern,
+ * which is checked against a fingerprint taken at creation; its values may
+ * change in place); ms receives its device time.  sp3mm_hier_level gives
+ * views (owned = 0, valid until sp3mm_hier_free) of R_l, P_l and A_{l+1}. */
+typedef struct sp3mm_hier sp3mm_hier;
+int sp3mm_hier_create(const sp3mm_dmat *A, int32_t nlevels, const int64_t *const *aggs, const int64_t *naggs,
+                      double omega, int32_t diag_mode, int32_t use_graph, sp3mm_hier **out);
+int sp3mm_hier_update(sp3mm_hier *h, int32_t use_graph, double *ms);
+int sp3mm_hier_level(sp3mm_hier *h, int32_t level, sp3mm_dmat *R, sp3mm_dmat *P, sp3mm_dmat *Ac);
+int sp3mm_hier_info(sp3mm_hier *h, int32_t *nlevels, double *ms_build, int32_t *has_graph);
+void sp3mm_hier_free(sp3mm_hier *h);
+
 /* Consumers of a level (no counterpart in the reference, whose products feed
  * the caller's own solver), so that a V-cycle over a device hierarchy runs on
  * the GPU; FP64 operators, device vectors (x and y distinct):

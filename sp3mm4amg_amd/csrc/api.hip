@@ -442,6 +442,56 @@ int sp3mm_sa_prolongator(const sp3mm_dmat *A, const int64_t *agg, int64_t nagg, 
     return sa_prolongator(*A, agg, nagg, omega, diag_mode, *P);
 }
 
+int sp3mm_hier_create(const sp3mm_dmat *A, int32_t nlevels, const int64_t *const *aggs, const int64_t *naggs,
+                      double omega, int32_t diag_mode, int32_t use_graph, sp3mm_hier **out)
+{
+    clear_error();
+    if (!A || !aggs || !naggs || !out || nlevels <= 0) {
+        set_error("hierarchy: null argument or no levels");
+        return -1;
+    }
+    for (int32_t l = 0; l < nlevels; l++)
+        if (!aggs[l]) {
+            set_error("hierarchy: level %d has no aggregate ids", l);
+            return -1;
+        }
+    *out = static_cast<sp3mm_hier *>(hier_create(*A, nlevels, aggs, naggs, omega, diag_mode, use_graph != 0));
+    return *out ? 0 : -1;
+}
+
+int sp3mm_hier_update(sp3mm_hier *h, int32_t use_graph, double *ms)
+{
+    clear_error();
+    if (!h) {
+        set_error("hierarchy: null handle");
+        return -1;
+    }
+    return hier_update(h, use_graph != 0, ms);
+}
+
+int sp3mm_hier_level(sp3mm_hier *h, int32_t level, sp3mm_dmat *R, sp3mm_dmat *P, sp3mm_dmat *Ac)
+{
+    clear_error();
+    if (!h || !R || !P || !Ac) {
+        set_error("hierarchy: null argument");
+        return -1;
+    }
+    return hier_level(h, level, *R, *P, *Ac);
+}
+
+int sp3mm_hier_info(sp3mm_hier *h, int32_t *nlevels, double *ms_build, int32_t *has_graph)
+{
+    clear_error();
+    if (!h) {
+        set_error("hierarchy: null handle");
+        return -1;
+    }
+    hier_info(h, nlevels, ms_build, has_graph);
+    return 0;
+}
+
+void sp3mm_hier_free(sp3mm_hier *h) { hier_free(h); }
+
 int sp3mm_spmv(const sp3mm_dmat *A, const double *x, double *y, double alpha, double beta)
 {
     clear_error();

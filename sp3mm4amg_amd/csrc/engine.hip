@@ -9,6 +9,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 
 #include "device_common.hpp"
 #include "engine.hpp"
@@ -1396,32 +1397,104 @@ int rap_direct_mixed(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat 
 // --------------------------------------------------------------------------
 // setup operations: transpose, smoothed-aggregation prolongator
 // --------------------------------------------------------------------------
+// AT = A^T with buffers that persist, so that record() can enqueue the same
+// kernels again (column counts, scan, scatter, per-row sort) without host
+// synchronisation, e.g. into a HIP graph
+struct TransposePlan {
+    sp3mm_dmat A{}, T{};
+    Buf cnt, cursor, tiles, total, tcol, tval;
+
+    int build(Context *c, const sp3mm_dmat &a)
+    {
+        A = a;
+        const bool f32 = A.vtype == SP3MM_F32;
+        if (dmat_alloc(T, A.N, A.M, A.nnz, A.vtype)) return -1;
+        cnt = Buf((size_t)(A.N + 1) * 4);
+        cursor = Buf((size_t)(A.N + 1) * 4);
+        tiles = Buf((size_t)scan_tiles(A.N) * 8);
+        total = Buf(8);
+        tcol = Buf((size_t)(A.nnz ? A.nnz : 1) * 4);
+        tval = Buf((size_t)(A.nnz ? A.nnz : 1) * (f32 ? 4 : 8));
+        if (!cnt.p || !cursor.p || !tiles.p || !total.p || !tcol.p || !tval.p) {
+            set_error("transpose: scratch allocation failed");
+            dmat_release(T);
+            return -1;
+        }
+        record(c, c->stream);
+        SP3MM_HIP(hipGetLastError());
+        return 0;
+    }
+
+    void record(Context *, hipStream_t s)
+    {
+        const bool f32 = A.vtype == SP3MM_F32;
+        launch_fill_i32(cnt.as<int32_t>(), A.N + 1, 0, s);
+        launch_fill_i32(cursor.as<int32_t>(), A.N + 1, 0, s);
+        launch_col_count(A.col, A.nnz, cnt.as<int32_t>(), s);
+        launch_scan(cnt.as<int32_t>(), A.N, T.rp, T.rp32, tiles.as<long long>(), total.as<int64_t>(), s);
+        launch_transpose_scatter(A.rp, A.col, A.val, f32, A.M, T.rp, cursor.as<int32_t>(), tcol.as<int32_t>(),
+                                 tval.p, s);
+        launch_sort_rows(T.rp, T.M, tcol.as<int32_t>(), tval.p, f32, T.col, T.val, s);
+    }
+};
+
 int transpose(const sp3mm_dmat &A, sp3mm_dmat &T)
 {
     Context *c = ctx();
     if (!c) return -1;
     std::lock_guard<std::recursive_mutex> lk(c->mu);
-    hipStream_t s = c->stream;
-    const bool f32 = A.vtype == SP3MM_F32;
-    if (dmat_alloc(T, A.N, A.M, A.nnz, A.vtype)) return -1;
-    Buf cnt((size_t)(A.N + 1) * 4), cursor((size_t)(A.N + 1) * 4), tiles((size_t)scan_tiles(A.N) * 8),
-        total(8), tcol((size_t)(A.nnz ? A.nnz : 1) * 4), tval((size_t)(A.nnz ? A.nnz : 1) * (f32 ? 4 : 8));
-    if (!cnt.p || !cursor.p || !tiles.p || !total.p || !tcol.p || !tval.p) {
-        set_error("transpose: scratch allocation failed");
-        dmat_release(T);
-        return -1;
-    }
-    launch_fill_i32(cnt.as<int32_t>(), A.N + 1, 0, s);
-    launch_fill_i32(cursor.as<int32_t>(), A.N + 1, 0, s);
-    launch_col_count(A.col, A.nnz, cnt.as<int32_t>(), s);
-    launch_scan(cnt.as<int32_t>(), A.N, T.rp, T.rp32, tiles.as<long long>(), total.as<int64_t>(), s);
-    launch_transpose_scatter(A.rp, A.col, A.val, f32, A.M, T.rp, cursor.as<int32_t>(), tcol.as<int32_t>(), tval.p,
-                             s);
-    launch_sort_rows(T.rp, T.M, tcol.as<int32_t>(), tval.p, f32, T.col, T.val, s);
-    SP3MM_HIP(hipGetLastError());
-    SP3MM_HIP(hipStreamSynchronize(s));
+    TransposePlan tp;
+    if (tp.build(c, A)) return -1;
+    SP3MM_HIP(hipStreamSynchronize(c->stream));
+    T = tp.T;
     return 0;
 }
+
+// S = I - omega D^-1 A on A's pattern plus any missing diagonal, into buffers
+// sized for A->nnz + A->M entries (the most that pattern can hold), so that
+// record() can enqueue it again; build() runs it once and reads S's size and
+// the zero-diagonal flag back
+struct SaPlan {
+    sp3mm_dmat A{}, S{};
+    double omega = 0;
+    int mode = 0;
+    Buf ctl, len, tiles;
+    Ctl *d() const { return ctl.as<Ctl>(); }
+
+    int build(Context *c, const sp3mm_dmat &a, double om, int md)
+    {
+        A = a;
+        omega = om;
+        mode = md;
+        const int64_t M = A.M;
+        ctl = Buf(sizeof(Ctl));
+        len = Buf((size_t)(M + 1) * 4);
+        tiles = Buf((size_t)scan_tiles(M) * 8);
+        if (!ctl.p || !len.p || !tiles.p || A.nnz + M >= INT32_MAX) {
+            set_error("prolongator: scratch allocation failed (or A too large)");
+            return -1;
+        }
+        if (dmat_alloc(S, M, M, A.nnz + M, SP3MM_F64)) return -1;
+        record(c, c->stream);
+        Ctl h;
+        if (read_ctl(c, d(), h)) return -1;
+        if (h.err) {
+            set_error("prolongator: %s (zero diagonal)", err_text(h.err));
+            return -1;
+        }
+        S.nnz = h.nnz;
+        return 0;
+    }
+
+    void record(Context *, hipStream_t s)
+    {
+        launch_fill_i32(reinterpret_cast<int32_t *>(d()), (int64_t)(sizeof(Ctl) / 4), 0, s);
+        launch_sa_count(A.rp, A.col, A.M, len.as<int32_t>(), s);
+        launch_scan(len.as<int32_t>(), A.M, S.rp, S.rp32, tiles.as<long long>(), &d()->nnz, s);
+        launch_sa_fill(A.rp, A.col, static_cast<const double *>(A.val), A.M, omega, mode, S.rp, S.col,
+                       static_cast<double *>(S.val), &d()->err, s);
+    }
+};
 
 int sa_prolongator(const sp3mm_dmat &A, const int64_t *agg, int64_t nagg, double omega, int mode, sp3mm_dmat &P)
 {
@@ -1944,6 +2017,278 @@ void batch_free(void *h)
     dmat_release(b->Ab);
     dmat_release(b->Pb);
     delete b;
+}
+
+// --------------------------------------------------------------------------
+// a smoothed-aggregation hierarchy as one replayable plan
+// --------------------------------------------------------------------------
+// Level l, from its operator A_l: S = I - omega D^-1 A_l (SaPlan), P = I*S*P0
+// (the triple product with the identity on the left: P0 sums S's columns by
+// aggregate), R = P^T (TransposePlan), A_{l+1} = R*A_l*P (FusedPlan).  The
+// identity and P0 are fixed; every other step is re-enqueued by record(), so
+// that the whole chain is captured as one HIP graph and replayed when A_0's
+// values change (its pattern must not: a fingerprint of it is checked first).
+struct HierLevel {
+    SaPlan sa;
+    sp3mm_dmat I{}, P0{};
+    FusedPlan pplan;    // P = I * S * P0
+    TransposePlan tr;   // R = P^T
+    FusedPlan rap;      // A_{l+1} = R * A_l * P
+
+    void record(Context *c, hipStream_t s)
+    {
+        sa.record(c, s);
+        pplan.record(c, s);
+        tr.record(c, s);
+        rap.record(c, s);
+    }
+    void release_all()
+    {
+        dmat_release(sa.S);
+        dmat_release(I);
+        dmat_release(P0);
+        pplan.release_all();
+        dmat_release(tr.T);
+        rap.release_all();
+    }
+};
+
+struct Hier {
+    sp3mm_dmat A0{};
+    std::vector<std::unique_ptr<HierLevel>> lv;
+    unsigned long long fp = 0;
+    Buf fpbuf, errs, errp;  // fingerprint, OR of the plans' error words, their addresses
+    int nerr = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    double ms_build = 0;
+};
+
+static int pattern_fingerprint(Context *c, const sp3mm_dmat &A, Buf &buf, unsigned long long &out)
+{
+    launch_fingerprint(A.rp, A.M, A.col, A.nnz, buf.as<unsigned long long>(), c->stream);
+    SP3MM_HIP(hipMemcpyAsync(c->host_small, buf.p, 8, hipMemcpyDeviceToHost, c->stream));
+    SP3MM_HIP(hipStreamSynchronize(c->stream));
+    memcpy(&out, c->host_small, 8);
+    return 0;
+}
+
+// the fixed 0/1 matrix with one entry per row, in column agg[row] (device ids)
+static int one_per_row(Context *c, const int64_t *dagg, int64_t M, int64_t ncol, sp3mm_dmat &m, Buf &ctl)
+{
+    if (dmat_alloc(m, M, ncol, M, SP3MM_F64)) return -1;
+    Ctl *d = ctl.as<Ctl>();
+    launch_fill_i32(reinterpret_cast<int32_t *>(d), (int64_t)(sizeof(Ctl) / 4), 0, c->stream);
+    launch_p0_fill(dagg, M, ncol, m.rp, m.rp32, m.col, static_cast<double *>(m.val), &d->err, c->stream);
+    Ctl h;
+    if (read_ctl(c, d, h)) return -1;
+    if (h.err) {
+        set_error("hierarchy: an aggregate id is out of range");
+        return -1;
+    }
+    return 0;
+}
+
+// records a failed HIP call as the library error; true if it failed
+static bool hip_failed(hipError_t e)
+{
+    if (e == hipSuccess) return false;
+    set_error("HIP call failed: %s", hipGetErrorString(e));
+    return true;
+}
+
+static void hier_release(Hier *h)
+{
+    if (h->exec) (void)hipGraphExecDestroy(h->exec);
+    if (h->graph) (void)hipGraphDestroy(h->graph);
+    for (auto &l : h->lv) l->release_all();
+    delete h;
+}
+
+void *hier_create(const sp3mm_dmat &A, int nlev, const int64_t *const *aggs, const int64_t *naggs, double omega,
+                  int mode, bool graph)
+{
+    Context *c = ctx();
+    if (!c) return nullptr;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    if (A.M != A.N || A.vtype != SP3MM_F64 || nlev <= 0 || (mode != 0 && mode != 1) || !A.rp32) {
+        set_error("hierarchy: A must be square FP64 (nnz < 2^31), with at least one level and diag mode 0 or 1");
+        return nullptr;
+    }
+    Hier *h = new Hier();
+    h->A0 = A;
+    h->A0.owned = 0;
+    h->fpbuf = Buf(8);
+    Buf ctl(sizeof(Ctl));
+    if (!h->fpbuf.p || !ctl.p) {
+        set_error("hierarchy: allocation failed");
+        delete h;
+        return nullptr;
+    }
+    Timer tm(c->stream, true);
+    const int e0 = tm.mark();
+    if (pattern_fingerprint(c, A, h->fpbuf, h->fp)) {
+        delete h;
+        return nullptr;
+    }
+    sp3mm_dmat cur = A;
+    for (int l = 0; l < nlev; l++) {
+        const int64_t M = cur.M, nagg = naggs[l];
+        if (nagg <= 0 || nagg > M) {
+            set_error("hierarchy: level %d has %lld aggregates for %lld rows", l, (long long)nagg, (long long)M);
+            hier_release(h);
+            return nullptr;
+        }
+        h->lv.emplace_back(new HierLevel());
+        HierLevel &L = *h->lv.back();
+        std::vector<int64_t> iota((size_t)M);
+        for (int64_t i = 0; i < M; i++) iota[(size_t)i] = i;
+        Buf dagg((size_t)(M ? M : 1) * 8), diota((size_t)(M ? M : 1) * 8);
+        if (!dagg.p || !diota.p) {
+            set_error("hierarchy: allocation failed");
+            hier_release(h);
+            return nullptr;
+        }
+        if (hip_failed(hipMemcpyAsync(dagg.p, aggs[l], (size_t)M * 8, hipMemcpyHostToDevice, c->stream)) ||
+            hip_failed(hipMemcpyAsync(diota.p, iota.data(), (size_t)M * 8, hipMemcpyHostToDevice, c->stream)) ||
+            L.sa.build(c, cur, omega, mode) || one_per_row(c, dagg.as<int64_t>(), M, nagg, L.P0, ctl) ||
+            one_per_row(c, diota.as<int64_t>(), M, M, L.I, ctl)) {
+            hier_release(h);
+            return nullptr;
+        }
+        Timer t2(c->stream, false);
+        L.pplan.R = L.I;
+        L.pplan.A = L.sa.S;
+        L.pplan.P = L.P0;
+        L.pplan.tu = c->tune;
+        if (L.pplan.build(c, t2) || L.tr.build(c, L.pplan.C)) {
+            hier_release(h);
+            return nullptr;
+        }
+        L.rap.R = L.tr.T;
+        L.rap.A = cur;
+        L.rap.P = L.pplan.C;
+        L.rap.tu = c->tune;
+        if (L.rap.build(c, t2)) {
+            hier_release(h);
+            return nullptr;
+        }
+        cur = L.rap.C;
+    }
+    // the error words every replay must leave clear
+    std::vector<const int *> words;
+    for (auto &l : h->lv) {
+        words.push_back(&l->sa.d()->err);
+        words.push_back(&l->pplan.d()->err);
+        words.push_back(&l->rap.d()->err);
+    }
+    h->nerr = (int)words.size();
+    h->errp = Buf(words.size() * sizeof(int *));
+    h->errs = Buf(8);
+    if (!h->errp.p || !h->errs.p) {
+        set_error("hierarchy: allocation failed");
+        hier_release(h);
+        return nullptr;
+    }
+    const int e1 = tm.mark();
+    if (hip_failed(hipMemcpyAsync(h->errp.p, words.data(), words.size() * sizeof(int *), hipMemcpyHostToDevice,
+                                  c->stream)) ||
+        hip_failed(hipStreamSynchronize(c->stream))) {
+        hier_release(h);
+        return nullptr;
+    }
+    h->ms_build = tm.ms(e0, e1);
+    if (graph) {
+        hipStream_t s = c->stream;
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            return h;  // usable without a graph
+        }
+        for (auto &l : h->lv) l->record(c, s);
+        if (hipStreamEndCapture(s, &h->graph) != hipSuccess ||
+            hipGraphInstantiate(&h->exec, h->graph, nullptr, nullptr, 0) != hipSuccess) {
+            (void)hipGetLastError();
+            if (h->graph) (void)hipGraphDestroy(h->graph);
+            h->graph = nullptr;
+            h->exec = nullptr;
+        }
+    }
+    return h;
+}
+
+int hier_update(void *hp, bool use_graph, double *ms)
+{
+    Context *c = ctx();
+    if (!c || !hp) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    Hier *h = static_cast<Hier *>(hp);
+    if (use_graph && !h->exec) {
+        set_error("hierarchy: no graph was instantiated");
+        return -1;
+    }
+    unsigned long long fp = 0;
+    if (pattern_fingerprint(c, h->A0, h->fpbuf, fp)) return -1;
+    if (fp != h->fp) {
+        set_error("hierarchy: the pattern of A changed since the hierarchy was built (only values may change)");
+        return -1;
+    }
+    hipEvent_t e0, e1;
+    SP3MM_HIP(hipEventCreate(&e0));
+    SP3MM_HIP(hipEventCreate(&e1));
+    SP3MM_HIP(hipEventRecord(e0, c->stream));
+    if (use_graph) SP3MM_HIP(hipGraphLaunch(h->exec, c->stream));
+    else
+        for (auto &l : h->lv) l->record(c, c->stream);
+    SP3MM_HIP(hipEventRecord(e1, c->stream));
+    launch_or_words(h->errp.as<const int *>(), h->nerr, h->errs.as<int>(), c->stream);
+    SP3MM_HIP(hipMemcpyAsync(c->host_small, h->errs.p, 4, hipMemcpyDeviceToHost, c->stream));
+    SP3MM_HIP(hipStreamSynchronize(c->stream));
+    int err = 0;
+    memcpy(&err, c->host_small, 4);
+    float t = 0.f;
+    (void)hipEventElapsedTime(&t, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (ms) *ms = t;
+    if (err) {
+        set_error("hierarchy update: %s", err_text(err));
+        return -1;
+    }
+    return 0;
+}
+
+int hier_level(void *hp, int l, sp3mm_dmat &R, sp3mm_dmat &P, sp3mm_dmat &Ac)
+{
+    Hier *h = static_cast<Hier *>(hp);
+    if (!h || l < 0 || l >= (int)h->lv.size()) {
+        set_error("hierarchy: level %d out of range", l);
+        return -1;
+    }
+    HierLevel &L = *h->lv[(size_t)l];
+    R = L.tr.T;
+    P = L.pplan.C;
+    Ac = L.rap.C;
+    R.owned = P.owned = Ac.owned = 0;
+    return 0;
+}
+
+void hier_info(void *hp, int32_t *nlev, double *ms_build, int32_t *has_graph)
+{
+    Hier *h = static_cast<Hier *>(hp);
+    if (!h) return;
+    if (nlev) *nlev = (int32_t)h->lv.size();
+    if (ms_build) *ms_build = h->ms_build;
+    if (has_graph) *has_graph = h->exec ? 1 : 0;
+}
+
+void hier_free(void *hp)
+{
+    Context *c = ctx();
+    Hier *h = static_cast<Hier *>(hp);
+    if (!h || !c) return;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    (void)hipStreamSynchronize(c->stream);
+    hier_release(h);
 }
 
 // --------------------------------------------------------------------------

@@ -113,6 +113,13 @@ int batch_run(void *b, bool use_graph, double *ms);
 int batch_autotune(void *b, int reps, Tuning &best, double &ms_best, double &ms_default);
 int tuning_valid(const Tuning &t);
 int transpose(const sp3mm_dmat &A, sp3mm_dmat &T);
+// smoothed-aggregation hierarchy of A as one replayable plan (see engine.hip)
+void *hier_create(const sp3mm_dmat &A, int nlev, const int64_t *const *aggs, const int64_t *naggs, double omega,
+                  int mode, bool graph);
+int hier_update(void *h, bool use_graph, double *ms);
+int hier_level(void *h, int l, sp3mm_dmat &R, sp3mm_dmat &P, sp3mm_dmat &Ac);
+void hier_info(void *h, int32_t *nlev, double *ms_build, int32_t *has_graph);
+void hier_free(void *h);
 int sa_prolongator(const sp3mm_dmat &A, const int64_t *agg, int64_t nagg, double omega, int mode, sp3mm_dmat &P);
 // Queue later calls on `s` (nullptr: the library's own stream), after work
 // already queued on the current stream has finished.

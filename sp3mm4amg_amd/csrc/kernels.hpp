@@ -146,6 +146,11 @@ void launch_rp_shift(const int64_t *rp, int64_t n, int base, int64_t *out, hipSt
 void launch_f64_to_f32(const double *in, int64_t n, float *out, hipStream_t st);
 void launch_f32_to_f64(const float *in, int64_t n, double *out, hipStream_t st);
 void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
+// order-sensitive 64-bit fingerprint of a CSR pattern (offsets and columns) into *out
+void launch_fingerprint(const int64_t *rp, int64_t M, const int32_t *col, int64_t nnz, unsigned long long *out,
+                        hipStream_t st);
+// *out = OR of the n device words *words[i] (one wavefront)
+void launch_or_words(const int *const *words, int n, int *out, hipStream_t st);
 // appends bin `from`'s row list to bin `to`'s and moves its count (one workgroup)
 void launch_merge_bin(int32_t *lists, int *counts, int from, int to, int64_t M, hipStream_t st);
 // setup.hip: transpose and smoothed-aggregation prolongator

@@ -424,4 +424,56 @@ void launch_jacobi(const int64_t *rp, const int32_t *col, const double *val, int
 }
 #undef SP3MM_BY_LANES
 
+// ---------------------------------------------------------------------------
+// replay guards of a hierarchy plan
+// ---------------------------------------------------------------------------
+// 64-bit finaliser of the splitmix64 generator (Steele, Lea, Flood 2014)
+__device__ __forceinline__ uint64_t mix64(uint64_t x)
+{
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
+// out += sum over the M+1 offsets and nnz columns of mix64(mix64(position) +
+// entry): an order-sensitive fingerprint of a CSR pattern (wrapping sums, so
+// the result does not depend on the schedule)
+__global__ void __launch_bounds__(256) k_fingerprint(const int64_t *__restrict__ rp, int64_t M,
+                                                     const int32_t *__restrict__ col, int64_t nnz,
+                                                     unsigned long long *out)
+{
+    unsigned long long acc = 0;
+    const int64_t n = M + 1 + nnz;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t v = i <= M ? (uint64_t)rp[i] : (uint64_t)(uint32_t)col[i - M - 1];
+        acc += mix64(mix64((uint64_t)i) + v);
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane() == 0) atomicAdd(out, acc);
+}
+
+void launch_fingerprint(const int64_t *rp, int64_t M, const int32_t *col, int64_t nnz, unsigned long long *out,
+                        hipStream_t st)
+{
+    launch_fill_i32(reinterpret_cast<int32_t *>(out), 2, 0, st);
+    const unsigned g = (unsigned)std::min<int64_t>((M + 1 + nnz + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_fingerprint, dim3(g), dim3(256), 0, st, rp, M, col, nnz, out);
+}
+
+// *out = bitwise OR of the n words *words[i]
+__global__ void k_or_words(const int *const *words, int n, int *out)
+{
+    int v = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) v |= *words[i];
+    for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o);
+    if (threadIdx.x == 0) *out = v;
+}
+
+void launch_or_words(const int *const *words, int n, int *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_or_words, dim3(1), dim3(64), 0, st, words, n, out);
+}
+
 }  // namespace sp3mm

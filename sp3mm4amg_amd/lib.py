@@ -164,6 +164,13 @@ def lib():
                                          P(C.c_int32), C.c_void_p, P(C.c_double)]
         L.sp3mm_vcycle_free.argtypes = [C.c_void_p]
         L.sp3mm_vcycle_free.restype = None
+        L.sp3mm_hier_create.argtypes = [P(DMat), C.c_int32, C.c_void_p, C.c_void_p, C.c_double, C.c_int32,
+                                        C.c_int32, P(C.c_void_p)]
+        L.sp3mm_hier_update.argtypes = [C.c_void_p, C.c_int32, P(C.c_double)]
+        L.sp3mm_hier_level.argtypes = [C.c_void_p, C.c_int32, P(DMat), P(DMat), P(DMat)]
+        L.sp3mm_hier_info.argtypes = [C.c_void_p, P(C.c_int32), P(C.c_double), P(C.c_int32)]
+        L.sp3mm_hier_free.argtypes = [C.c_void_p]
+        L.sp3mm_hier_free.restype = None
         L.sp3mm_sync.restype = C.c_int
         L.sp3mm_set_stream.argtypes = [C.c_void_p]
         L.sp3mm_get_stream.restype = C.c_void_p
@@ -411,6 +418,55 @@ def sa_prolongator(A: DeviceCSR, agg: np.ndarray, nagg: int, omega: float = 2.0 
     _check(lib().sp3mm_sa_prolongator(C.byref(A.d), a.ctypes.data, int(nagg), float(omega), int(diag_mode),
                                       C.byref(d)), "sp3mm_sa_prolongator")
     return DeviceCSR(d)
+
+
+class Hierarchy:
+    """A smoothed-aggregation hierarchy of A built on the device as one
+    replayable plan (sp3mm_hier_*): level l forms P = (I - omega D^-1 A_l) P0,
+    R = P^T and A_{l+1} = R A_l P.  update() recomputes every level from A's
+    current values (same pattern), by default as one HIP graph replay; the
+    matrices of level(l) are views owned by the hierarchy."""
+
+    def __init__(self, A: DeviceCSR, aggs, omega: float = 2.0 / 3.0, diag_mode: int = 0, graph: bool = True):
+        self._aggs = [np.ascontiguousarray(a, dtype=np.int64) for a, _ in aggs]
+        self._nagg = np.array([int(n) for _, n in aggs], dtype=np.int64)
+        ptrs = (C.c_void_p * len(self._aggs))(*[a.ctypes.data for a in self._aggs])
+        self._A = A
+        h = C.c_void_p()
+        _check(lib().sp3mm_hier_create(C.byref(A.d), len(self._aggs), ptrs, self._nagg.ctypes.data, float(omega),
+                                       int(diag_mode), int(graph), C.byref(h)), "sp3mm_hier_create")
+        self.h = h
+
+    def update(self, graph: bool = True) -> float:
+        ms = C.c_double()
+        _check(lib().sp3mm_hier_update(self.h, int(graph), C.byref(ms)), "sp3mm_hier_update")
+        return ms.value
+
+    def level(self, l: int):
+        """(R_l, P_l, A_{l+1}) as views."""
+        R, P, Ac = DMat(), DMat(), DMat()
+        _check(lib().sp3mm_hier_level(self.h, int(l), C.byref(R), C.byref(P), C.byref(Ac)), "sp3mm_hier_level")
+        return DeviceCSR(R), DeviceCSR(P), DeviceCSR(Ac)
+
+    def info(self):
+        n, ms, g = C.c_int32(), C.c_double(), C.c_int32()
+        _check(lib().sp3mm_hier_info(self.h, C.byref(n), C.byref(ms), C.byref(g)), "sp3mm_hier_info")
+        return {"levels": n.value, "ms_build": ms.value, "graph": bool(g.value)}
+
+    @property
+    def coarse(self) -> DeviceCSR:
+        return self.level(len(self._aggs) - 1)[2]
+
+    def free(self) -> None:
+        if getattr(self, "h", None):
+            lib().sp3mm_hier_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def hierarchy_device(A: DeviceCSR, aggs, omega: float = 2.0 / 3.0, diag_mode: int = 0, path: int = DIRECT):
