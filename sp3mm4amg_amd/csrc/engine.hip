@@ -1397,44 +1397,49 @@ int rap_direct_mixed(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat 
 // --------------------------------------------------------------------------
 // setup operations: transpose, smoothed-aggregation prolongator
 // --------------------------------------------------------------------------
-// AT = A^T with buffers that persist, so that record() can enqueue the same
-// kernels again (column counts, scan, scatter, per-row sort) without host
-// synchronisation, e.g. into a HIP graph
+// AT = A^T.  build() forms it (column counts, scan, scatter, per-row sort);
+// with `replayable` it also keeps the permutation that maps AT's entries to
+// A's, so that record() recomputes AT's values for new values of A (same
+// pattern) with one gather, and can be enqueued without host synchronisation
+// (e.g. into a HIP graph).
 struct TransposePlan {
     sp3mm_dmat A{}, T{};
-    Buf cnt, cursor, tiles, total, tcol, tval;
+    Buf perm;
 
-    int build(Context *c, const sp3mm_dmat &a)
+    int build(Context *c, const sp3mm_dmat &a, bool replayable)
     {
         A = a;
         const bool f32 = A.vtype == SP3MM_F32;
+        hipStream_t s = c->stream;
         if (dmat_alloc(T, A.N, A.M, A.nnz, A.vtype)) return -1;
-        cnt = Buf((size_t)(A.N + 1) * 4);
-        cursor = Buf((size_t)(A.N + 1) * 4);
-        tiles = Buf((size_t)scan_tiles(A.N) * 8);
-        total = Buf(8);
-        tcol = Buf((size_t)(A.nnz ? A.nnz : 1) * 4);
-        tval = Buf((size_t)(A.nnz ? A.nnz : 1) * (f32 ? 4 : 8));
-        if (!cnt.p || !cursor.p || !tiles.p || !total.p || !tcol.p || !tval.p) {
+        const size_t nz = (size_t)(A.nnz ? A.nnz : 1);
+        Buf cnt((size_t)(A.N + 1) * 4), cursor((size_t)(A.N + 1) * 4), tiles((size_t)scan_tiles(A.N) * 8),
+            total(8), tcol(nz * 4), tval(nz * (f32 ? 4 : 8)), tsrc(replayable ? nz * 4 : 0);
+        if (replayable) perm = Buf(nz * 4);
+        if (!cnt.p || !cursor.p || !tiles.p || !total.p || !tcol.p || !tval.p ||
+            (replayable && (!tsrc.p || !perm.p))) {
             set_error("transpose: scratch allocation failed");
             dmat_release(T);
             return -1;
         }
-        record(c, c->stream);
-        SP3MM_HIP(hipGetLastError());
-        return 0;
-    }
-
-    void record(Context *, hipStream_t s)
-    {
-        const bool f32 = A.vtype == SP3MM_F32;
         launch_fill_i32(cnt.as<int32_t>(), A.N + 1, 0, s);
         launch_fill_i32(cursor.as<int32_t>(), A.N + 1, 0, s);
         launch_col_count(A.col, A.nnz, cnt.as<int32_t>(), s);
         launch_scan(cnt.as<int32_t>(), A.N, T.rp, T.rp32, tiles.as<long long>(), total.as<int64_t>(), s);
-        launch_transpose_scatter(A.rp, A.col, A.val, f32, A.M, T.rp, cursor.as<int32_t>(), tcol.as<int32_t>(),
-                                 tval.p, s);
-        launch_sort_rows(T.rp, T.M, tcol.as<int32_t>(), tval.p, f32, T.col, T.val, s);
+        launch_transpose_scatter(A.rp, A.col, A.val, f32, A.M, A.nnz, T.rp, cursor.as<int32_t>(),
+                                 tcol.as<int32_t>(), tval.p, replayable ? tsrc.as<int32_t>() : nullptr, s);
+        launch_sort_rows(T.rp, T.M, tcol.as<int32_t>(), tval.p, replayable ? tsrc.as<int32_t>() : nullptr, f32,
+                         T.col, T.val, replayable ? perm.as<int32_t>() : nullptr, s);
+        SP3MM_HIP(hipGetLastError());
+        // the scratch buffers are released at return: wait for the kernels
+        SP3MM_HIP(hipStreamSynchronize(s));
+        return 0;
+    }
+
+    // AT's values from A's current values (build() with `replayable` first)
+    void record(Context *, hipStream_t s)
+    {
+        launch_gather_vals(A.val, perm.as<int32_t>(), A.nnz, A.vtype == SP3MM_F32, T.val, s);
     }
 };
 
@@ -1444,21 +1449,20 @@ int transpose(const sp3mm_dmat &A, sp3mm_dmat &T)
     if (!c) return -1;
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     TransposePlan tp;
-    if (tp.build(c, A)) return -1;
-    SP3MM_HIP(hipStreamSynchronize(c->stream));
+    if (tp.build(c, A, false)) return -1;
     T = tp.T;
     return 0;
 }
 
-// S = I - omega D^-1 A on A's pattern plus any missing diagonal, into buffers
-// sized for A->nnz + A->M entries (the most that pattern can hold), so that
-// record() can enqueue it again; build() runs it once and reads S's size and
-// the zero-diagonal flag back
+// S = I - omega D^-1 A on A's pattern plus any missing diagonal.  build()
+// sizes it (count pass, scan) and fills it; record() refills S's columns and
+// values for new values of A (same pattern: the offsets stay), without host
+// synchronisation.  The zero-diagonal flag is in the plan's control block.
 struct SaPlan {
     sp3mm_dmat A{}, S{};
     double omega = 0;
     int mode = 0;
-    Buf ctl, len, tiles;
+    Buf ctl;
     Ctl *d() const { return ctl.as<Ctl>(); }
 
     int build(Context *c, const sp3mm_dmat &a, double om, int md)
@@ -1467,15 +1471,18 @@ struct SaPlan {
         omega = om;
         mode = md;
         const int64_t M = A.M;
+        hipStream_t s = c->stream;
         ctl = Buf(sizeof(Ctl));
-        len = Buf((size_t)(M + 1) * 4);
-        tiles = Buf((size_t)scan_tiles(M) * 8);
+        Buf len((size_t)(M + 1) * 4), tiles((size_t)scan_tiles(M) * 8);
         if (!ctl.p || !len.p || !tiles.p || A.nnz + M >= INT32_MAX) {
             set_error("prolongator: scratch allocation failed (or A too large)");
             return -1;
         }
         if (dmat_alloc(S, M, M, A.nnz + M, SP3MM_F64)) return -1;
-        record(c, c->stream);
+        launch_fill_i32(reinterpret_cast<int32_t *>(d()), (int64_t)(sizeof(Ctl) / 4), 0, s);
+        launch_sa_count(A.rp, A.col, M, A.nnz, len.as<int32_t>(), s);
+        launch_scan(len.as<int32_t>(), M, S.rp, S.rp32, tiles.as<long long>(), &d()->nnz, s);
+        record(c, s);
         Ctl h;
         if (read_ctl(c, d(), h)) return -1;
         if (h.err) {
@@ -1488,10 +1495,8 @@ struct SaPlan {
 
     void record(Context *, hipStream_t s)
     {
-        launch_fill_i32(reinterpret_cast<int32_t *>(d()), (int64_t)(sizeof(Ctl) / 4), 0, s);
-        launch_sa_count(A.rp, A.col, A.M, len.as<int32_t>(), s);
-        launch_scan(len.as<int32_t>(), A.M, S.rp, S.rp32, tiles.as<long long>(), &d()->nnz, s);
-        launch_sa_fill(A.rp, A.col, static_cast<const double *>(A.val), A.M, omega, mode, S.rp, S.col,
+        launch_fill_i32(&d()->err, 1, 0, s);
+        launch_sa_fill(A.rp, A.col, static_cast<const double *>(A.val), A.M, A.nnz, omega, mode, S.rp, S.col,
                        static_cast<double *>(S.val), &d()->err, s);
     }
 };
@@ -1526,7 +1531,7 @@ int sa_prolongator(const sp3mm_dmat &A, const int64_t *agg, int64_t nagg, double
     if (M) SP3MM_HIP(hipMemcpyAsync(dagg.p, agg, (size_t)M * 8, hipMemcpyHostToDevice, s));
     // S = I - omega D^-1 A (pattern of A plus any missing diagonal)
     sp3mm_dmat S{}, P0{};
-    launch_sa_count(A.rp, A.col, M, len.as<int32_t>(), s);
+    launch_sa_count(A.rp, A.col, M, A.nnz, len.as<int32_t>(), s);
     if (dmat_alloc(S, M, M, 0, SP3MM_F64)) return -1;
     launch_scan(len.as<int32_t>(), M, S.rp, S.rp32, tiles.as<long long>(), &d->nnz, s);
     if (read_ctl(c, d, h)) {
@@ -1543,7 +1548,7 @@ int sa_prolongator(const sp3mm_dmat &A, const int64_t *agg, int64_t nagg, double
         dmat_release(S);
         return -1;
     }
-    launch_sa_fill(A.rp, A.col, static_cast<const double *>(A.val), M, omega, mode, S.rp, S.col,
+    launch_sa_fill(A.rp, A.col, static_cast<const double *>(A.val), M, A.nnz, omega, mode, S.rp, S.col,
                    static_cast<double *>(S.val), &d->err, s);
     // P0: one 1.0 per row at its aggregate
     if (dmat_alloc(P0, M, nagg, M, SP3MM_F64)) {
@@ -2161,7 +2166,7 @@ void *hier_create(const sp3mm_dmat &A, int nlev, const int64_t *const *aggs, con
         L.pplan.A = L.sa.S;
         L.pplan.P = L.P0;
         L.pplan.tu = c->tune;
-        if (L.pplan.build(c, t2) || L.tr.build(c, L.pplan.C)) {
+        if (L.pplan.build(c, t2) || L.tr.build(c, L.pplan.C, true)) {
             hier_release(h);
             return nullptr;
         }

@@ -1453,21 +1453,36 @@ __global__ void __launch_bounds__(kWin) k_window_direct(SpOperand R, SpOperand A
         if constexpr (NUM)
             for (int j = t; j < span1; j += kWin) V[j] = 0.0;
         __syncthreads();
-        // stage 1: sub-groups of S lanes per R entry walk its A row
+        // stage 1: sub-groups of S lanes per R entry walk its A row, four
+        // entries per lane at a time (their loads in flight together)
         for (int64_t e = e0 + (t >> slog1); e < e1; e += (kWin >> slog1)) {
             const int k = R.col[e];
             double rv = 0.0;
             if constexpr (NUM) rv = (double)rval[e];
             const int a1 = A.rp32[k + 1];
-            for (int q = A.rp32[k] + (t & (S - 1)); q < a1; q += S) {
-                const int j = A.col[q] - lo1;
-                if ((unsigned)j >= (unsigned)span1) {  // an unsorted A row
-                    atomicOr(err, kErrBadInput);
-                    continue;
+            for (int q0 = A.rp32[k] + (t & (S - 1)); q0 < a1; q0 += 4 * S) {
+                int jj[4];
+                double av[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const int q = q0 + u * S;
+                    const bool ok = q < a1;
+                    jj[u] = ok ? A.col[q] - lo1 : 0;
+                    av[u] = 0.0;
+                    if constexpr (NUM) av[u] = ok ? (double)aval[q] : 0.0;
                 }
-                const unsigned bit = 1u << (j & 31);
-                if (!(B[j >> 5] & bit)) atomicOr(B + (j >> 5), bit);  // set bits are never cleared here
-                if constexpr (NUM) atomicAdd(V + j, rv * (double)aval[q]);
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    if (q0 + u * S >= a1) break;
+                    const int j = jj[u];
+                    if ((unsigned)j >= (unsigned)span1) {  // an unsorted A row
+                        atomicOr(err, kErrBadInput);
+                        continue;
+                    }
+                    const unsigned bit = 1u << (j & 31);
+                    if (!(B[j >> 5] & bit)) atomicOr(B + (j >> 5), bit);  // set bits are never cleared here
+                    if constexpr (NUM) atomicAdd(V + j, rv * av[u]);
+                }
             }
         }
         __syncthreads();

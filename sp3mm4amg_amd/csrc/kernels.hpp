@@ -156,12 +156,15 @@ void launch_merge_bin(int32_t *lists, int *counts, int from, int to, int64_t M, 
 // setup.hip: transpose and smoothed-aggregation prolongator
 void launch_col_count(const int32_t *col, int64_t nnz, int32_t *cnt, hipStream_t st);
 void launch_transpose_scatter(const int64_t *rp, const int32_t *col, const void *val, bool f32, int64_t M,
-                              const int64_t *trp, int32_t *cursor, int32_t *tcol, void *tval, hipStream_t st);
-void launch_sort_rows(const int64_t *rp, int64_t M, const int32_t *icol, const void *ival, bool f32, int32_t *ocol,
-                      void *oval, hipStream_t st);
-void launch_sa_count(const int64_t *rp, const int32_t *col, int64_t M, int32_t *len, hipStream_t st);
-void launch_sa_fill(const int64_t *rp, const int32_t *col, const double *val, int64_t M, double omega, int mode,
-                    const int64_t *srp, int32_t *scol, double *sval, int *err, hipStream_t st);
+                              int64_t nnz, const int64_t *trp, int32_t *cursor, int32_t *tcol, void *tval,
+                              int32_t *tsrc, hipStream_t st);
+void launch_sort_rows(const int64_t *rp, int64_t M, const int32_t *icol, const void *ival, const int32_t *isrc,
+                      bool f32, int32_t *ocol, void *oval, int32_t *osrc, hipStream_t st);
+// out[j] = src[perm[j]] for n values (FP32 or FP64)
+void launch_gather_vals(const void *src, const int32_t *perm, int64_t n, bool f32, void *out, hipStream_t st);
+void launch_sa_count(const int64_t *rp, const int32_t *col, int64_t M, int64_t nnz, int32_t *len, hipStream_t st);
+void launch_sa_fill(const int64_t *rp, const int32_t *col, const double *val, int64_t M, int64_t nnz, double omega,
+                    int mode, const int64_t *srp, int32_t *scol, double *sval, int *err, hipStream_t st);
 void launch_p0_fill(const int64_t *agg, int64_t M, int64_t nagg, int64_t *rp, int32_t *rp32, int32_t *col,
                     double *val, int *err, hipStream_t st);
 void launch_spmv(const int64_t *rp, const int32_t *col, const double *val, int64_t M, int64_t nnz, const double *x,

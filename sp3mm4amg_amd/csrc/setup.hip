@@ -10,6 +10,32 @@
 
 namespace sp3mm {
 
+// lanes per row for row-parallel kernels: a power of two near the average row
+// length / SP3MM_SPMV_ROW_DIV (default 4), at most a wavefront
+static int lanes_for(int64_t M, int64_t nnz)
+{
+    static const double div = [] {
+        const char *e = getenv("SP3MM_SPMV_ROW_DIV");
+        const double v = e ? atof(e) : 4.0;
+        return v >= 1.0 && v <= 64.0 ? v : 4.0;
+    }();
+    const double avg = M ? (double)nnz / (double)M / div : 1.0;
+    int L = 1;
+    while (L < 64 && L < avg) L *= 2;
+    return L;
+}
+
+#define SP3MM_BY_LANES(K, ...)                                                                   \
+    switch (L) {                                                                                 \
+    case 1: hipLaunchKernelGGL(K<1>, __VA_ARGS__); break;                                        \
+    case 2: hipLaunchKernelGGL(K<2>, __VA_ARGS__); break;                                        \
+    case 4: hipLaunchKernelGGL(K<4>, __VA_ARGS__); break;                                        \
+    case 8: hipLaunchKernelGGL(K<8>, __VA_ARGS__); break;                                        \
+    case 16: hipLaunchKernelGGL(K<16>, __VA_ARGS__); break;                                      \
+    case 32: hipLaunchKernelGGL(K<32>, __VA_ARGS__); break;                                      \
+    default: hipLaunchKernelGGL(K<64>, __VA_ARGS__); break;                                      \
+    }
+
 // ---------------------------------------------------------------------------
 // transpose: per-column counts, scan (the caller), scatter, per-row sort
 // ---------------------------------------------------------------------------
@@ -19,33 +45,40 @@ __global__ void k_col_count(const int32_t *__restrict__ col, int64_t nnz, int32_
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < nnz; e += stride) atomicAdd(cnt + col[e], 1);
 }
 
-// one thread per row of A: its entries go to the rows of A^T named by their
-// columns, at the next free position of that row (order fixed afterwards)
-template <typename VT>
-__global__ void k_transpose_scatter(const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
-                                    const VT *__restrict__ val, int64_t M, const int64_t *__restrict__ trp,
-                                    int32_t *__restrict__ cursor, int32_t *__restrict__ tcol, VT *__restrict__ tval)
+// L lanes per row of A: its entries go to the rows of A^T named by their
+// columns, at the next free position of that row (order fixed afterwards);
+// with tsrc, each entry's index in A goes along (the transpose's permutation)
+template <typename VT, int L>
+__global__ void __launch_bounds__(256) k_transpose_scatter(const int64_t *__restrict__ rp,
+                                                            const int32_t *__restrict__ col,
+                                                            const VT *__restrict__ val, int64_t M,
+                                                            const int64_t *__restrict__ trp,
+                                                            int32_t *__restrict__ cursor, int32_t *__restrict__ tcol,
+                                                            VT *__restrict__ tval, int32_t *__restrict__ tsrc)
 {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < M; r += stride) {
-        for (int64_t e = rp[r]; e < rp[r + 1]; e++) {
+    const int sub = threadIdx.x & (L - 1);
+    const int64_t stride = (int64_t)gridDim.x * (blockDim.x / L);
+    for (int64_t r = (int64_t)blockIdx.x * (blockDim.x / L) + threadIdx.x / L; r < M; r += stride) {
+        for (int64_t e = rp[r] + sub; e < rp[r + 1]; e += L) {
             const int c = col[e];
             const int64_t pos = trp[c] + atomicAdd(cursor + c, 1);
             tcol[pos] = (int32_t)r;
             tval[pos] = val[e];
+            if (tsrc) tsrc[pos] = (int32_t)e;
         }
     }
 }
 
 // one wavefront per row of A^T: rank sort of its (distinct) column indices;
 // keys of rows up to kSortLds entries are staged in LDS, longer rows are
-// ranked from global memory
+// ranked from global memory; isrc/osrc (optional) move with the values
 constexpr int kSortLds = 1024;
 
 template <typename VT>
 __global__ void __launch_bounds__(256) k_sort_rows(const int64_t *__restrict__ rp, int64_t M,
                                                     const int32_t *__restrict__ icol, const VT *__restrict__ ival,
-                                                    int32_t *__restrict__ ocol, VT *__restrict__ oval)
+                                                    const int32_t *__restrict__ isrc, int32_t *__restrict__ ocol,
+                                                    VT *__restrict__ oval, int32_t *__restrict__ osrc)
 {
     __shared__ int keys[4][kSortLds];
     const int w = threadIdx.x / kWave;
@@ -65,62 +98,96 @@ __global__ void __launch_bounds__(256) k_sort_rows(const int64_t *__restrict__ r
             for (int i = 0; i < n; i++) rank += src[i] < k;
             ocol[b + rank] = k;
             oval[b + rank] = ival[b + j];
+            if (osrc) osrc[b + rank] = isrc[b + j];
         }
         wave_sync();
     }
 }
 
+// out[j] = src[perm[j]]: the values of a transpose whose pattern is known
+template <typename VT>
+__global__ void __launch_bounds__(256) k_gather_vals(const VT *__restrict__ src, const int32_t *__restrict__ perm,
+                                                      int64_t n, VT *__restrict__ out)
+{
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x)
+        out[j] = src[perm[j]];
+}
+
 // ---------------------------------------------------------------------------
 // S = I - omega D^-1 A on the pattern of A (a missing diagonal is inserted)
 // ---------------------------------------------------------------------------
-// d_i = a_ii (mode 0) or |a_ii| + 1 (mode 1).  Count pass: row lengths of S.
-__global__ void k_sa_count(const int64_t *__restrict__ rp, const int32_t *__restrict__ col, int64_t M,
-                           int32_t *__restrict__ len)
+// d_i = a_ii (mode 0) or |a_ii| + 1 (mode 1).  L lanes per row; row r's
+// diagonal value, whether it is stored and how many of its columns are
+// smaller, summed over the lanes.
+template <int L>
+__device__ __forceinline__ void sa_row_scan(const int32_t *__restrict__ col, const double *__restrict__ val,
+                                            int64_t b, int64_t e1, int64_t r, int sub, double &d, int &nless,
+                                            int &has)
 {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < M; r += stride) {
-        bool diag = false;
-        for (int64_t e = rp[r]; e < rp[r + 1]; e++) diag |= col[e] == r;
-        len[r] = (int32_t)(rp[r + 1] - rp[r]) + (diag ? 0 : 1);
+    d = 0.0;
+    nless = 0;
+    has = 0;
+    for (int64_t e = b + sub; e < e1; e += L) {
+        const int c = col[e];
+        if (c == r) {
+            if (val) d = val[e];
+            has = 1;
+        }
+        nless += c < r;
+    }
+    for (int o = L / 2; o > 0; o >>= 1) {
+        d += __shfl_xor(d, o, L);
+        nless += __shfl_xor(nless, o, L);
+        has |= __shfl_xor(has, o, L);
     }
 }
 
-__global__ void k_sa_fill(const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
-                          const double *__restrict__ val, int64_t M, double omega, int mode,
-                          const int64_t *__restrict__ srp, int32_t *__restrict__ scol, double *__restrict__ sval,
-                          int *__restrict__ err)
+// count pass: row lengths of S
+template <int L>
+__global__ void __launch_bounds__(256) k_sa_count(const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                   int64_t M, int32_t *__restrict__ len)
 {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < M; r += stride) {
-        double d = 0.0;
-        for (int64_t e = rp[r]; e < rp[r + 1]; e++)
-            if (col[e] == r) d = val[e];
+    const int sub = threadIdx.x & (L - 1);
+    const int64_t stride = (int64_t)gridDim.x * (blockDim.x / L);
+    for (int64_t r = (int64_t)blockIdx.x * (blockDim.x / L) + threadIdx.x / L; r < M; r += stride) {
+        double d;
+        int nless, has;
+        sa_row_scan<L>(col, nullptr, rp[r], rp[r + 1], r, sub, d, nless, has);
+        if (sub == 0) len[r] = (int32_t)(rp[r + 1] - rp[r]) + (has ? 0 : 1);
+    }
+}
+
+// fill pass: S's columns and values at the offsets srp of the count pass (the
+// missing diagonal goes before the first larger column)
+template <int L>
+__global__ void __launch_bounds__(256) k_sa_fill(const int64_t *__restrict__ rp, const int32_t *__restrict__ col,
+                                                  const double *__restrict__ val, int64_t M, double omega, int mode,
+                                                  const int64_t *__restrict__ srp, int32_t *__restrict__ scol,
+                                                  double *__restrict__ sval, int *__restrict__ err)
+{
+    const int sub = threadIdx.x & (L - 1);
+    const int64_t stride = (int64_t)gridDim.x * (blockDim.x / L);
+    for (int64_t r = (int64_t)blockIdx.x * (blockDim.x / L) + threadIdx.x / L; r < M; r += stride) {
+        const int64_t b = rp[r], e1 = rp[r + 1];
+        double d;
+        int nless, has;
+        sa_row_scan<L>(col, val, b, e1, r, sub, d, nless, has);
         if (mode == 1) d = fabs(d) + 1.0;
         if (d == 0.0) {
-            atomicOr(err, kErrBadInput);
+            if (sub == 0) atomicOr(err, kErrBadInput);
             continue;
         }
         const double w = omega / d;
-        int64_t o = srp[r];
-        bool placed = false;
-        for (int64_t e = rp[r]; e < rp[r + 1]; e++) {
+        const int64_t o0 = srp[r];
+        for (int64_t e = b + sub; e < e1; e += L) {
             const int c = col[e];
-            if (!placed && c > r) {  // the missing diagonal goes before the first larger column
-                scol[o] = (int32_t)r;
-                sval[o++] = 1.0;
-                placed = true;
-            }
-            double s = -w * val[e];
-            if (c == r) {
-                s += 1.0;
-                placed = true;
-            }
-            scol[o] = c;
-            sval[o++] = s;
+            const int64_t pos = o0 + (e - b) + (!has && c > r ? 1 : 0);
+            scol[pos] = c;
+            sval[pos] = -w * val[e] + (c == r ? 1.0 : 0.0);
         }
-        if (!placed) {
-            scol[o] = (int32_t)r;
-            sval[o++] = 1.0;
+        if (!has && sub == 0) {
+            scol[o0 + nless] = (int32_t)r;
+            sval[o0 + nless] = 1.0;
         }
     }
 }
@@ -156,41 +223,77 @@ void launch_col_count(const int32_t *col, int64_t nnz, int32_t *cnt, hipStream_t
 }
 
 void launch_transpose_scatter(const int64_t *rp, const int32_t *col, const void *val, bool f32, int64_t M,
-                              const int64_t *trp, int32_t *cursor, int32_t *tcol, void *tval, hipStream_t st)
+                              int64_t nnz, const int64_t *trp, int32_t *cursor, int32_t *tcol, void *tval,
+                              int32_t *tsrc, hipStream_t st)
 {
     if (M == 0) return;
-    if (f32)
-        hipLaunchKernelGGL(k_transpose_scatter<float>, dim3(grid1(M)), dim3(256), 0, st, rp, col,
-                           static_cast<const float *>(val), M, trp, cursor, tcol, static_cast<float *>(tval));
-    else
-        hipLaunchKernelGGL(k_transpose_scatter<double>, dim3(grid1(M)), dim3(256), 0, st, rp, col,
-                           static_cast<const double *>(val), M, trp, cursor, tcol, static_cast<double *>(tval));
+    const int L = lanes_for(M, nnz);
+    const unsigned g = grid1(M, 256 / L);
+    if (f32) {
+        const float *v = static_cast<const float *>(val);
+        float *tv = static_cast<float *>(tval);
+        switch (L) {
+        case 1: hipLaunchKernelGGL((k_transpose_scatter<float, 1>), dim3(g), dim3(256), 0, st, rp, col, v, M, trp, cursor, tcol, tv, tsrc); break;
+        case 2: hipLaunchKernelGGL((k_transpose_scatter<float, 2>), dim3(g), dim3(256), 0, st, rp, col, v, M, trp, cursor, tcol, tv, tsrc); break;
+        case 4: hipLaunchKernelGGL((k_transpose_scatter<float, 4>), dim3(g), dim3(256), 0, st, rp, col, v, M, trp, cursor, tcol, tv, tsrc); break;
+        case 8: hipLaunchKernelGGL((k_transpose_scatter<float, 8>), dim3(g), dim3(256), 0, st, rp, col, v, M, trp, cursor, tcol, tv, tsrc); break;
+        case 16: hipLaunchKernelGGL((k_transpose_scatter<float, 16>), dim3(g), dim3(256), 0, st, rp, col, v, M, trp, cursor, tcol, tv, tsrc); break;
+        case 32: hipLaunchKernelGGL((k_transpose_scatter<float, 32>), dim3(g), dim3(256), 0, st, rp, col, v, M, trp, cursor, tcol, tv, tsrc); break;
+        default: hipLaunchKernelGGL((k_transpose_scatter<float, 64>), dim3(g), dim3(256), 0, st, rp, col, v, M, trp, cursor, tcol, tv, tsrc); break;
+        }
+    } else {
+        const double *v = static_cast<const double *>(val);
+        double *tv = static_cast<double *>(tval);
+        switch (L) {
+        case 1: hipLaunchKernelGGL((k_transpose_scatter<double, 1>), dim3(g), dim3(256), 0, st, rp, col, v, M, trp, cursor, tcol, tv, tsrc); break;
+        case 2: hipLaunchKernelGGL((k_transpose_scatter<double, 2>), dim3(g), dim3(256), 0, st, rp, col, v, M, trp, cursor, tcol, tv, tsrc); break;
+        case 4: hipLaunchKernelGGL((k_transpose_scatter<double, 4>), dim3(g), dim3(256), 0, st, rp, col, v, M, trp, cursor, tcol, tv, tsrc); break;
+        case 8: hipLaunchKernelGGL((k_transpose_scatter<double, 8>), dim3(g), dim3(256), 0, st, rp, col, v, M, trp, cursor, tcol, tv, tsrc); break;
+        case 16: hipLaunchKernelGGL((k_transpose_scatter<double, 16>), dim3(g), dim3(256), 0, st, rp, col, v, M, trp, cursor, tcol, tv, tsrc); break;
+        case 32: hipLaunchKernelGGL((k_transpose_scatter<double, 32>), dim3(g), dim3(256), 0, st, rp, col, v, M, trp, cursor, tcol, tv, tsrc); break;
+        default: hipLaunchKernelGGL((k_transpose_scatter<double, 64>), dim3(g), dim3(256), 0, st, rp, col, v, M, trp, cursor, tcol, tv, tsrc); break;
+        }
+    }
 }
 
-void launch_sort_rows(const int64_t *rp, int64_t M, const int32_t *icol, const void *ival, bool f32, int32_t *ocol,
-                      void *oval, hipStream_t st)
+void launch_sort_rows(const int64_t *rp, int64_t M, const int32_t *icol, const void *ival, const int32_t *isrc,
+                      bool f32, int32_t *ocol, void *oval, int32_t *osrc, hipStream_t st)
 {
     if (M == 0) return;
     const unsigned g = grid1(M, 4);
     if (f32)
         hipLaunchKernelGGL(k_sort_rows<float>, dim3(g), dim3(256), 0, st, rp, M, icol,
-                           static_cast<const float *>(ival), ocol, static_cast<float *>(oval));
+                           static_cast<const float *>(ival), isrc, ocol, static_cast<float *>(oval), osrc);
     else
         hipLaunchKernelGGL(k_sort_rows<double>, dim3(g), dim3(256), 0, st, rp, M, icol,
-                           static_cast<const double *>(ival), ocol, static_cast<double *>(oval));
+                           static_cast<const double *>(ival), isrc, ocol, static_cast<double *>(oval), osrc);
 }
 
-void launch_sa_count(const int64_t *rp, const int32_t *col, int64_t M, int32_t *len, hipStream_t st)
+void launch_gather_vals(const void *src, const int32_t *perm, int64_t n, bool f32, void *out, hipStream_t st)
 {
-    if (M > 0) hipLaunchKernelGGL(k_sa_count, dim3(grid1(M)), dim3(256), 0, st, rp, col, M, len);
+    if (n == 0) return;
+    if (f32)
+        hipLaunchKernelGGL(k_gather_vals<float>, dim3(grid1(n)), dim3(256), 0, st, static_cast<const float *>(src),
+                           perm, n, static_cast<float *>(out));
+    else
+        hipLaunchKernelGGL(k_gather_vals<double>, dim3(grid1(n)), dim3(256), 0, st,
+                           static_cast<const double *>(src), perm, n, static_cast<double *>(out));
 }
 
-void launch_sa_fill(const int64_t *rp, const int32_t *col, const double *val, int64_t M, double omega, int mode,
-                    const int64_t *srp, int32_t *scol, double *sval, int *err, hipStream_t st)
+void launch_sa_count(const int64_t *rp, const int32_t *col, int64_t M, int64_t nnz, int32_t *len, hipStream_t st)
 {
-    if (M > 0)
-        hipLaunchKernelGGL(k_sa_fill, dim3(grid1(M)), dim3(256), 0, st, rp, col, val, M, omega, mode, srp, scol,
-                           sval, err);
+    if (M == 0) return;
+    const int L = lanes_for(M, nnz);
+    SP3MM_BY_LANES(k_sa_count, dim3(grid1(M, 256 / L)), dim3(256), 0, st, rp, col, M, len)
+}
+
+void launch_sa_fill(const int64_t *rp, const int32_t *col, const double *val, int64_t M, int64_t nnz, double omega,
+                    int mode, const int64_t *srp, int32_t *scol, double *sval, int *err, hipStream_t st)
+{
+    if (M == 0) return;
+    const int L = lanes_for(M, nnz);
+    SP3MM_BY_LANES(k_sa_fill, dim3(grid1(M, 256 / L)), dim3(256), 0, st, rp, col, val, M, omega, mode, srp, scol,
+                   sval, err)
 }
 
 void launch_p0_fill(const int64_t *agg, int64_t M, int64_t nagg, int64_t *rp, int32_t *rp32, int32_t *col,
@@ -354,30 +457,6 @@ static size_t jacobi_small_lds(int64_t M, int64_t nnz) { return (size_t)nnz * 12
 // Lanes per row: the smallest power of two (at most 64) not below the mean
 // row length divided by SP3MM_SPMV_ROW_DIV (default 4: about four entries
 // per lane, measured best on SA hierarchies of 7-point operators).
-static int lanes_for(int64_t M, int64_t nnz)
-{
-    static const double div = [] {
-        const char *e = getenv("SP3MM_SPMV_ROW_DIV");
-        const double v = e ? atof(e) : 4.0;
-        return v >= 1.0 && v <= 64.0 ? v : 4.0;
-    }();
-    const double avg = M ? (double)nnz / (double)M / div : 1.0;
-    int L = 1;
-    while (L < 64 && L < avg) L *= 2;
-    return L;
-}
-
-#define SP3MM_BY_LANES(K, ...)                                                                   \
-    switch (L) {                                                                                 \
-    case 1: hipLaunchKernelGGL(K<1>, __VA_ARGS__); break;                                        \
-    case 2: hipLaunchKernelGGL(K<2>, __VA_ARGS__); break;                                        \
-    case 4: hipLaunchKernelGGL(K<4>, __VA_ARGS__); break;                                        \
-    case 8: hipLaunchKernelGGL(K<8>, __VA_ARGS__); break;                                        \
-    case 16: hipLaunchKernelGGL(K<16>, __VA_ARGS__); break;                                      \
-    case 32: hipLaunchKernelGGL(K<32>, __VA_ARGS__); break;                                      \
-    default: hipLaunchKernelGGL(K<64>, __VA_ARGS__); break;                                      \
-    }
-
 void launch_spmv(const int64_t *rp, const int32_t *col, const double *val, int64_t M, int64_t nnz, const double *x,
                  const double *z, double *y, double alpha, double beta, hipStream_t st)
 {
