@@ -139,6 +139,23 @@ idx_t *SpMM_Symb___1(ROW_MMSYM_IMPL_MODE mode, spmat *a, spmat *b);
 idx_t *SpMM_Symb_OutIdxs__0(ROW_MMSYM_IMPL_MODE mode, spmat *a, spmat *b, idx_t ***outIdxs);
 idx_t *SpMM_Symb_OutIdxs__1(ROW_MMSYM_IMPL_MODE mode, spmat *a, spmat *b, idx_t ***outIdxs);
 
+/* code/src/Sp3MM_CSR_OMP_SymbStep_Generic.c:281-417 built with COL_PARTS (and
+ * OUT_IDXS for the second pair): row lengths and indices as above, and
+ * *rowColPartsLens = malloc'd a->M x gridCols array (row-major) of each row's
+ * nonzeros per column share.  The shares split b's N columns uniformly as the
+ * reference's column partitioning does (macros.h:49-52: share g starts at
+ * g*(N/gridCols) + min(g, N%gridCols)).  Deviation: the reference sizes the
+ * shares from the row's own length and counts into uninitialised memory
+ * (SymbStep_Generic.c:196-220, 329-334), so its counts are not defined. */
+idx_t *SpMM_Symb__ColParts_0(ROW_MMSYM_IMPL_MODE mode, spmat *a, spmat *b, unsigned short gridCols,
+                             idx_t **rowColPartsLens);
+idx_t *SpMM_Symb__ColParts_1(ROW_MMSYM_IMPL_MODE mode, spmat *a, spmat *b, unsigned short gridCols,
+                             idx_t **rowColPartsLens);
+idx_t *SpMM_Symb_OutIdxs_ColParts_0(ROW_MMSYM_IMPL_MODE mode, spmat *a, spmat *b, idx_t ***outIdxs,
+                                    unsigned short gridCols, idx_t **rowColPartsLens);
+idx_t *SpMM_Symb_OutIdxs_ColParts_1(ROW_MMSYM_IMPL_MODE mode, spmat *a, spmat *b, idx_t ***outIdxs,
+                                    unsigned short gridCols, idx_t **rowColPartsLens);
+
 /* code/src/Sp3MM_CSR_OMP_SymbStep_Generic.c:453-579: exact row lengths of
  * a*b*c by direct row forwarding, plus the total (a->M+1 entries), and the
  * sorted row indices as for SpMM_Symb_OutIdxs__ when outIdxs != NULL. */

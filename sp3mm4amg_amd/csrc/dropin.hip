@@ -304,6 +304,45 @@ idx_t *symb_out(spmat *a, spmat *b, spmat *c, int base, idx_t ***outIdxs, const 
     return rl;
 }
 
+// Row lengths of A*B and, per row, the nonzeros in each of gridCols column
+// shares of B's N columns: share g starts at g*(N/gridCols) + min(g, N %
+// gridCols) and the first N % gridCols shares hold one column more (the
+// reference's uniform shares, macros.h:49-52).  outIdxs as SpMM_Symb_OutIdxs__.
+idx_t *symb_colparts(spmat *a, spmat *b, int base, idx_t ***outIdxs, unsigned short gridCols,
+                     idx_t **rowColPartsLens, const char *fn)
+{
+    if (!a || !b || !rowColPartsLens || gridCols == 0) {
+        fprintf(stderr, "%s: null argument or no column partitions\n", fn);
+        return nullptr;
+    }
+    idx_t **rows = nullptr;
+    idx_t *rl = symb_out(a, b, nullptr, base, &rows, fn);
+    if (!rl) return nullptr;
+    const idx_t M = a->M, N = b->N, div = N / gridCols, rem = N % gridCols, wide = rem * (div + 1);
+    idx_t *parts = static_cast<idx_t *>(calloc((size_t)(M ? M : 1) * gridCols, sizeof(idx_t)));
+    if (!parts) {
+        fprintf(stderr, "%s: malloc failed\n", fn);
+        free(rl);
+        if (M) free(rows[0]);
+        free(rows);
+        return nullptr;
+    }
+    for (idx_t r = 0; r < M; r++)
+        for (idx_t j = 0; j < rl[r]; j++) {
+            const idx_t c = rows[r][j] - base;
+            const idx_t g = c < wide ? c / (div + 1) : rem + (c - wide) / div;
+            parts[r * gridCols + g]++;
+        }
+    *rowColPartsLens = parts;
+    if (outIdxs) {
+        *outIdxs = rows;
+    } else {
+        free(rows[0]);
+        free(rows);
+    }
+    return rl;
+}
+
 }  // namespace
 
 extern "C" {
@@ -398,6 +437,26 @@ idx_t *SpMM_Symb_OutIdxs__0(ROW_MMSYM_IMPL_MODE, spmat *a, spmat *b, idx_t ***ou
 idx_t *SpMM_Symb_OutIdxs__1(ROW_MMSYM_IMPL_MODE, spmat *a, spmat *b, idx_t ***outIdxs)
 {
     return symb_out(a, b, nullptr, 1, outIdxs, "SpMM_Symb_OutIdxs__1");
+}
+idx_t *SpMM_Symb__ColParts_0(ROW_MMSYM_IMPL_MODE, spmat *a, spmat *b, unsigned short gridCols,
+                             idx_t **rowColPartsLens)
+{
+    return symb_colparts(a, b, 0, nullptr, gridCols, rowColPartsLens, "SpMM_Symb__ColParts_0");
+}
+idx_t *SpMM_Symb__ColParts_1(ROW_MMSYM_IMPL_MODE, spmat *a, spmat *b, unsigned short gridCols,
+                             idx_t **rowColPartsLens)
+{
+    return symb_colparts(a, b, 1, nullptr, gridCols, rowColPartsLens, "SpMM_Symb__ColParts_1");
+}
+idx_t *SpMM_Symb_OutIdxs_ColParts_0(ROW_MMSYM_IMPL_MODE, spmat *a, spmat *b, idx_t ***outIdxs,
+                                    unsigned short gridCols, idx_t **rowColPartsLens)
+{
+    return symb_colparts(a, b, 0, outIdxs, gridCols, rowColPartsLens, "SpMM_Symb_OutIdxs_ColParts_0");
+}
+idx_t *SpMM_Symb_OutIdxs_ColParts_1(ROW_MMSYM_IMPL_MODE, spmat *a, spmat *b, idx_t ***outIdxs,
+                                    unsigned short gridCols, idx_t **rowColPartsLens)
+{
+    return symb_colparts(a, b, 1, outIdxs, gridCols, rowColPartsLens, "SpMM_Symb_OutIdxs_ColParts_1");
 }
 idx_t *Sp3MM_Symb_OutIdxs_0(ROW_MMSYM_IMPL_MODE, spmat *a, spmat *b, spmat *c, idx_t ***outIdxs)
 {

@@ -113,6 +113,12 @@ def lib():
             f = getattr(L, f"Sp3MM_Symb_OutIdxs_{b}")
             f.argtypes = [C.c_int, P(Spmat), P(Spmat), P(Spmat), P(P(P(C.c_ulong)))]
             f.restype = P(C.c_ulong)
+            f = getattr(L, f"SpMM_Symb__ColParts_{b}")
+            f.argtypes = [C.c_int, P(Spmat), P(Spmat), C.c_ushort, P(P(C.c_ulong))]
+            f.restype = P(C.c_ulong)
+            f = getattr(L, f"SpMM_Symb_OutIdxs_ColParts_{b}")
+            f.argtypes = [C.c_int, P(Spmat), P(Spmat), P(P(P(C.c_ulong))), C.c_ushort, P(P(C.c_ulong))]
+            f.restype = P(C.c_ulong)
         L.freeSpmat.argtypes = [P(Spmat)]
         L.C_FortranShiftIdxs.argtypes = [P(Spmat)]
         L.Fortran_C_ShiftIdxs.argtypes = [P(Spmat)]
@@ -701,6 +707,34 @@ def symb_out_idxs(A: CSR, B: CSR, Cm: CSR | None = None, base: int = 0, mode: in
     _libc.free(first)
     _libc.free(C.cast(rows, C.c_void_p))
     return lens, out
+
+
+def symb_col_parts(A: CSR, B: CSR, grid_cols: int, base: int = 0, out_idxs: bool = False, mode: int = IDXMAP):
+    """SpMM_Symb__ColParts_ / SpMM_Symb_OutIdxs_ColParts_: (row lengths, A.M x
+    grid_cols per-share counts[, list of row index arrays])."""
+    L = lib()
+    hs = [HostSpmat(m, base) for m in (A, B)]
+    parts = C.POINTER(C.c_ulong)()
+    rows = C.POINTER(C.POINTER(C.c_ulong))()
+    if out_idxs:
+        p = getattr(L, f"SpMM_Symb_OutIdxs_ColParts_{base}")(mode, C.byref(hs[0].s), C.byref(hs[1].s), C.byref(rows),
+                                                            grid_cols, C.byref(parts))
+    else:
+        p = getattr(L, f"SpMM_Symb__ColParts_{base}")(mode, C.byref(hs[0].s), C.byref(hs[1].s), grid_cols,
+                                                     C.byref(parts))
+    lens = take_idx(p, A.M + 1)
+    counts = take_idx(parts, A.M * grid_cols).reshape(A.M, grid_cols)
+    if not out_idxs:
+        return lens, counts
+    out = []
+    for r in range(A.M):
+        n = int(lens[r])
+        out.append(np.ctypeslib.as_array(rows[r], shape=(n,)).astype(np.int64) - base if n
+                   else np.zeros(0, np.int64))
+    first = C.cast(rows[0], C.c_void_p).value if A.M else None
+    _libc.free(first)
+    _libc.free(C.cast(rows, C.c_void_p))
+    return lens, counts, out
 
 
 def elapsed_internal() -> float:

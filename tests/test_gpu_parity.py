@@ -131,6 +131,26 @@ def test_sizing_and_symbolic_entry_points(poisson2d64, fixture_2744, base):
         assert all(np.array_equal(rows[r], C3.ja[C3.irp[r]:C3.irp[r + 1]]) for r in range(R.M))
 
 
+@pytest.mark.parametrize("base", [0, 1])
+def test_symbolic_column_parts(poisson2d64, fixture_2744, base):
+    """SpMM_Symb__ColParts_ / _OutIdxs_ColParts_: per row, the product's
+    nonzeros in each uniform share of B's columns (the first N % g shares one
+    column wider), against the oracle's pattern of R*A."""
+    for R, A, _ in (poisson2d64, (fixture_2744["r"], fixture_2744["ac"], fixture_2744["p"])):
+        RA = O.spmm(R, A)
+        for g in (1, 3, 7, 64):
+            sizes = [A.N // g + (1 if i < A.N % g else 0) for i in range(g)]
+            edges = np.concatenate([[0], np.cumsum(sizes)])
+            want = np.stack([np.histogram(RA.ja[RA.irp[r]:RA.irp[r + 1]], bins=edges)[0] for r in range(R.M)])
+            lens, counts = L.symb_col_parts(R, A, g, base=base)
+            assert np.array_equal(lens, O.symb(R, A))
+            assert np.array_equal(counts, want), f"g={g}"
+            assert np.array_equal(counts.sum(axis=1), lens[:-1])
+            lens2, counts2, rows = L.symb_col_parts(R, A, g, base=base, out_idxs=True)
+            assert np.array_equal(lens2, lens) and np.array_equal(counts2, want)
+            assert all(np.array_equal(rows[r], RA.ja[RA.irp[r]:RA.irp[r + 1]]) for r in range(R.M))
+
+
 # ---------------------------------------------------------------------------
 # device-resident path on scaled versions of the benchmark workloads
 @pytest.mark.parametrize("name", ["poisson2d64", "poisson3d32", "stencil27_15", "irregular3000"])

@@ -33,7 +33,8 @@ def test_header_declares_the_reference_entry_points():
                   "spmmRowByRow2DBlocksAllocated_", "spmmRowByRow_SymbNum_",
                   "spmmRowByRow1DBlocks_SymbNum_", "spmmRowByRow2DBlocks_SymbNum_",
                   "spMMSizeUpperbound_", "spMMSizeUpperboundColParts_", "SpMM_Symb___",
-                  "SpMM_Symb_OutIdxs__", "Sp3MM_Symb_OutIdxs_"):
+                  "SpMM_Symb_OutIdxs__", "Sp3MM_Symb_OutIdxs_", "SpMM_Symb__ColParts_",
+                  "SpMM_Symb_OutIdxs_ColParts_"):
             assert f"{n}{base}" in names
     for n in ("allocSpMatrix", "freeSpmat", "freeSpmatInternal", "C_FortranShiftIdxs",
               "Fortran_C_ShiftIdxs"):
