@@ -2165,7 +2165,10 @@ void *hier_create(const sp3mm_dmat &A, int nlev, const int64_t *const *aggs, con
         L.pplan.R = L.I;
         L.pplan.A = L.sa.S;
         L.pplan.P = L.P0;
+        // with the identity on the left, a row's bound is its exact distinct
+        // count: a table of twice the bound keeps probe sequences short
         L.pplan.tu = c->tune;
+        L.pplan.tu.h1_shift = -1;
         if (L.pplan.build(c, t2) || L.tr.build(c, L.pplan.C, true)) {
             hier_release(h);
             return nullptr;

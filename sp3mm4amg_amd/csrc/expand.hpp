@@ -172,28 +172,46 @@ __device__ __forceinline__ void expand_flat(const int32_t *lcol, const LV *lval,
                 atomicAdd(vals + (hit ? sl[u] : dslot), lv[u] * w[u]);
             }
             if (pend) {
+                // Linear probing of the inserts whose first slot held another
+                // key: every pending insert of the lane advances one slot per
+                // iteration (all of them in one loop, so the group iterates as
+                // often as the longest probe sequence, not once per insert);
+                // inserts no longer pending swap and add on the lane's dummy
+                // slot, so an iteration has no per-insert branch.  A probe
+                // sequence is bounded by the table size, and once any lane of
+                // the group found the table full the row is redone elsewhere.
+                bool pd[U];
+                unsigned s2[U];
+#pragma unroll
                 for (int u = 0; u < U; u++) {
-                    if (old[u] == kEmpty || old[u] == c[u]) continue;
-                    // once any lane of the group found the table full, the row
-                    // is redone elsewhere: do not walk the probe sequence again
-                    if (__hip_atomic_load(fullflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
-                        full = true;
-                        continue;
-                    }
-                    unsigned s2 = sl[u];
-                    bool done = false;
-                    for (unsigned probe = 1; probe <= mask; probe++) {
-                        s2 = (s2 + 1) & mask;
-                        const int o = atomicCAS(keys + s2, kEmpty, c[u]);
-                        if (o == kEmpty || o == c[u]) {
-                            atomicAdd(vals + s2, lv[u] * w[u]);
-                            done = true;
-                            break;
-                        }
-                    }
-                    if (!done) {
+                    pd[u] = !(old[u] == kEmpty || old[u] == c[u]);
+                    s2[u] = sl[u];
+                }
+                for (unsigned probe = 1;; probe++) {
+                    bool anyp = false;
+#pragma unroll
+                    for (int u = 0; u < U; u++) anyp |= pd[u];
+                    if (!anyp) break;
+                    if (probe > mask ||
+                        __hip_atomic_load(fullflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
                         full = true;
                         __hip_atomic_store(fullflag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        break;
+                    }
+                    int o[U];
+                    unsigned a[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        s2[u] = (s2[u] + 1) & mask;
+                        a[u] = pd[u] ? s2[u] : dslot;
+                        o[u] = atomicCAS(keys + a[u], kEmpty, pd[u] ? c[u] : kEmpty);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const bool hit = pd[u] && (o[u] == kEmpty || o[u] == c[u]);
+                        atomicAdd(vals + (hit ? a[u] : dslot), lv[u] * w[u]);
+                        pd[u] = pd[u] && !hit;
                     }
                 }
             }
