@@ -1741,6 +1741,28 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
         ms = v[v.size() / 2];
         return 0;
     };
+    // a candidate that looked faster is timed again against the current
+    // schedule, interleaved (2*reps runs each, medians): one noisy median of a
+    // few runs does not decide the step
+    auto confirm = [&](const Tuning &t, const Tuning &u, double &ms_t, double &ms_u) -> int {
+        std::vector<double> vt, vu;
+        for (int k = 0; k < 2 * reps + 1; k++) {
+            for (int w = 0; w < 2; w++) {
+                c->tune = w ? u : t;
+                sp3mm_dmat C{};
+                sp3mm_stats st{};
+                const int rc = rap_direct_fused(R, A, P, C, &st, prec, guard > 0 ? guard : 1e-2, nullptr);
+                dmat_release(C);
+                if (rc) return -1;
+                (w ? vu : vt).push_back(st.ms_total);
+            }
+        }
+        std::sort(vt.begin(), vt.end());
+        std::sort(vu.begin(), vu.end());
+        ms_t = vt[vt.size() / 2];
+        ms_u = vu[vu.size() / 2];
+        return 0;
+    };
     Tuning cur = saved;
     double cur_ms = 0;
     if (measure(cur, cur_ms)) {
@@ -1770,8 +1792,15 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
                 return -1;
             }
             if (ms < 0.99 * cur_ms) {
-                cur = t;
-                cur_ms = ms;
+                double ms_t = 0, ms_cur = 0;
+                if (confirm(t, cur, ms_t, ms_cur)) {
+                    c->tune = saved;
+                    return -1;
+                }
+                if (ms_t < 0.99 * ms_cur) {
+                    cur = t;
+                    cur_ms = ms_t;
+                }
             }
         }
     }
