@@ -219,6 +219,159 @@ __device__ __forceinline__ void expand_flat(const int32_t *lcol, const LV *lval,
     }
 }
 
+// expand_flat for the large tables of groups of several wavefronts, where
+// long probe sequences are common: the pending inserts of a lane advance
+// together for kMergedProbes slots only, then one at a time (the settled ones
+// no longer cost swaps); a sequence ends after max_probe slots, which a caller
+// that redoes rows elsewhere sets below the table size.  (A separate function:
+// the one-wavefront kernels keep expand_flat's register allocation.)
+// probes per insert taken by all of a lane's pending inserts together
+constexpr unsigned kMergedProbes = 4;
+
+template <int G, typename LV, typename BV, int U>
+__device__ __forceinline__ void expand_flat_crowded(const int32_t *lcol, const LV *lval, int64_t e0, int64_t e1,
+                                            const int32_t *__restrict__ brp, const int32_t *__restrict__ bcol,
+                                            const BV *__restrict__ bval, int slog, int *keys, double *vals,
+                                            unsigned mask, int shift, unsigned dslot, bool &full, int *fullflag,
+                                            unsigned max_probe)
+{
+    constexpr int kLogG = G >= 256 ? 8 : G >= 128 ? 7 : G >= 64 ? 6 : G >= 32 ? 5 : 4;
+    if (slog > kLogG) slog = kLogG;
+    const int t = Group<G>::tid();
+    const int S = 1 << slog;
+    const int sub = t & (S - 1);
+    const int ngrp = G >> slog;
+    for (int64_t eb = e0 + (t >> slog); eb < e1; eb += (int64_t)ngrp * U) {
+        int q[U], qe[U];
+        double lv[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t e = eb + (int64_t)u * ngrp;
+            const bool ok = e < e1;
+            const int64_t es = ok ? e : eb;  // eb < e1: a valid entry
+            const int k = lcol[es];
+            const double x = (double)lval[es];
+            const int b0 = brp[k], b1 = brp[k + 1];
+            lv[u] = ok ? x : 0.0;
+            q[u] = ok ? b0 + sub : 0;
+            qe[u] = ok ? b1 : 0;
+        }
+        while (true) {
+            bool more = false;
+#pragma unroll
+            for (int u = 0; u < U; u++) more |= q[u] < qe[u];
+            if (!more) break;
+            int c[U], old[U];
+            unsigned sl[U];
+            double w[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const bool ok = q[u] < qe[u];
+                // some valid entry of B (the loop runs only while B has entries)
+                const unsigned qs = (unsigned)max(min(q[u], qe[u] - 1), 0);
+                const int cc = bcol[qs];
+                const double ww = (double)bval[qs];
+                c[u] = ok ? cc : kEmpty;
+                w[u] = ww;
+                q[u] += S;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) sl[u] = c[u] != kEmpty ? (hslot(c[u], shift) & mask) : dslot;
+#pragma unroll
+            for (int u = 0; u < U; u++) old[u] = atomicCAS(keys + sl[u], kEmpty, c[u]);
+            // keep the U compare-and-swaps in flight together: nothing that
+            // consumes their results is scheduled above this point
+            __builtin_amdgcn_sched_barrier(0);
+            // An absent product swapped its dummy slot (old == kEmpty), so it
+            // counts as a hit on sl == dslot and needs no separate test.
+            bool pend = false;
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const bool hit = old[u] == kEmpty || old[u] == c[u];
+                pend |= !hit;
+                atomicAdd(vals + (hit ? sl[u] : dslot), lv[u] * w[u]);
+            }
+            if (pend) {
+                // Linear probing of the inserts whose first slot held another
+                // key.  Every pending insert of the lane advances one slot per
+                // iteration (all of them in one loop, so the group iterates as
+                // often as the longest sequence, not once per insert); inserts
+                // no longer pending swap and add on the lane's dummy slot, so
+                // an iteration has no per-insert branch.  A probe sequence
+                // ends after the table size (groups of several wavefronts:
+                // after max_probe slots, which a caller that redoes rows
+                // elsewhere sets lower, so that a nearly full table is given
+                // up early), and once any lane of the group gave up, the
+                // others stop.
+                bool pd[U];
+                unsigned s2[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    pd[u] = !(old[u] == kEmpty || old[u] == c[u]);
+                    s2[u] = sl[u];
+                }
+                constexpr bool kTail = true;
+                unsigned probe = 1;
+                for (; !kTail || probe <= kMergedProbes; probe++) {
+                    bool anyp = false;
+#pragma unroll
+                    for (int u = 0; u < U; u++) anyp |= pd[u];
+                    if (!anyp) break;
+                    if (probe > (kTail ? max_probe : mask) ||
+                        __hip_atomic_load(fullflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                        full = true;
+                        __hip_atomic_store(fullflag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                        if constexpr (kTail) {
+#pragma unroll
+                            for (int u = 0; u < U; u++) pd[u] = false;
+                        }
+                        break;
+                    }
+                    int o[U];
+                    unsigned a[U];
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        s2[u] = (s2[u] + 1) & mask;
+                        a[u] = pd[u] ? s2[u] : dslot;
+                        o[u] = atomicCAS(keys + a[u], kEmpty, pd[u] ? c[u] : kEmpty);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int u = 0; u < U; u++) {
+                        const bool hit = pd[u] && (o[u] == kEmpty || o[u] == c[u]);
+                        atomicAdd(vals + (hit ? a[u] : dslot), lv[u] * w[u]);
+                        pd[u] = pd[u] && !hit;
+                    }
+                }
+                // longer sequences (crowded tables) go on one insert at a
+                // time, so the settled inserts stop costing swaps
+                if constexpr (kTail)
+                for (int u = 0; u < U; u++) {
+                    if (!pd[u]) continue;
+                    if (__hip_atomic_load(fullflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                        full = true;
+                        continue;
+                    }
+                    bool done = false;
+                    for (unsigned q = probe; q <= max_probe; q++) {
+                        s2[u] = (s2[u] + 1) & mask;
+                        const int o = atomicCAS(keys + s2[u], kEmpty, c[u]);
+                        if (o == kEmpty || o == c[u]) {
+                            atomicAdd(vals + s2[u], lv[u] * w[u]);
+                            done = true;
+                            break;
+                        }
+                    }
+                    if (!done) {
+                        full = true;
+                        __hip_atomic_store(fullflag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                }
+            }
+        }
+    }
+}
+
 // Appends, for every lane of the group with `claim` set, `slot` to the group's
 // list of claimed slots (list[n++] = slot): one LDS atomic per wavefront.
 template <int G>

@@ -572,14 +572,26 @@ __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand 
         if (t == 0) *fullflag = 0;
         Group<G>::sync();
         bool full1 = false, full2 = false;
-        expand_flat<G, VT, VT, U1>(R.col, rval, rb, re, A.rp32, A.col, aval, slog1, k1, v1, m1, sh1,
-                                   (unsigned)(H1 + H2 + t), full1, fullflag);
+        // multi-wavefront groups (large tables): a row whose table is nearly
+        // full is given up after half a table of probes, and the retry path
+        // redoes it
+        if constexpr (G > kWave)
+            expand_flat_crowded<G, VT, VT, U1>(R.col, rval, rb, re, A.rp32, A.col, aval, slog1, k1, v1, m1, sh1,
+                                               (unsigned)(H1 + H2 + t), full1, fullflag, m1 >> 1);
+        else
+            expand_flat<G, VT, VT, U1>(R.col, rval, rb, re, A.rp32, A.col, aval, slog1, k1, v1, m1, sh1,
+                                       (unsigned)(H1 + H2 + t), full1, fullflag);
         const int64_t rbn = R.rp[rn], ren = R.rp[rn + 1];
         Group<G>::sync();
         const int nn1 = compact_table<G, double>(k1, v1, H1, ired);
-        if (!full1 && !__hip_atomic_load(fullflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
-            expand_flat<G, double, VT, U2>(k1, v1, 0, nn1, P.rp32, P.col, pval, slog2, k2, v2, m2, sh2,
-                                           (unsigned)(H2 + t), full2, fullflag);
+        if (!full1 && !__hip_atomic_load(fullflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+            if constexpr (G > kWave)
+                expand_flat_crowded<G, double, VT, U2>(k1, v1, 0, nn1, P.rp32, P.col, pval, slog2, k2, v2, m2, sh2,
+                                                       (unsigned)(H2 + t), full2, fullflag, m2 >> 1);
+            else
+                expand_flat<G, double, VT, U2>(k1, v1, 0, nn1, P.rp32, P.col, pval, slog2, k2, v2, m2, sh2,
+                                               (unsigned)(H2 + t), full2, fullflag);
+        }
         const long long anyfull = group_max<G>((full1 || full2) ? 1 : 0, red);
         if (anyfull) {
             if (t == 0) {
