@@ -428,6 +428,36 @@ def irregular_leg(lib, problems, reps=2, n=1_000_000):
             "bytes_compulsory": bc, "roofline_frac_compulsory": round(bc / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
+def host_dropin_leg(lib, R, A, P, flop, reps=3):
+    """The drop-in entry point on host buffers, as the reference's callers use
+    it: sp3mmRowByRowMerged_0 takes host spmat operands (u64 indices), uploads
+    them, computes on the GPU and returns a malloc'd host spmat.  Times the C
+    call only (the numpy-owned operands are prepared before); this is the
+    PCIe-inclusive rate, never the headline value."""
+    import ctypes as C
+
+    import numpy as np
+    L = lib.lib()
+    hs = [lib.HostSpmat(m, 0) for m in (R, A, P)]
+    cfg = lib.default_config()
+    ms, nnz = [], 0
+    for k in range(reps + 1):
+        t0 = time.perf_counter()
+        out = L.sp3mmRowByRowMerged_0(C.byref(hs[0].s), C.byref(hs[1].s), C.byref(hs[2].s), C.byref(cfg), None)
+        t = (time.perf_counter() - t0) * 1e3
+        if not out:
+            raise RuntimeError("sp3mmRowByRowMerged_0 returned NULL")
+        nnz = int(out.contents.NZ)
+        L.freeSpmat(out)
+        if k:
+            ms.append(t)
+    med = float(np.median(ms))
+    moved = sum(m.nnz * 16 + (m.M + 1) * 8 for m in (R, A, P)) + nnz * 16 + (R.M + 1) * 8
+    return {"call": "sp3mmRowByRowMerged_0 (host spmat in, malloc'd host spmat out)", "ms_per_call": round(med, 3),
+            "gflops": round(flop / (med * 1e-3) / 1e9, 2), "device_ms_of_call": round(lib.elapsed_internal() * 1e3, 4),
+            "host_bytes_moved": moved, "host_GBs": round(moved / (med * 1e-3) / 1e9, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -440,6 +470,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched leg")
     ap.add_argument("--no-consumer", action="store_true", help="skip the V-cycle / SpMV leg")
     ap.add_argument("--no-irregular", action="store_true", help="skip the 1M irregular-rows leg")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-buffer drop-in leg")
     ap.add_argument("--schedule", default=None,
                     help="install this schedule instead of autotuning, e.g. u1=4,u2=3,subwave_h=0 "
                          "(fields of sp3mm_tuning; the others keep their defaults)")
@@ -549,6 +580,9 @@ def main():
     if not args.no_consumer:
         log("consumer leg (device hierarchy + V-cycle + SpMV) ...")
         rec["consumer"] = consumer_leg(lib, problems, dA, args.n)
+    if not args.no_host:
+        log("host-buffer drop-in leg")
+        rec["host_dropin"] = host_dropin_leg(lib, R, A, P, flop)
     if not args.no_irregular:
         log("irregular leg (1M lognormal rows) ...")
         rec["irregular"] = irregular_leg(lib, problems)
