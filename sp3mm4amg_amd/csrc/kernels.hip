@@ -1536,18 +1536,32 @@ __global__ void __launch_bounds__(kWin) k_window_direct(SpOperand R, SpOperand A
             double x = 0.0;
             if constexpr (NUM) x = V[j];
             const int p1 = P.rp32[c + 1];
-            for (int q = P.rp32[c]; q < p1; q++) {
-                const int j2 = P.col[q] - lo2;
-                if ((unsigned)j2 >= (unsigned)span2) {  // an unsorted P row
-                    atomicOr(err, kErrBadInput);
-                    continue;
+            // four entries of the P row at a time (loads in flight together)
+            for (int q0 = P.rp32[c]; q0 < p1; q0 += 4) {
+                int jj[4];
+                double pv[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const bool ok = q0 + u < p1;
+                    jj[u] = ok ? P.col[q0 + u] - lo2 : 0;
+                    pv[u] = 0.0;
+                    if constexpr (NUM) pv[u] = ok ? (double)pval[q0 + u] : 0.0;
                 }
-                const unsigned bit2 = 1u << (j2 & 31);
-                if (!(B2[j2 >> 5] & bit2)) atomicOr(B2 + (j2 >> 5), bit2);
-                if constexpr (NUM) {
-                    const double pr = x * (double)pval[q];
-                    atomicAdd(V2 + j2, pr);
-                    ps += pr;
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    if (q0 + u >= p1) break;
+                    const int j2 = jj[u];
+                    if ((unsigned)j2 >= (unsigned)span2) {  // an unsorted P row
+                        atomicOr(err, kErrBadInput);
+                        continue;
+                    }
+                    const unsigned bit2 = 1u << (j2 & 31);
+                    if (!(B2[j2 >> 5] & bit2)) atomicOr(B2 + (j2 >> 5), bit2);
+                    if constexpr (NUM) {
+                        const double pr = x * pv[u];
+                        atomicAdd(V2 + j2, pr);
+                        ps += pr;
+                    }
                 }
             }
         }
