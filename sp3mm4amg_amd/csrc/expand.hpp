@@ -219,21 +219,22 @@ __device__ __forceinline__ void expand_flat(const int32_t *lcol, const LV *lval,
     }
 }
 
+// probes per insert that expand_flat_crowded takes for all of a lane's
+// pending inserts together
+constexpr unsigned kMergedProbes = 4;
+
 // expand_flat for the large tables of groups of several wavefronts, where
 // long probe sequences are common: the pending inserts of a lane advance
 // together for kMergedProbes slots only, then one at a time (the settled ones
 // no longer cost swaps); a sequence ends after max_probe slots, which a caller
 // that redoes rows elsewhere sets below the table size.  (A separate function:
 // the one-wavefront kernels keep expand_flat's register allocation.)
-// probes per insert taken by all of a lane's pending inserts together
-constexpr unsigned kMergedProbes = 4;
-
 template <int G, typename LV, typename BV, int U>
 __device__ __forceinline__ void expand_flat_crowded(const int32_t *lcol, const LV *lval, int64_t e0, int64_t e1,
-                                            const int32_t *__restrict__ brp, const int32_t *__restrict__ bcol,
-                                            const BV *__restrict__ bval, int slog, int *keys, double *vals,
-                                            unsigned mask, int shift, unsigned dslot, bool &full, int *fullflag,
-                                            unsigned max_probe)
+                                                    const int32_t *__restrict__ brp,
+                                                    const int32_t *__restrict__ bcol, const BV *__restrict__ bval,
+                                                    int slog, int *keys, double *vals, unsigned mask, int shift,
+                                                    unsigned dslot, bool &full, int *fullflag, unsigned max_probe)
 {
     constexpr int kLogG = G >= 256 ? 8 : G >= 128 ? 7 : G >= 64 ? 6 : G >= 32 ? 5 : 4;
     if (slog > kLogG) slog = kLogG;
@@ -293,15 +294,11 @@ __device__ __forceinline__ void expand_flat_crowded(const int32_t *lcol, const L
             }
             if (pend) {
                 // Linear probing of the inserts whose first slot held another
-                // key.  Every pending insert of the lane advances one slot per
-                // iteration (all of them in one loop, so the group iterates as
-                // often as the longest sequence, not once per insert); inserts
-                // no longer pending swap and add on the lane's dummy slot, so
-                // an iteration has no per-insert branch.  A probe sequence
-                // ends after the table size (groups of several wavefronts:
-                // after max_probe slots, which a caller that redoes rows
-                // elsewhere sets lower, so that a nearly full table is given
-                // up early), and once any lane of the group gave up, the
+                // key.  For kMergedProbes slots every pending insert of the
+                // lane advances one slot per iteration, settled ones swapping
+                // and adding on the lane's dummy slot (no per-insert branch);
+                // the rest go on one at a time.  A sequence ends after
+                // max_probe slots, and once any lane of the group gave up, the
                 // others stop.
                 bool pd[U];
                 unsigned s2[U];
@@ -310,21 +307,18 @@ __device__ __forceinline__ void expand_flat_crowded(const int32_t *lcol, const L
                     pd[u] = !(old[u] == kEmpty || old[u] == c[u]);
                     s2[u] = sl[u];
                 }
-                constexpr bool kTail = true;
                 unsigned probe = 1;
-                for (; !kTail || probe <= kMergedProbes; probe++) {
+                for (; probe <= kMergedProbes; probe++) {
                     bool anyp = false;
 #pragma unroll
                     for (int u = 0; u < U; u++) anyp |= pd[u];
                     if (!anyp) break;
-                    if (probe > (kTail ? max_probe : mask) ||
+                    if (probe > max_probe ||
                         __hip_atomic_load(fullflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
                         full = true;
                         __hip_atomic_store(fullflag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-                        if constexpr (kTail) {
 #pragma unroll
-                            for (int u = 0; u < U; u++) pd[u] = false;
-                        }
+                        for (int u = 0; u < U; u++) pd[u] = false;
                         break;
                     }
                     int o[U];
@@ -345,7 +339,6 @@ __device__ __forceinline__ void expand_flat_crowded(const int32_t *lcol, const L
                 }
                 // longer sequences (crowded tables) go on one insert at a
                 // time, so the settled inserts stop costing swaps
-                if constexpr (kTail)
                 for (int u = 0; u < U; u++) {
                     if (!pd[u]) continue;
                     if (__hip_atomic_load(fullflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
