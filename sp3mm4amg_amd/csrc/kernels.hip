@@ -603,11 +603,13 @@ __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand 
             Group<G>::sync();
             const int nn2 = compact_table<G, double>(k2, v2, H2, ired);
             const int64_t off = (int64_t)i * H2;
+            // streaming stores: the slots are read once, by the gather, and
+            // should not evict the A and P rows other groups are gathering
             for (int j = t; j < nn2; j += G) {
                 const int k = k2[j];
                 const int rk = key_rank(k2, nn2, k);
-                scol[off + rk] = k;
-                sval[off + rk] = v2[j];
+                __builtin_nontemporal_store(k, scol + off + rk);
+                __builtin_nontemporal_store(v2[j], sval + off + rk);
             }
             if (t == 0) {
                 slotn[i] = nn2;
