@@ -25,6 +25,19 @@ enum : int {
 
 __device__ __forceinline__ int lane() { return (int)__lane_id(); }
 
+// A row count read from device memory, bounded by the count the launch (and
+// the buffers it indexes) was sized for; a larger count raises kErrOverflow
+// once per launch and the extra rows are not processed.
+__device__ __forceinline__ int clamp_count(const int *count, int cap, int *err)
+{
+    const int n = *count;
+    if (n > cap) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(err, kErrOverflow);
+        return cap;
+    }
+    return n;
+}
+
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
 // number of set bits of `mask` strictly below this lane

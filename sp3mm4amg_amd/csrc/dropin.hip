@@ -171,7 +171,8 @@ int allocSpMatrixInternal(idx_t rows, idx_t cols, spmat *mat)
     mat->M = rows;
     mat->N = cols;
     if (!(mat->IRP = static_cast<idx_t *>(calloc(rows + 1, sizeof(idx_t))))) {
-        fprintf(stderr, "IRP calloc err\n");
+        fprintf(stderr, "allocSpMatrixInternal: no host memory for %llu row offsets\n",
+                (unsigned long long)(rows + 1));
         freeSpmatInternal(mat);
         return EXIT_FAILURE;
     }
@@ -182,7 +183,7 @@ spmat *allocSpMatrix(idx_t rows, idx_t cols)
 {
     spmat *mat = static_cast<spmat *>(calloc(1, sizeof(spmat)));
     if (!mat) {
-        fprintf(stderr, "mat  calloc failed\n");
+        fprintf(stderr, "allocSpMatrix: no host memory for the matrix header\n");
         return nullptr;
     }
     if (allocSpMatrixInternal(rows, cols, mat)) {

@@ -1083,7 +1083,8 @@ struct FusedPlan {
                 launch_window_direct(true, std::min(nlong, c->num_cu), wl, r(), a(), p(), llist,
                                      d()->counts + thr.nb, dlog1, spill.as<int32_t>(), &d()->nspill,
                                      rown.as<int32_t>(), nullptr, nullptr, nullptr, mixed ? rowsum : nullptr, 0,
-                                     &d()->err, s, f32, lslotn.as<int32_t>(), lcol.as<int32_t>(), lval.as<double>());
+                                     &d()->err, s, f32, lslotn.as<int32_t>(), lcol.as<int32_t>(), lval.as<double>(),
+                                     nlong);
             else
                 launch_gather_rows(llist, d()->counts + thr.nb, nlong, lslotn.as<int32_t>(), lcap,
                                    lcol.as<int32_t>(), lval.as<double>(), C.rp, C.col, static_cast<double *>(C.val),

@@ -226,22 +226,33 @@ constexpr unsigned kMergedProbes = 4;
 // expand_flat for the large tables of groups of several wavefronts, where
 // long probe sequences are common: the pending inserts of a lane advance
 // together for kMergedProbes slots only, then one at a time (the settled ones
-// no longer cost swaps); a sequence ends after max_probe slots, which a caller
-// that redoes rows elsewhere sets below the table size.  (A separate function:
-// the one-wavefront kernels keep expand_flat's register allocation.)
+// no longer cost swaps).  The group counts the keys it claims in *claimed
+// (LDS, zeroed by the caller); once the count passes `limit` the row is given
+// up (`full`), and the caller, after a group barrier, also gives the row up
+// when the final count passes `limit`.  The count of distinct keys does not
+// depend on the order the lanes insert in, so whether a row is given up
+// depends on its pattern only (a probe sequence cannot exceed the table
+// while at most limit < H keys are in it).  (A separate function: the
+// one-wavefront kernels keep expand_flat's register allocation.)
 template <int G, typename LV, typename BV, int U>
 __device__ __forceinline__ void expand_flat_crowded(const int32_t *lcol, const LV *lval, int64_t e0, int64_t e1,
                                                     const int32_t *__restrict__ brp,
                                                     const int32_t *__restrict__ bcol, const BV *__restrict__ bval,
                                                     int slog, int *keys, double *vals, unsigned mask, int shift,
-                                                    unsigned dslot, bool &full, int *fullflag, unsigned max_probe)
+                                                    unsigned dslot, bool &full, int *fullflag, int *claimed,
+                                                    int limit)
 {
+    static_assert(G > kWave, "the claim count is kept per wavefront of a multi-wavefront group");
     constexpr int kLogG = G >= 256 ? 8 : G >= 128 ? 7 : G >= 64 ? 6 : G >= 32 ? 5 : 4;
     if (slog > kLogG) slog = kLogG;
     const int t = Group<G>::tid();
     const int S = 1 << slog;
     const int sub = t & (S - 1);
     const int ngrp = G >> slog;
+    auto given_up = [&]() {
+        return __hip_atomic_load(fullflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) ||
+               __hip_atomic_load(claimed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) > limit;
+    };
     for (int64_t eb = e0 + (t >> slog); eb < e1; eb += (int64_t)ngrp * U) {
         int q[U], qe[U];
         double lv[U];
@@ -286,20 +297,22 @@ __device__ __forceinline__ void expand_flat_crowded(const int32_t *lcol, const L
             // An absent product swapped its dummy slot (old == kEmpty), so it
             // counts as a hit on sl == dslot and needs no separate test.
             bool pend = false;
+            int fresh = 0;  // keys this wavefront claimed in this step
 #pragma unroll
             for (int u = 0; u < U; u++) {
                 const bool hit = old[u] == kEmpty || old[u] == c[u];
                 pend |= !hit;
                 atomicAdd(vals + (hit ? sl[u] : dslot), lv[u] * w[u]);
+                fresh += __popcll(ballot(c[u] != kEmpty && old[u] == kEmpty));
             }
+            if (fresh && lane() == __ffsll((unsigned long long)ballot(true)) - 1) atomicAdd(claimed, fresh);
             if (pend) {
                 // Linear probing of the inserts whose first slot held another
                 // key.  For kMergedProbes slots every pending insert of the
                 // lane advances one slot per iteration, settled ones swapping
                 // and adding on the lane's dummy slot (no per-insert branch);
-                // the rest go on one at a time.  A sequence ends after
-                // max_probe slots, and once any lane of the group gave up, the
-                // others stop.
+                // the rest go on one at a time.  Once the row is given up
+                // (here or by another lane) the pending inserts stop.
                 bool pd[U];
                 unsigned s2[U];
 #pragma unroll
@@ -313,8 +326,7 @@ __device__ __forceinline__ void expand_flat_crowded(const int32_t *lcol, const L
 #pragma unroll
                     for (int u = 0; u < U; u++) anyp |= pd[u];
                     if (!anyp) break;
-                    if (probe > max_probe ||
-                        __hip_atomic_load(fullflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                    if (probe > mask || given_up()) {
                         full = true;
                         __hip_atomic_store(fullflag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 #pragma unroll
@@ -330,27 +342,32 @@ __device__ __forceinline__ void expand_flat_crowded(const int32_t *lcol, const L
                         o[u] = atomicCAS(keys + a[u], kEmpty, pd[u] ? c[u] : kEmpty);
                     }
                     __builtin_amdgcn_sched_barrier(0);
+                    int nf = 0;
 #pragma unroll
                     for (int u = 0; u < U; u++) {
                         const bool hit = pd[u] && (o[u] == kEmpty || o[u] == c[u]);
                         atomicAdd(vals + (hit ? a[u] : dslot), lv[u] * w[u]);
+                        nf += (pd[u] && o[u] == kEmpty) ? 1 : 0;
                         pd[u] = pd[u] && !hit;
                     }
+                    if (nf) atomicAdd(claimed, nf);
                 }
                 // longer sequences (crowded tables) go on one insert at a
                 // time, so the settled inserts stop costing swaps
                 for (int u = 0; u < U; u++) {
                     if (!pd[u]) continue;
-                    if (__hip_atomic_load(fullflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                    if (given_up()) {
                         full = true;
+                        __hip_atomic_store(fullflag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                         continue;
                     }
                     bool done = false;
-                    for (unsigned q = probe; q <= max_probe; q++) {
+                    for (unsigned q = probe; q <= mask; q++) {
                         s2[u] = (s2[u] + 1) & mask;
                         const int o = atomicCAS(keys + s2[u], kEmpty, c[u]);
                         if (o == kEmpty || o == c[u]) {
                             atomicAdd(vals + s2[u], lv[u] * w[u]);
+                            if (o == kEmpty) atomicAdd(claimed, 1);
                             done = true;
                             break;
                         }

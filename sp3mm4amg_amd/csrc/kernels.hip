@@ -425,7 +425,7 @@ __global__ void __launch_bounds__(256) k_direct(SpOperand R, SpOperand A, SpOper
                                                  const int64_t *__restrict__ crp,
                                                  int32_t *__restrict__ ccol, double *__restrict__ cval,
                                                  int32_t *__restrict__ retry, int *__restrict__ nretry,
-                                                 int *__restrict__ err)
+                                                 int *__restrict__ err, int cap_rows)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int GPB = 256 / G;
@@ -449,7 +449,8 @@ __global__ void __launch_bounds__(256) k_direct(SpOperand R, SpOperand A, SpOper
 
     const unsigned m1 = (unsigned)H1 - 1, m2 = (unsigned)H2 - 1;
     const int sh1 = 32 - ilog2((unsigned)H1), sh2 = 32 - ilog2((unsigned)H2);
-    const int n = *count;
+    // the row list was sized for cap_rows rows (a larger count is reported)
+    const int n = clamp_count(count, cap_rows, err);
     const VT *rval = static_cast<const VT *>(R.val);
     const VT *aval = static_cast<const VT *>(A.val);
     const VT *pval = static_cast<const VT *>(P.val);
@@ -525,7 +526,7 @@ __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand 
                                                        int32_t *__restrict__ slotn,
                                                        int32_t *__restrict__ scol, double *__restrict__ sval,
                                                        int32_t *__restrict__ retry, int *__restrict__ nretry,
-                                                       int *__restrict__ err, int nxcd)
+                                                       int *__restrict__ err, int nxcd, int cap_rows)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int GPB = 256 / G;
@@ -543,12 +544,18 @@ __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand 
     p += (size_t)GPB * W * 4;
     long long *red = reinterpret_cast<long long *>(p) + g * Group<G>::kWaves;
     p += (size_t)GPB * Group<G>::kWaves * 8;
-    int *ired = reinterpret_cast<int *>(p) + g * (Group<G>::kWaves + 2);
+    int *ired = reinterpret_cast<int *>(p) + g * (Group<G>::kWaves + 4);
     int *fullflag = ired + Group<G>::kWaves + 1;  // a table of this row filled
+    // keys claimed in table 1 and table 2 (multi-wavefront groups, whose
+    // rows are given up past 7/8 of a table)
+    int *claimed1 = ired + Group<G>::kWaves + 2, *claimed2 = ired + Group<G>::kWaves + 3;
+    const int lim1 = H1 - H1 / 8, lim2 = H2 - H2 / 8;
 
     const unsigned m1 = (unsigned)H1 - 1, m2 = (unsigned)H2 - 1;
     const int sh1 = 32 - ilog2((unsigned)H1), sh2 = 32 - ilog2((unsigned)H2);
-    const int n = *count;
+    // the scratch slots hold cap_rows rows: a larger count (a replay whose
+    // bins differ from the plan's) is reported and never written past
+    const int n = clamp_count(count, cap_rows, err);
     const VT *rval = static_cast<const VT *>(R.val);
     const VT *aval = static_cast<const VT *>(A.val);
     const VT *pval = static_cast<const VT *>(P.val);
@@ -569,28 +576,36 @@ __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand 
             k1[s] = kEmpty;
             v1[s] = 0.0;
         }
-        if (t == 0) *fullflag = 0;
+        if (t == 0) {
+            *fullflag = 0;
+            *claimed1 = 0;
+            *claimed2 = 0;
+        }
         Group<G>::sync();
         bool full1 = false, full2 = false;
-        // multi-wavefront groups (large tables): a row whose table is nearly
-        // full is given up after half a table of probes, and the retry path
-        // redoes it
+        // multi-wavefront groups (large tables): a row that claims more than
+        // 7/8 of a table is given up, and the retry path redoes it
         if constexpr (G > kWave)
             expand_flat_crowded<G, VT, VT, U1>(R.col, rval, rb, re, A.rp32, A.col, aval, slog1, k1, v1, m1, sh1,
-                                               (unsigned)(H1 + H2 + t), full1, fullflag, m1 >> 1);
+                                               (unsigned)(H1 + H2 + t), full1, fullflag, claimed1, lim1);
         else
             expand_flat<G, VT, VT, U1>(R.col, rval, rb, re, A.rp32, A.col, aval, slog1, k1, v1, m1, sh1,
                                        (unsigned)(H1 + H2 + t), full1, fullflag);
         const int64_t rbn = R.rp[rn], ren = R.rp[rn + 1];
         Group<G>::sync();
+        if constexpr (G > kWave) full1 = full1 || *claimed1 > lim1;
         const int nn1 = compact_table<G, double>(k1, v1, H1, ired);
         if (!full1 && !__hip_atomic_load(fullflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
             if constexpr (G > kWave)
                 expand_flat_crowded<G, double, VT, U2>(k1, v1, 0, nn1, P.rp32, P.col, pval, slog2, k2, v2, m2, sh2,
-                                                       (unsigned)(H2 + t), full2, fullflag, m2 >> 1);
+                                                       (unsigned)(H2 + t), full2, fullflag, claimed2, lim2);
             else
                 expand_flat<G, double, VT, U2>(k1, v1, 0, nn1, P.rp32, P.col, pval, slog2, k2, v2, m2, sh2,
                                                (unsigned)(H2 + t), full2, fullflag);
+        }
+        if constexpr (G > kWave) {
+            Group<G>::sync();  // every wavefront's claims are counted
+            full2 = full2 || *claimed2 > lim2;
         }
         const long long anyfull = group_max<G>((full1 || full2) ? 1 : 0, red);
         if (anyfull) {
@@ -730,11 +745,11 @@ __global__ void __launch_bounds__(256) k_gather_rows(const int32_t *__restrict__
                                                       const double *__restrict__ sval,
                                                       const int64_t *__restrict__ crp, int32_t *__restrict__ ccol,
                                                       double *__restrict__ cval, int64_t out_cap,
-                                                      int *__restrict__ err)
+                                                      int *__restrict__ err, int cap_rows)
 {
     const int gpb = blockDim.x / L;
     const int hl = threadIdx.x & (L - 1);
-    const int n = *count;
+    const int n = clamp_count(count, cap_rows, err);
     for (int i = blockIdx.x * gpb + threadIdx.x / L; i < n; i += gridDim.x * gpb) {
         const int len = slotn[i];
         if (len <= 0) continue;
@@ -1335,7 +1350,7 @@ void launch_direct(int G, bool num, int H1, int H2, int slog1, int slog2, const 
     const unsigned grid = grid_for(rows_host, G, max_blocks);
     const size_t lds = direct_lds(G, H1, H2, num);
 #define SP3MM_DIRECT_ARGS R, A, P, rows, count, H1, H2, slog1, slog2, n1out, rownnz, crp, ccol, cval, \
-                          retry, nretry, err
+                          retry, nretry, err, rows_host
     if (f32in) {
         if (!num) return;  // the mixed path only recomputes numerics from FP32 inputs
         if (G == 64) hipLaunchKernelGGL((k_direct<64, true, float>), dim3(grid), dim3(256), lds, st, SP3MM_DIRECT_ARGS);
@@ -1424,7 +1439,8 @@ __global__ void __launch_bounds__(kWin) k_window_direct(SpOperand R, SpOperand A
                                                            int32_t *__restrict__ ccol, double *__restrict__ cval,
                                                            double *__restrict__ rowsum, int64_t out_cap,
                                                            int *__restrict__ err, int32_t *__restrict__ slotn,
-                                                           int32_t *__restrict__ scol, double *__restrict__ sval)
+                                                           int32_t *__restrict__ scol, double *__restrict__ sval,
+                                                           int cap_rows)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     const bool slots = NUM && scol != nullptr;
@@ -1438,7 +1454,8 @@ __global__ void __launch_bounds__(kWin) k_window_direct(SpOperand R, SpOperand A
     const VT *aval = static_cast<const VT *>(A.val);
     const VT *pval = static_cast<const VT *>(P.val);
     const int S = 1 << slog1;
-    const int n = *count;
+    // list entries i index the per-row slots: at most cap_rows of them
+    const int n = clamp_count(count, cap_rows, err);
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
         const int r = rows[i];
         const int64_t e0 = R.rp[r], e1 = R.rp[r + 1];
@@ -1646,13 +1663,13 @@ void launch_window_direct(bool num, int blocks, size_t lds, const SpOperand &R, 
                           const SpOperand &P, const int32_t *rows, const int *count, int slog1, int32_t *spill,
                           int *nspill, int32_t *rownnz, const int64_t *crp, int32_t *ccol, double *cval,
                           double *rowsum, int64_t out_cap, int *err, hipStream_t st, bool f32in,
-                          int32_t *slotn, int32_t *scol, double *sval)
+                          int32_t *slotn, int32_t *scol, double *sval, int cap_rows)
 {
     const int cap = window_cap(lds);
     const size_t bytes = (size_t)cap * 8 + (size_t)cap / 32 * 4;
     if (slog1 > 6) slog1 = 6;
 #define SP3MM_WIN_ARGS R, A, P, rows, count, cap, slog1, spill, nspill, rownnz, crp, ccol, cval, rowsum, out_cap, \
-                       err, slotn, scol, sval
+                       err, slotn, scol, sval, cap_rows
     if (f32in) {
         if (num) hipLaunchKernelGGL((k_window_direct<true, float>), dim3(blocks), dim3(kWin), bytes, st, SP3MM_WIN_ARGS);
         else hipLaunchKernelGGL((k_window_direct<false, float>), dim3(blocks), dim3(kWin), bytes, st, SP3MM_WIN_ARGS);
@@ -1731,11 +1748,11 @@ void launch_dense_direct(bool num, int blocks, const SpOperand &R, const SpOpera
 size_t fused_lds(int G, int H1, int H2)
 {
     const size_t gpb = 256 / G;
-    return gpb * ((size_t)H1 + H2 + (size_t)G) * 12 + gpb * group_waves(G) * 8 + gpb * (group_waves(G) + 2) * 4;
+    return gpb * ((size_t)H1 + H2 + (size_t)G) * 12 + gpb * group_waves(G) * 8 + gpb * (group_waves(G) + 4) * 4;
 }
 
 using FusedFn = void (*)(SpOperand, SpOperand, SpOperand, const int32_t *, const int *, int, int, int, int,
-                        int32_t *, int32_t *, int32_t *, double *, int32_t *, int *, int *, int);
+                        int32_t *, int32_t *, int32_t *, double *, int32_t *, int *, int *, int, int);
 
 // left entries held per lane: stage 1 U1 in {2, 4}, stage 2 U2 in {2, 3, 4}
 template <int G, typename VT>
@@ -1768,7 +1785,7 @@ void launch_direct_fused(int G, int H1, int H2, int slog1, int slog2, const SpOp
                                           : fused_variant<256, double>(u1, u2));
     const int nxcd = tu.xcd_map && grid >= (unsigned)kXcds ? kXcds : 1;
     hipLaunchKernelGGL(fn, dim3(grid), dim3(256), lds, st, R, A, P, rows, count, H1, H2, slog1, slog2, rownnz,
-                       slotn, scol, sval, retry, nretry, err, nxcd);
+                       slotn, scol, sval, retry, nretry, err, nxcd, rows_host);
 }
 
 void launch_gather_rows(const int32_t *rows, const int *count, int rows_host, const int32_t *slotn, int cap,
@@ -1781,13 +1798,13 @@ void launch_gather_rows(const int32_t *rows, const int *count, int rows_host, co
     if (grid < 1) grid = 1;
     if (L == 256)
         hipLaunchKernelGGL(k_gather_rows<256>, dim3(grid), dim3(256), 0, st, rows, count, slotn, cap, scol, sval,
-                           crp, ccol, cval, out_cap, err);
+                           crp, ccol, cval, out_cap, err, rows_host);
     else if (L == 16)
         hipLaunchKernelGGL(k_gather_rows<16>, dim3(grid), dim3(256), 0, st, rows, count, slotn, cap, scol, sval, crp,
-                           ccol, cval, out_cap, err);
+                           ccol, cval, out_cap, err, rows_host);
     else
         hipLaunchKernelGGL(k_gather_rows<32>, dim3(grid), dim3(256), 0, st, rows, count, slotn, cap, scol, sval, crp,
-                           ccol, cval, out_cap, err);
+                           ccol, cval, out_cap, err, rows_host);
 }
 
 __global__ void k_check_nnz(const int64_t *nnz, int64_t want, const int *nretry, int want_retry,

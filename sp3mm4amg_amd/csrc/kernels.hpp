@@ -75,7 +75,8 @@ void launch_window_direct(bool num, int blocks, size_t lds, const SpOperand &R, 
                           const SpOperand &P, const int32_t *rows, const int *count, int slog1, int32_t *spill,
                           int *nspill, int32_t *rownnz, const int64_t *crp, int32_t *ccol, double *cval,
                           double *rowsum, int64_t out_cap, int *err, hipStream_t st, bool f32in,
-                          int32_t *slotn = nullptr, int32_t *scol = nullptr, double *sval = nullptr);
+                          int32_t *slotn = nullptr, int32_t *scol = nullptr, double *sval = nullptr,
+                          int cap_rows = 0x7fffffff);
 void launch_dense_direct(bool num, int blocks, const SpOperand &R, const SpOperand &A, const SpOperand &P,
                          const int32_t *rows, const int *count, const DenseWs &ws, int slog1, int slog2,
                          int32_t *n1out, int32_t *rownnz, const int64_t *crp, int32_t *ccol, double *cval,

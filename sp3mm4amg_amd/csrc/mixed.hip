@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(256) k_direct_mixed(SpOperand R, SpOperand A, 
                                                        double *__restrict__ rowsum,
                                                        int32_t *__restrict__ retry, int *__restrict__ nretry,
                                                        int32_t *__restrict__ fix, int *__restrict__ nfix,
-                                                       int *__restrict__ err, int nxcd)
+                                                       int *__restrict__ err, int nxcd, int cap_rows)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int GPB = 256 / G;
@@ -271,7 +271,8 @@ __global__ void __launch_bounds__(256) k_direct_mixed(SpOperand R, SpOperand A, 
 
     const unsigned m1 = (unsigned)H1 - 1, m2 = (unsigned)H2 - 1;
     const int sh1 = 32 - ilog2((unsigned)H1), sh2 = 32 - ilog2((unsigned)H2);
-    const int n = *count;
+    // the scratch slots hold cap_rows rows (a larger count is reported)
+    const int n = clamp_count(count, cap_rows, err);
     const float *rv = static_cast<const float *>(R.val);
     const float *av = static_cast<const float *>(A.val);
     const float *pv = static_cast<const float *>(P.val);
@@ -410,7 +411,7 @@ void launch_direct_mixed(int G, int H1, int H2, int slog1, int slog2, float guar
     auto fn = G == 32 ? k_direct_mixed<32> : G == 64 ? k_direct_mixed<64> : k_direct_mixed<256>;
     const int nxcd = tu.xcd_map && grid >= (unsigned)kXcds ? kXcds : 1;
     hipLaunchKernelGGL(fn, dim3(grid), dim3(256), lds, st, R, A, P, rows, count, H1, H2, slog1, slog2, guard,
-                       rownnz, slotn, scol, sval, rowsum, retry, nretry, fix, nfix, err, nxcd);
+                       rownnz, slotn, scol, sval, rowsum, retry, nretry, fix, nfix, err, nxcd, rows_host);
 }
 
 }  // namespace sp3mm

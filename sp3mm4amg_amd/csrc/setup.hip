@@ -179,6 +179,12 @@ __global__ void __launch_bounds__(256) k_sa_fill(const int64_t *__restrict__ rp,
         }
         const double w = omega / d;
         const int64_t o0 = srp[r];
+        // S's row r was sized when the plan was built; a row of A that has
+        // grown since (its pattern changed) is reported, not written past
+        if (o0 + (e1 - b) + (has ? 0 : 1) != srp[r + 1]) {
+            if (sub == 0) atomicOr(err, kErrOverflow);
+            continue;
+        }
         for (int64_t e = b + sub; e < e1; e += L) {
             const int c = col[e];
             const int64_t pos = o0 + (e - b) + (!has && c > r ? 1 : 0);

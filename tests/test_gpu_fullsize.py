@@ -1,0 +1,136 @@
+"""GPU parity at the stated sizes of the mixed-precision and batched workloads.
+
+* 27-point random stencil on 100^3 with 3x3x3 smoothed aggregation
+  (1,000,000 fine rows, 39,304 coarse rows): FP64 on both paths against the
+  oracle; MIXED64 and MIXED on the FP32-rounded inputs against the oracle of
+  the same rounded inputs accumulated in long double.
+* The batch of 261 products: a 5-level smoothed-aggregation hierarchy from the
+  96^3 7-point operator (coarse operators made by the oracle) plus 256 2D
+  5-point problems with sides Uniform{16..256} (PCG64 seed 5), in one packed
+  product replayed as a HIP graph, every product against the oracle.
+
+Bars: row offsets and columns bit-exact; FP64 within the rounding-error model
+of the oracle (oracle.error_bound); MIXED and MIXED64 within 1e-10 relative
+per nonzero of the long-double oracle on the same FP32 inputs.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from sp3mm4amg_amd import lib as L
+from sp3mm4amg_amd import problems
+
+pytestmark = [pytest.mark.gpu, pytest.mark.slow]
+MIXED_TOL = 1e-10
+
+
+def _rounded32(m):
+    return m.with_vals(m.vals.astype(np.float32).astype(np.float64))
+
+
+def _check_pattern(got, want):
+    got.check()
+    assert got.shape == want.shape
+    assert np.array_equal(got.irp, want.irp), "row offsets differ"
+    assert np.array_equal(got.ja, want.ja), "column indices differ"
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu_lib():
+    L.init(0)
+    yield
+
+
+@pytest.fixture(scope="module")
+def stencil27_100():
+    p = problems.stencil27_sa(100, seed=3)
+    return p["R"], p["A"], p["P"]
+
+
+def test_stencil27_100_fp64_both_paths(stencil27_100):
+    R, A, P = stencil27_100
+    print(f"stencil27 100^3 FP64: R {R.shape} nnz {R.nnz}, A nnz {A.nnz}", flush=True)
+    want = O.sp3mm_pair(R, A, P)
+    bound = O.error_bound(R, A, P, want)
+    d = [L.DeviceCSR.upload(m) for m in (R, A, P)]
+    try:
+        for path in (L.DIRECT, L.PAIR):
+            out, st = L.rap(*d, path=path)
+            got = out.download()
+            out.free()
+            _check_pattern(got, want)
+            assert np.all(np.abs(got.vals - want.vals) <= bound), f"path {path}: beyond the error model"
+            print(f"  path {path}: nnz {got.nnz} ({st['ms_total']:.3f} ms)", flush=True)
+        assert want.nnz == 1_000_000
+    finally:
+        for m in d:
+            m.free()
+
+
+@pytest.mark.parametrize("prec", ["mixed64", "mixed"])
+def test_stencil27_100_mixed_forms(stencil27_100, prec):
+    R, A, P = (_rounded32(m) for m in stencil27_100)
+    print(f"stencil27 100^3 {prec}", flush=True)
+    want = O.sp3mm_pair_ld(R, A, P)
+    d = [L.DeviceCSR.upload(m, vtype=L.F32) for m in (R, A, P)]
+    code = L.MIXED64 if prec == "mixed64" else L.MIXED
+    try:
+        out, st = L.rap(*d, path=L.DIRECT, precision=code)
+        got = out.download()
+        out.free()
+    finally:
+        for m in d:
+            m.free()
+    _check_pattern(got, want)
+    err = O.rel_errors(got, want)
+    print(f"  max rel {err.max():.3e} median {np.median(err):.3e} "
+          f"rows recomputed in FP64 {st['guarded_rows']}", flush=True)
+    assert err.max() <= MIXED_TOL
+
+
+@pytest.fixture(scope="module")
+def batch_items():
+    hier = problems.hierarchy_3d(96, 5, rap=lambda R, A, P: O.sp3mm_pair(R, A, P))
+    items = [(h["R"], h["A"], h["P"]) for h in hier]
+    items += [(p["R"], p["A"], p["P"]) for p in problems.poisson2d_batch(256, seed=5)]
+    return items
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed64", "mixed"])
+def test_batch_261_products_graph(batch_items, prec):
+    mixed = prec != "fp64"
+    items = [tuple(_rounded32(m) for m in t) for t in batch_items] if mixed else batch_items
+    print(f"batch of {len(items)} products, {prec}", flush=True)
+    vt = L.F32 if mixed else L.F64
+    code = {"fp64": L.FP64, "mixed64": L.MIXED64, "mixed": L.MIXED}[prec]
+    if mixed:
+        wants = [O.sp3mm_pair_ld(*t) for t in items]
+        bounds = None
+    else:
+        wants = [O.sp3mm_pair(*t) for t in items]
+        bounds = [O.error_bound(*t, w) for t, w in zip(items, wants)]
+    dev = [[L.DeviceCSR.upload(m, vtype=vt) for m in t] for t in items]
+    b = L.Batch([t[0] for t in dev], [t[1] for t in dev], [t[2] for t in dev], graph=True, precision=code)
+    try:
+        assert b.info()["graph"]
+        for rep in range(2):
+            b.run(graph=True)
+            worst = 0.0
+            for i, want in enumerate(wants):
+                out = b.result(i)
+                got = out.download()
+                out.free()
+                _check_pattern(got, want)
+                if mixed:
+                    if want.nnz:
+                        err = O.rel_errors(got, want)
+                        worst = max(worst, float(err.max()))
+                        assert err.max() <= MIXED_TOL, f"product {i}: max rel {err.max():.3e}"
+                else:
+                    assert np.all(np.abs(got.vals - want.vals) <= bounds[i]), f"product {i}"
+            print(f"  replay {rep}: all {len(items)} products match (worst rel {worst:.3e})", flush=True)
+    finally:
+        b.free()
+        for t in dev:
+            for m in t:
+                m.free()
