@@ -275,6 +275,30 @@ void sp3mm_dmat_free(sp3mm_dmat *m);
 /* out = R*A*P on device-resident operands. */
 int sp3mm_rap(const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P,
               sp3mm_dmat *out, const sp3mm_opts *opts, sp3mm_stats *stats);
+/* Bytes of the fixed device scratch array behind sp3mm_rap_noalloc (one copy
+ * per GPU the library runs on, held until the process ends). */
+#define SP3MM_NOALLOC_SCRATCH_BYTES (256ll << 20)
+
+/* out = R*A*P, FP64, direct path (as sp3mm_rap with SP3MM_PATH_DIRECT and
+ * SP3MM_PREC_FP64), with no device allocation at all: no hipMalloc/hipFree,
+ * no pool, no HIP events.  It has no counterpart in the reference; it is for
+ * callers that cannot allocate device memory (code around HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The caller passes device
+ * arrays for the result: out_rp (R->M + 1 int64 offsets, 0-based), out_col
+ * (int32) and out_val (double) of `cap` entries each.  Every scratch buffer of
+ * the product comes from a fixed device array of SP3MM_NOALLOC_SCRATCH_BYTES
+ * bytes (roughly 64 bytes per row of R plus 12 bytes per hash-table slot of
+ * every row); a product that needs more is refused with an error that says
+ * so.  On success *nnz receives the number of nonzeros written.  If the
+ * product has more than `cap` nonzeros the call fails and *nnz receives the
+ * capacity a repeated call needs (else -1).  The scratch array is shared by
+ * every call on a GPU and keeps the last call's contents; calls return after
+ * their device work has finished and are serialised by the library, so calls
+ * that overlap on one GPU (e.g. from several processes' streams sharing a
+ * device by other means) race and are not supported. */
+int sp3mm_rap_noalloc(const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P,
+                      int64_t *out_rp, int32_t *out_col, double *out_val, int64_t cap,
+                      int64_t *nnz, sp3mm_stats *stats);
 /* out = A*B. */
 int sp3mm_spgemm(const sp3mm_dmat *A, const sp3mm_dmat *B, sp3mm_dmat *out,
                  sp3mm_stats *stats);

@@ -305,6 +305,17 @@ int sp3mm_rap(const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P, sp3
     return rap_pair(*R, *A, *P, *out, stats, o.timing != 0);
 }
 
+int sp3mm_rap_noalloc(const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P, int64_t *out_rp,
+                      int32_t *out_col, double *out_val, int64_t cap, int64_t *nnz, sp3mm_stats *stats)
+{
+    clear_error();
+    if (!R || !A || !P || !nnz) {
+        set_error("rap_noalloc: null argument");
+        return -1;
+    }
+    return rap_direct_noalloc(*R, *A, *P, out_rp, out_col, out_val, cap, *nnz, stats);
+}
+
 int sp3mm_spgemm(const sp3mm_dmat *A, const sp3mm_dmat *B, sp3mm_dmat *out, sp3mm_stats *stats)
 {
     clear_error();

@@ -97,6 +97,14 @@ void Pool::trim()
 static Context *g_ctx = nullptr;
 static std::mutex g_ctx_mu;
 
+// Fixed device scratch of the allocation-free entry point (sp3mm_rap_noalloc).
+// Every GPU the library runs on holds its own copy until the process ends.  In
+// host code `scratch` is only a host-side stand-in for the symbol: its device
+// address is resolved once per device at init (hipGetSymbolAddress, with the
+// device current) into Context::noalloc_base, and only that address is handed
+// to kernels, fills and copies.
+static __device__ __align__(256) unsigned char scratch[SP3MM_NOALLOC_SCRATCH_BYTES];
+
 int tuning_valid(const Tuning &t)
 {
     const bool pow2h = t.subwave_h == 0 || (t.subwave_h >= 64 && t.subwave_h <= 4096 &&
@@ -160,6 +168,13 @@ int init_device(int dev)
     SP3MM_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
     c->stream = c->own_stream;
     SP3MM_HIP(hipHostMalloc((void **)&c->host_small, 4096, hipHostMallocDefault));
+    // this device's copy of the fixed scratch array (sp3mm_rap_noalloc reports
+    // an error if it cannot be resolved; the other entry points do not use it)
+    void *sp = nullptr;
+    if (hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)) == hipSuccess && sp)
+        c->noalloc_base = static_cast<unsigned char *>(sp);
+    else
+        (void)hipGetLastError();
     if (hipHostGetDevicePointer((void **)&c->host_small_dev, c->host_small, 0) != hipSuccess) {
         (void)hipGetLastError();
         c->host_small_dev = nullptr;
@@ -176,23 +191,46 @@ Context *ctx()
     return g_ctx;
 }
 
+static thread_local Arena *t_arena = nullptr;
+
+void *Arena::take(size_t b)
+{
+    const size_t a = (b + 255) & ~(size_t)255;
+    if (a > size - used) {
+        if (!refused) refused = a;
+        return nullptr;
+    }
+    void *p = base + used;
+    used += a;
+    return p;
+}
+
+ArenaScope::ArenaScope(Arena *a) : prev(t_arena) { t_arena = a; }
+ArenaScope::~ArenaScope() { t_arena = prev; }
+
 Buf::Buf(size_t b) : bytes(b)
 {
+    if (t_arena) {
+        p = t_arena->take(b ? b : 1);
+        arena = true;
+        return;
+    }
     Context *c = ctx();
     p = c ? c->pool.alloc(b) : nullptr;
 }
 
 Buf::~Buf()
 {
-    if (p && g_ctx) g_ctx->pool.release(p);
+    if (p && !arena && g_ctx) g_ctx->pool.release(p);
 }
 
 Buf &Buf::operator=(Buf &&o) noexcept
 {
     if (this != &o) {
-        if (p && g_ctx) g_ctx->pool.release(p);
+        if (p && !arena && g_ctx) g_ctx->pool.release(p);
         p = o.p;
         bytes = o.bytes;
+        arena = o.arena;
         o.p = nullptr;
         o.bytes = 0;
     }
@@ -415,6 +453,7 @@ struct Timer {
     }
     int mark()
     {
+        if (!on) return -1;  // disabled: no events are created
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return -1;
         (void)hipEventRecord(e, s);
@@ -1002,10 +1041,24 @@ struct FusedPlan {
     bool built = false;  // build() has read the retry and spill counts
     bool spec = false;   // tail launched before the retry count is known
     int64_t ccap = 0;    // entries allocated for C.col / C.val
+    // Output arrays supplied by the caller (C.rp, C.col, C.val set before
+    // build(), ext_cap entries in C.col / C.val); the plan then allocates no
+    // output and keeps no long-row slots (scratch comes from an Arena).
+    bool ext_out = false;
+    int64_t ext_cap = 0;
 
     // (re)allocates the output arrays for `cap` entries
     int alloc_out(Context *c, int64_t cap)
     {
+        if (ext_out) {
+            if (cap > ext_cap) {
+                set_error("output capacity of %lld entries is below the product's %lld nonzeros",
+                          (long long)ext_cap, (long long)cap);
+                return -1;
+            }
+            ccap = ext_cap;
+            return 0;
+        }
         c->pool.release(C.col);
         c->pool.release(C.val);
         C.col = static_cast<int32_t *>(c->pool.alloc((size_t)(cap ? cap : 1) * 4));
@@ -1139,6 +1192,7 @@ struct FusedPlan {
     }
     static constexpr int kMergeRows = 1024;
 
+
     int build(Context *c, Timer &tm)
     {
         hipStream_t s = c->stream;
@@ -1190,7 +1244,7 @@ struct FusedPlan {
         // long rows in one pass when their slots stay within 1 GiB
         lcap = window_cap(c->max_lds - 1024);
         lslots = false;
-        if (nlong > 0 && (size_t)nlong * lcap * 12 <= ((size_t)1 << 30)) {
+        if (nlong > 0 && !ext_out && (size_t)nlong * lcap * 12 <= ((size_t)1 << 30)) {
             lslotn = Buf((size_t)nlong * 4);
             lcol = Buf((size_t)nlong * lcap * 4);
             lval = Buf((size_t)nlong * lcap * 8);
@@ -1207,7 +1261,15 @@ struct FusedPlan {
         // long and retried rows: count pass of the window kernel (it lists the
         // rows whose windows exceed LDS), then the global dense count for those
         launch_long(c, s, false);
-        if (dmat_alloc(C, M, P.N, 0, SP3MM_F64)) return -1;
+        if (ext_out) {
+            C.M = M;
+            C.N = P.N;
+            C.vtype = SP3MM_F64;
+            C.owned = 0;
+            C.rp32 = nullptr;
+        } else if (dmat_alloc(C, M, P.N, 0, SP3MM_F64)) {
+            return -1;
+        }
         launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d()->nnz, s);
         // Without long rows every row that stayed on the hash tables fits its
         // scratch slot, so the slot total is a likely output size: allocate it
@@ -1215,7 +1277,10 @@ struct FusedPlan {
         // checks the capacity; if a retried row outgrew it, or a row spilled
         // to the global dense kernel, the exact path below redoes the tail.
         if (nlong == 0 && !mixed && stot > 0 && stot < INT32_MAX) {
-            if (alloc_out(c, stot)) return -1;
+            if (ext_out)
+                ccap = ext_cap;  // the caller's arrays; a product that outgrows them is caught below
+            else if (alloc_out(c, stot))
+                return -1;
             spec = true;
             launch_tail(c, s);
             spec = false;
@@ -1393,6 +1458,77 @@ int rap_direct_mixed(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat 
         return -1;
     }
     return rap_direct_fused(R, A, P, C, st, prec, guard > 0 ? guard : 1e-2, rowsum);
+}
+
+// The direct FP64 product of rap_direct without a single device allocation:
+// the output goes to the caller's arrays (crp: M+1 offsets, ccol/cval: cap
+// entries) and every scratch buffer of the plan (control block, bounds, bin
+// lists, scratch slots, dense workspaces) is carved from this device's fixed
+// scratch array.  The array keeps the previous call's contents; the plan
+// writes every buffer it reads, and the control block is cleared first.  No
+// HIP events are created.  Calls are serialised by the context mutex and
+// each returns after its stream work has finished.
+int rap_direct_noalloc(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int64_t *crp, int32_t *ccol,
+                       double *cval, int64_t cap, int64_t &nnz, sp3mm_stats *st)
+{
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    nnz = -1;
+    if (check_pair(R, A) || check_pair(A, P)) return -1;
+    if (R.vtype != SP3MM_F64 || A.vtype != SP3MM_F64 || P.vtype != SP3MM_F64) {
+        set_error("rap_noalloc: FP64 operands only");
+        return -1;
+    }
+    if (!crp || (cap > 0 && (!ccol || !cval)) || cap < 0) {
+        set_error("rap_noalloc: null output array or negative capacity");
+        return -1;
+    }
+    if (!c->noalloc_base) {
+        set_error("rap_noalloc: the fixed device scratch of device %d could not be resolved", c->device);
+        return -1;
+    }
+    Arena arena;
+    arena.base = c->noalloc_base;
+    arena.size = SP3MM_NOALLOC_SCRATCH_BYTES;
+    ArenaScope scope(&arena);
+    Timer tm(c->stream, false);
+    FusedPlan plan;
+    plan.R = R;
+    plan.A = A;
+    plan.P = P;
+    plan.tu = c->tune;
+    plan.ext_out = true;
+    plan.ext_cap = cap;
+    plan.C.rp = crp;
+    plan.C.col = ccol;
+    plan.C.val = cval;
+    const int rc = plan.build(c, tm);
+    if (rc) {
+        if (arena.refused) {
+            char why[600];
+            snprintf(why, sizeof why, "%s", last_error_text());
+            set_error("rap_noalloc: the product needs more than the %lld bytes of fixed device scratch (%lld in use "
+                      "when a further %lld were asked for): %s",
+                      (long long)SP3MM_NOALLOC_SCRATCH_BYTES, (long long)arena.used, (long long)arena.refused, why);
+        }
+        if (plan.built && plan.nnz > cap) nnz = plan.nnz;  // the capacity a retry needs
+        (void)hipStreamSynchronize(c->stream);
+        return -1;
+    }
+    // build() ends with a read-back of the control block: the stream is idle
+    nnz = plan.nnz;
+    if (st) {
+        memset(st, 0, sizeof *st);
+        st->mac1 = plan.mac1;
+        st->nnz_out = plan.nnz;
+        int launches = 0;
+        for (int i = 0; i < plan.thr.nb; i++) launches += plan.cnt[i] ? 1 : 0;
+        st->main_kernel_launches = launches;
+        st->guarded_entries = plan.nretry + plan.nlong;
+        st->spilled_rows = plan.nspill;
+    }
+    return 0;
 }
 
 // --------------------------------------------------------------------------

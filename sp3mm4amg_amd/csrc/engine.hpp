@@ -53,6 +53,9 @@ struct Context {
     Pool pool;
     // scheduling of the one-pass direct kernels (sp3mm_set_tuning / sp3mm_autotune)
     Tuning tune;
+    // device address of the fixed scratch array of the allocation-free entry
+    // point on this device (null if it could not be resolved)
+    unsigned char *noalloc_base = nullptr;
     // small pinned host mirror for counters read back between phases
     long long *host_small = nullptr;
     long long *host_small_dev = nullptr;  // its device-side address (kernels write it directly)
@@ -82,16 +85,32 @@ void queue_jacobi(const sp3mm_dmat &A, const double *b, double *x, double *work,
 
 Context *ctx();  // initialised on first use (device 0 unless sp3mm_init chose one)
 
+// Bump allocator over a fixed device region: while one is installed on the
+// calling thread (ArenaScope), every Buf of that thread is carved from it
+// instead of the pool, 256-byte aligned, and never returned.
+struct Arena {
+    unsigned char *base = nullptr;
+    size_t size = 0, used = 0;
+    size_t refused = 0;  // size of the first request that did not fit (0: none)
+    void *take(size_t bytes);
+};
+struct ArenaScope {
+    explicit ArenaScope(Arena *a);
+    ~ArenaScope();
+    Arena *prev;
+};
+
 // scratch buffer owned by one call
 struct Buf {
     void *p = nullptr;
     size_t bytes = 0;
+    bool arena = false;  // carved from an Arena (not returned to the pool)
     Buf() = default;
     explicit Buf(size_t b);
     ~Buf();
     Buf(const Buf &) = delete;
     Buf &operator=(const Buf &) = delete;
-    Buf(Buf &&o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+    Buf(Buf &&o) noexcept : p(o.p), bytes(o.bytes), arena(o.arena) { o.p = nullptr; o.bytes = 0; }
     Buf &operator=(Buf &&o) noexcept;
     template <typename T> T *as() const { return static_cast<T *>(p); }
 };
@@ -104,6 +123,10 @@ int rap_direct(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, sp
                sp3mm_stats *st, bool timing, bool symbolic_only = false);
 int rap_direct_mixed(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, sp3mm_dmat &C, sp3mm_stats *st,
                      int prec, double guard, double *rowsum);
+// R*A*P (FP64, direct path) into caller-owned device arrays, every scratch
+// buffer carved from the fixed device scratch array: no device allocation
+int rap_direct_noalloc(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int64_t *crp, int32_t *ccol,
+                       double *cval, int64_t cap, int64_t &nnz, sp3mm_stats *st);
 void dmat_release(sp3mm_dmat &m);
 void *batch_create(int count, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P, int prec,
                    double guard, bool graph);

@@ -136,6 +136,8 @@ def lib():
         L.sp3mm_dmat_free.argtypes = [P(DMat)]
         L.sp3mm_dmat_free.restype = None
         L.sp3mm_rap.argtypes = [P(DMat), P(DMat), P(DMat), P(DMat), P(Opts), P(Stats)]
+        L.sp3mm_rap_noalloc.argtypes = [P(DMat), P(DMat), P(DMat), C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_int64, P(C.c_int64), P(Stats)]
         L.sp3mm_spgemm.argtypes = [P(DMat), P(DMat), P(DMat), P(Stats)]
         L.sp3mm_ub.argtypes = [P(DMat), P(DMat), C.c_void_p]
         L.sp3mm_symbolic.argtypes = [P(DMat), P(DMat), P(DMat), C.c_void_p]
@@ -278,6 +280,24 @@ def rap(R: DeviceCSR, A: DeviceCSR, P: DeviceCSR, path: int = DIRECT, precision:
     _check(lib().sp3mm_rap(C.byref(R.d), C.byref(A.d), C.byref(P.d), C.byref(out), C.byref(o),
                            C.byref(st)), "sp3mm_rap")
     return DeviceCSR(out), st.as_dict()
+
+
+def rap_noalloc(R: DeviceCSR, A: DeviceCSR, P: DeviceCSR, rp: "DeviceArray", col: "DeviceArray",
+                val: "DeviceArray", cap: int):
+    """R*A*P (FP64, direct path) into caller-owned device arrays (rp: M+1
+    int64, col: int32, val: float64, `cap` entries each) with no device
+    allocation inside the library.  Returns (nnz, stats dict); on failure
+    raises RuntimeError, whose `needed` attribute holds the capacity a repeat
+    needs when the output arrays were too short (else -1)."""
+    nnz = C.c_int64(-1)
+    st = Stats()
+    rc = lib().sp3mm_rap_noalloc(C.byref(R.d), C.byref(A.d), C.byref(P.d), rp.ptr, col.ptr, val.ptr, cap,
+                                 C.byref(nnz), C.byref(st))
+    if rc != 0:
+        e = RuntimeError(f"sp3mm_rap_noalloc failed: {last_error()}")
+        e.needed = nnz.value
+        raise e
+    return nnz.value, st.as_dict()
 
 
 def spgemm(A: DeviceCSR, B: DeviceCSR):

@@ -18,6 +18,8 @@ LIB = os.path.join(ROOT, "sp3mm4amg_amd", "libsp3mm_amd.so")
 def declared_functions():
     text = open(HEADER).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    # preprocessor lines (macro values such as SP3MM_NOALLOC_SCRATCH_BYTES) declare no symbols
+    text = re.sub(r"^\s*#.*$", "", text, flags=re.M)
     names = set(re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text))
     names -= {"if", "while", "for", "return", "sizeof"}
     # function-pointer typedef names are not symbols
