@@ -62,88 +62,102 @@ def compulsory_bytes(R, A, P, nnz_out, vbytes=8):
         nnz_out * (vbytes + 4) + (R.M + 1) * po(nnz_out)
 
 
-def cpu_baseline(R, A, P, flop_per_row_fn, budget_s=21.0):
-    """Reference CPU path (oracle/_ref: the reference library compiled from its
-    sources) on bounded row samples of the same workload, R[rows]*A*P, in its
-    three forms: the pair product with its default SpMM
-    (spmmRowByRow2DBlocks_0 on the test driver's 20 x 2 grid), the pair
-    product with spmmRowByRow_SymbNum_0 (IDXMAP), and the direct (merged)
-    product; `value` is the fastest.  Without the reference build, the oracle's
-    faithful restatement of the SymbNum pair form is timed instead."""
+def _save_products(path, name, products):
     import numpy as np
-    from oracle import oracle as O
-    from oracle import ref
-    from sp3mm4amg_amd.csr import CSR
+    arrs = {"n": len(products), "name": name}
+    for i, trio in enumerate(products):
+        for k, m in zip(("R", "A", "P"), trio):
+            arrs[f"{k}{i}_shape"] = np.array([m.M, m.N])
+            arrs[f"{k}{i}_irp"] = m.irp
+            arrs[f"{k}{i}_ja"] = m.ja.astype(np.int32)
+            arrs[f"{k}{i}_vals"] = m.vals
+    np.savez(path, **arrs)
 
+
+def cpu_baseline(workloads):
+    """The reference CPU path timed on this host's cores, one workload at a
+    time, by oracle/cpu_bench.py in a process of its own (the reference
+    library compiled from its sources, oracle/_ref, in its three forms; the
+    oracle's restatement where that build is absent), under the reference's
+    timing protocol: OMP_PROC_BIND=close, OMP_PLACES=cores, OMP_NUM_THREADS
+    threads, the default OpenMP schedule adapted by chunksFairFolded, one
+    untimed warm-up and `iters` timed runs per form on a bounded row sample.
+    `workloads` is a list of (key, name, products, budget_s, iters); the
+    first is the headline workload, whose fastest form gives `value`."""
+    import subprocess
+    import tempfile
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    kind = "reference" if ref.available() else "port"
-
-    def sample(frac):
-        k = max(1, int(R.M * frac))
-        return CSR(k, R.N, R.irp[:k + 1], R.ja[:R.irp[k]], R.vals[:R.irp[k]])
-
-    def flop_of(Rs):
-        mac1 = int(O.ub(Rs, A)[-1])
-        mac2 = int(O.ub(O.spmm(Rs, A), P)[-1])
-        return 2 * (mac1 + mac2)
-
-    if kind == "reference":
-        forms = [("sp3mmRowByRowPair_0 + spmmRowByRow2DBlocks_0 (20 x 2 grid)",
-                  lambda S: ref.sp3mm_pair(S, A, P, spmm="spmmRowByRow2DBlocks_", threads=threads, grid=(20, 2))),
-                 ("sp3mmRowByRowPair_0 + spmmRowByRow_SymbNum_0 (IDXMAP)",
-                  lambda S: ref.sp3mm_pair(S, A, P, spmm="spmmRowByRow_SymbNum_", threads=threads)),
-                 ("sp3mmRowByRowMerged_0", lambda S: ref.sp3mm_merged(S, A, P, threads=threads))]
-    else:
-        def port(S):
-            O.set_threads(threads)
-            O.sp3mm_pair(S, A, P, faithful=True)
-        forms = [("oracle restatement of sp3mmRowByRowPair_0 + spmmRowByRow_SymbNum_0", port)]
-    each = budget_s / len(forms)
-    results = []
-    for name, fn in forms:
-        # probe on 1/256 of the rows, then a sample sized to this form's budget
-        S = sample(1 / 256)
-        t0 = time.perf_counter()
-        fn(S)
-        t = time.perf_counter() - t0
-        S = sample(min(1.0, (1 / 256) * each / max(t, 1e-3)))
-        t0 = time.perf_counter()
-        fn(S)
-        t = time.perf_counter() - t0
-        f = flop_of(S)
-        results.append({"form": name, "value": round(f / t / 1e9, 4), "unit": "GFLOP/s",
-                        "sample": f"rows 0..{S.M - 1} of R ({S.M / R.M:.3%}) times A times P, {t:.2f} s for {f} FLOP",
-                        "seconds": round(t, 3)})
-    best = max(results, key=lambda r: r["value"])
-    return {"value": best["value"], "unit": "GFLOP/s", "cores": threads, "kind": kind,
-            "sample": f"{best['form']}: {best['sample']}", "seconds": best["seconds"], "forms": results}
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    out = {}
+    with tempfile.TemporaryDirectory() as d:
+        for key, name, products, budget, iters in workloads:
+            path = os.path.join(d, f"{key}.npz")
+            _save_products(path, name, products)
+            log(f"cpu baseline: {name} ({len(products)} product(s), budget {budget:.0f} s, {iters} timed runs)")
+            r = subprocess.run([sys.executable, "-m", "oracle.cpu_bench", path, str(budget), str(iters)],
+                               cwd=ROOT, env=env, capture_output=True, text=True, timeout=600)
+            if r.returncode != 0:
+                raise RuntimeError(f"cpu baseline on {name} failed: {r.stderr[-2000:]}")
+            out[key] = json.loads(r.stdout.strip().splitlines()[-1])
+            os.remove(path)
+    head = out[workloads[0][0]]
+    best = max(head["forms"], key=lambda f: f["value"])
+    return {"value": head["value"], "unit": "GFLOP/s", "cores": head["cores"], "kind": head["kind"],
+            "sample": f"{best['form']}: {best['sample']}, median of {best['runs']} runs "
+                      f"({best['min_s']}-{best['max_s']} s)",
+            "cpu_model": head["cpu_model"], "nproc": head["nproc"], "omp": head["omp"],
+            "chunk_distribution": head["chunk_distribution"], "forms": head["forms"],
+            "configs": {k: {"workload": v["workload"], "value": v["value"], "best_form": v["best_form"],
+                            "forms": v["forms"]} for k, v in out.items() if k != workloads[0][0]}}
 
 
 def time_form(lib, dR, dA, dP, steps, warmup, **kw):
     """warmup + steps products; returns (wall ms per product, mean main-kernel ms,
-    median device ms, last stats)."""
+    median device ms, last stats, device ms of every product)."""
     import numpy as np
     for _ in range(warmup):
         out, st = lib.rap(dR, dA, dP, **kw)
         out.free()
     lib.lib().sp3mm_sync()
-    main_ms, total_ms = [], []
+    total_ms = []
     t0 = time.perf_counter()
     for _ in range(steps):
         out, st = lib.rap(dR, dA, dP, **kw)
-        main_ms.append(st["ms_main"])
         total_ms.append(st["ms_total"])
         out.free()
     lib.lib().sp3mm_sync()
     wall = (time.perf_counter() - t0) * 1e3 / steps
-    return wall, float(np.mean(main_ms)), float(np.median(total_ms)), st
+    main_ms = main_kernel_ms(lib, dR, dA, dP, steps, **kw)
+    return wall, float(np.mean(main_ms)), float(np.median(total_ms)), st, total_ms
+
+
+def main_kernel_ms(lib, dR, dA, dP, steps, **kw):
+    """Device time of the main kernel(s) of a product, from per-phase HIP
+    events (sp3mm_opts.timing); measured on separate products, since the events
+    add gaps to the launch sequence."""
+    ms = []
+    for _ in range(steps):
+        out, st = lib.rap(dR, dA, dP, timing=True, **kw)
+        ms.append(st["ms_main"])
+        out.free()
+    return ms
+
+
+def spread(xs):
+    """median, minimum, 90th percentile and maximum of a list of timings"""
+    import numpy as np
+    a = np.asarray(xs, dtype=float)
+    return {"median": round(float(np.median(a)), 4), "min": round(float(a.min()), 4),
+            "p90": round(float(np.percentile(a, 90)), 4), "max": round(float(a.max()), 4), "n": int(a.size)}
 
 
 def mixed_leg(lib, problems, steps, warmup, tune, n=100):
     """Mixed precision on the 27-point random stencil (100^3, 3x3x3 aggregates):
     FP32 operands with FP32-pair accumulation (MIXED) and with FP64 accumulation
-    (MIXED64) against the FP64 product of the same FP32-rounded operands."""
+    (MIXED64), and FP64 operands, each against the CPU oracle of the same
+    FP32-rounded operands accumulated in long double (oracle.sp3mm_pair_ld)."""
     import numpy as np
+    from oracle import oracle as O
     prob = problems.stencil27_sa(n, seed=3)
     R, A, P = prob["R"], prob["A"], prob["P"]
     r32 = [m.with_vals(m.vals.astype(np.float32).astype(np.float64)) for m in (R, A, P)]
@@ -153,29 +167,31 @@ def mixed_leg(lib, problems, steps, warmup, tune, n=100):
     out.free()
     mac1, mac2, nnz_ra = st_pair["mac1"], st_pair["mac2"], st_pair["nnz_mid"]
     flop = 2 * (mac1 + mac2)
-    ref, _ = lib.rap(*d64, path=lib.DIRECT)
-    a = ref.download()
-    ref.free()
+    a = O.sp3mm_pair_ld(*r32)
     rec = {"workload": f"{prob['name']} Galerkin R*A*P, direct path", "flop": flop, "mac1": mac1,
-           "mac2": mac2, "nnz_RAP": a.nnz}
+           "mac2": mac2, "nnz_RAP": a.nnz,
+           "error_reference": "CPU oracle on the same FP32-rounded operands, long-double accumulation"}
     for name, d, prec, vb in (("fp64", d64, lib.FP64, 8), ("mixed", d32, lib.MIXED, 4),
                               ("mixed64", d32, lib.MIXED64, 4)):
         lib.set_tuning(None)
         sched = None
         if tune:
             sched, _, _ = lib.autotune(*d, precision=prec, reps=3)
-        wall, main, dev, st = time_form(lib, *d, steps, warmup, path=lib.DIRECT, precision=prec)
+        wall, main, dev, st, devs = time_form(lib, *d, steps, warmup, path=lib.DIRECT, precision=prec)
         got, _ = lib.rap(*d, path=lib.DIRECT, precision=prec)
         b = got.download()
         got.free()
         same = np.array_equal(a.irp, b.irp) and np.array_equal(a.ja, b.ja)
+        if not same:
+            raise RuntimeError(f"mixed leg: the {name} pattern differs from the oracle's")
         rel = np.abs(b.vals - a.vals) / np.maximum(np.abs(a.vals), 1e-300)
         rec[name] = {"ms_per_product": round(wall, 4), "device_ms_median": round(dev, 4),
+                     "device_ms": spread(devs),
                      "main_kernel_ms": round(main, 4), "gflops": round(flop / wall / 1e6, 2),
                      "bytes_alg": alg_bytes(R, A, P, mac1, mac2, nnz_ra, a.nnz, vb), "schedule": sched,
-                     "pattern_equal": bool(same),
-                     "max_rel_err_vs_fp64": float(rel.max()) if rel.size else 0.0,
-                     "median_rel_err_vs_fp64": float(np.median(rel)) if rel.size else 0.0}
+                     "pattern_equal_oracle": bool(same),
+                     "max_rel_err_vs_oracle": float(rel.max()) if rel.size else 0.0,
+                     "median_rel_err_vs_oracle": float(np.median(rel)) if rel.size else 0.0}
         if name == "mixed":
             rec[name]["fp64_rows"] = int(st["guarded_rows"])
     lib.set_tuning(None)
@@ -183,7 +199,7 @@ def mixed_leg(lib, problems, steps, warmup, tune, n=100):
         m.free()
     rec["speedup_mixed_vs_fp64"] = round(rec["fp64"]["ms_per_product"] / rec["mixed"]["ms_per_product"], 3)
     rec["speedup_mixed64_vs_fp64"] = round(rec["fp64"]["ms_per_product"] / rec["mixed64"]["ms_per_product"], 3)
-    return rec
+    return rec, (R, A, P)
 
 
 def batch_leg(lib, problems, reps, tune=True):
@@ -193,6 +209,7 @@ def batch_leg(lib, problems, reps, tune=True):
     (one block-diagonal product replayed as one HIP graph) is timed against a
     loop of per-product calls, in FP64 and in mixed precision."""
     import numpy as np
+    from oracle import oracle as O
     t0 = time.perf_counter()
     hier = problems.hierarchy_3d(96, 5, rap=lambda R, A, P: lib.rap_host(R, A, P)[0])
     b2d = problems.poisson2d_batch(256, seed=5)
@@ -244,18 +261,24 @@ def batch_leg(lib, problems, reps, tune=True):
         for _ in range(reps):
             dev.append(b.run(True))
         graph_ms = (time.perf_counter() - t0) * 1e3 / reps
-        # spot-check the batched results against per-product calls
+        # every batched product against the CPU oracle of its operands (FP64
+        # sums for the FP64 form; long-double sums of the FP32-rounded
+        # operands for the mixed forms)
         worst = 0.0
-        for i in (0, 1, len(items) // 2, len(items) - 1):
-            ref, _ = lib.rap(*mats[i], **kw)
+        for i, (R, A, P) in enumerate(items):
+            if vt == lib.F32:
+                R, A, P = (m.with_vals(m.vals.astype(np.float32).astype(np.float64)) for m in (R, A, P))
+                want = O.sp3mm_pair_ld(R, A, P)
+            else:
+                want = O.sp3mm_pair(R, A, P)
             got = b.result(i)
-            a, g = ref.download(), got.download()
-            ref.free()
+            g = got.download()
             got.free()
-            if not (np.array_equal(a.irp, g.irp) and np.array_equal(a.ja, g.ja)):
-                raise RuntimeError(f"batched result {i} differs in pattern from the per-product call")
-            if a.nnz:
-                worst = max(worst, float(np.max(np.abs(a.vals - g.vals) / np.maximum(np.abs(a.vals), 1e-300))))
+            if not (np.array_equal(want.irp, g.irp) and np.array_equal(want.ja, g.ja)):
+                raise RuntimeError(f"batched result {i} ({form}) differs in pattern from the oracle")
+            if want.nnz:
+                worst = max(worst, float(np.max(np.abs(want.vals - g.vals)
+                                                / np.maximum(np.abs(want.vals), 1e-300))))
         info = b.info()
         b.free()
         for m in mats:
@@ -264,10 +287,12 @@ def batch_leg(lib, problems, reps, tune=True):
         rec[form] = {"schedule": schedule,
                      "loop_ms_per_batch": round(loop_ms, 4), "graph_ms_per_batch": round(graph_ms, 4),
                      "graph_device_ms_median": round(float(np.median(dev)), 4),
+                     "graph_device_ms": spread(dev),
+                     "products_checked_against_oracle": len(items),
+                     "max_rel_err_vs_oracle": worst,
                      "speedup_graph_vs_loop": round(loop_ms / graph_ms, 3),
                      "gflops_graph": round(flop / graph_ms / 1e6, 2),
-                     "graph_build_ms": round(info["ms_build"], 3),
-                     "max_rel_diff_batched_vs_single": worst}
+                     "graph_build_ms": round(info["ms_build"], 3)}
     rec["flop"] = flop
     rec["bytes_alg_fp64"] = b_alg
     rec["bytes_compulsory_fp64"] = b_comp
@@ -313,7 +338,7 @@ def batch_leg(lib, problems, reps, tune=True):
         rec[f"speedup_{form}_graph_vs_fp64_graph"] = round(rec["fp64"]["graph_ms_per_batch"]
                                                            / rec[form]["graph_ms_per_batch"], 3)
     lib.set_tuning(None)
-    return rec
+    return rec, items
 
 
 def consumer_leg(lib, problems, dA, n, reps=50):
@@ -476,6 +501,8 @@ def main():
                          "(fields of sp3mm_tuning; the others keep their defaults)")
     ap.add_argument("--no-tune", action="store_true",
                     help="use the default schedule instead of autotuning each workload (untimed setup)")
+    ap.add_argument("--only-leg", choices=["mixed", "batch", "irregular", "consumer"], default=None,
+                    help="run only this leg and print its record (for profiling; not the benchmark line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; the product is sharded by rows "
@@ -486,6 +513,21 @@ def main():
     from sp3mm4amg_amd import lib, problems
 
     lib.init(0)
+    if args.only_leg:
+        steps = max(3, args.steps // 2)
+        if args.only_leg == "mixed":
+            leg = mixed_leg(lib, problems, steps, args.warmup, not args.no_tune)[0]
+        elif args.only_leg == "batch":
+            leg = batch_leg(lib, problems, max(3, args.steps // 4), tune=not args.no_tune)[0]
+        elif args.only_leg == "irregular":
+            leg = irregular_leg(lib, problems, reps=max(1, args.steps))
+        else:
+            p = problems.poisson3d_sa(args.n)
+            dA = lib.DeviceCSR.upload(p["A"])
+            leg = consumer_leg(lib, problems, dA, args.n)
+            dA.free()
+        print(json.dumps({"leg": args.only_leg, "record": leg}), flush=True)
+        return
     t0 = time.perf_counter()
     prob = problems.poisson3d_sa(args.n)
     R, A, P = prob["R"], prob["A"], prob["P"]
@@ -514,16 +556,18 @@ def main():
         out, st = lib.rap(dR, dA, dP, path=path)
         out.free()
     lib.lib().sp3mm_sync()
-    main_ms, total_ms = [], []
+    total_ms, step_ms = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
+        ts = time.perf_counter()
         out, st = lib.rap(dR, dA, dP, path=path)
-        main_ms.append(st["ms_main"])
         total_ms.append(st["ms_total"])
         nnz_out = st["nnz_out"]
         out.free()
+        step_ms.append((time.perf_counter() - ts) * 1e3)
     lib.lib().sp3mm_sync()
     wall = time.perf_counter() - t0
+    main_ms = main_kernel_ms(lib, dR, dA, dP, args.steps, path=path)
     ms_per_step = wall * 1e3 / args.steps
     value = flop * args.steps / wall / 1e9
 
@@ -551,13 +595,22 @@ def main():
                    "rows_fine": A.M, "rows_coarse": R.M, "nnz_R": R.nnz, "nnz_A": A.nnz,
                    "nnz_P": P.nnz, "nnz_RA": nnz_ra, "nnz_RAP": nnz_out, "mac1": mac1,
                    "mac2": mac2, "flop": flop, "schedule": schedule, "autotune_s": round(tune_s, 2)},
+        # the dominant kernel's roofline (algorithmic bytes of the product over
+        # the main kernel's mean duration), and beside it the target figure:
+        # the same bytes over the whole product's device time
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": committed_traffic() if args.path == "direct" and args.n == 128 else None,
                      "traffic_source": "profiles/pmc_direct_poisson3d128.md",
                      "kernel": f"numeric {args.path} kernel(s)", "kernel_ms": round(main_avg, 4),
-                     "bytes_alg": bytes_alg},
+                     "kernel_ms_spread": spread(main_ms),
+                     "bytes_alg": bytes_alg,
+                     "target_frac_whole_product": round(bytes_alg / (dev_med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "target_definition": "bytes_alg / (device time of the complete product: binning, "
+                                          "main kernel, scan, output allocation, gather) / 8 TB/s"},
         "product": {"device_ms_median": round(dev_med, 4),
+                    "device_ms": spread(total_ms),
+                    "wall_ms_per_step": spread(step_ms),
                     "roofline_frac": round(bytes_alg / (dev_med * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                     "gflops_device": round(flop / (dev_med * 1e-3) / 1e9, 2),
                     "bytes_compulsory": compulsory_bytes(R, A, P, nnz_out),
@@ -567,16 +620,18 @@ def main():
     }
     if path == lib.DIRECT:
         # the same workload on the pair path (R*A)*P, both SpGEMMs one-pass
-        wp, _, devp, _ = time_form(lib, dR, dA, dP, max(5, args.steps // 2), 1, path=lib.PAIR)
+        wp, _, devp, _, devsp = time_form(lib, dR, dA, dP, max(5, args.steps // 2), 1, path=lib.PAIR)
         rec["pair_path"] = {"ms_per_product": round(wp, 4), "device_ms_median": round(devp, 4),
+                            "device_ms": spread(devsp),
                             "gflops": round(flop / wp / 1e6, 2),
                             "roofline_frac_product": round(bytes_alg / (devp * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    stencil27 = batch_items = None
     if not args.no_mixed:
         log("mixed-precision leg (27-point stencil) ...")
-        rec["mixed_precision"] = mixed_leg(lib, problems, max(3, args.steps // 2), 1, not args.no_tune)
+        rec["mixed_precision"], stencil27 = mixed_leg(lib, problems, max(3, args.steps // 2), 1, not args.no_tune)
     if not args.no_batch:
         log("batched leg (hierarchy + 256 problems) ...")
-        rec["batched"] = batch_leg(lib, problems, max(3, args.steps // 4), tune=not args.no_tune)
+        rec["batched"], batch_items = batch_leg(lib, problems, max(3, args.steps // 4), tune=not args.no_tune)
     if not args.no_consumer:
         log("consumer leg (device hierarchy + V-cycle + SpMV) ...")
         rec["consumer"] = consumer_leg(lib, problems, dA, args.n)
@@ -587,8 +642,17 @@ def main():
         log("irregular leg (1M lognormal rows) ...")
         rec["irregular"] = irregular_leg(lib, problems)
     if not args.no_cpu:
-        log("timing the reference CPU path on a sample ...")
-        rec["cpu_baseline"] = cpu_baseline(R, A, P, None)
+        log("timing the reference CPU path on samples ...")
+        p1 = problems.poisson2d_sa(64)
+        work = [("config2", f"{prob['name']} (FP64)", [(R, A, P)], 18.0, 3),
+                ("config1", f"{p1['name']} (FP64, whole product, 40 timed runs)", [(p1["R"], p1["A"], p1["P"])],
+                 0.0, 40)]
+        if stencil27 is not None:
+            work.append(("config3", "stencil27_100^3_sa3x3x3_seed3 (FP64 values)", [stencil27], 8.0, 3))
+        if batch_items is not None:
+            work.append(("config5", "5-level 96^3 hierarchy + 256 2D problems (FP64, every product sampled)",
+                         batch_items, 8.0, 3))
+        rec["cpu_baseline"] = cpu_baseline(work)
     for m in (dR, dA, dP):
         m.free()
     print(json.dumps(rec), flush=True)

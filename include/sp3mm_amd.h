@@ -234,6 +234,10 @@ typedef struct {
     int32_t main_kernel_launches;
     int32_t spilled_rows;  /* direct path: of those, rows whose column windows
                               exceeded LDS (global dense accumulators)          */
+    int32_t reassociated;  /* direct path: 1 if the product ran as R*(A*P)
+                              (most rows beyond the one-wavefront tables and
+                              an output wider than an LDS window)               */
+    int32_t reserved2;
 } sp3mm_stats;
 
 /* Select (and initialise) the HIP device; called implicitly with device 0. */
@@ -313,8 +317,10 @@ int sp3mm_symbolic(const sp3mm_dmat *A, const sp3mm_dmat *B, const sp3mm_dmat *C
  * block-diagonally into one product.  create() packs the operands and runs the
  * product once (sizing bins, scratch and output); with use_graph the whole
  * launch sequence is captured into one HIP graph.  run() recomputes every
- * product (operand values may change, patterns may not; a changed pattern is
- * reported as an error) with one graph launch or a plain launch sequence, and
+ * product from the packed operands (repack() refreshes them from the
+ * caller's matrices: values may change, patterns may not, and a pattern
+ * change that alters the sizes the batch was built for is reported as an
+ * error) with one graph launch or a plain launch sequence, and
  * returns the device time in *ms.  result() copies product i into a new
  * device matrix.  Direct path; FP64 operands, or FP32 operands with
  * opts->precision == SP3MM_PREC_MIXED or SP3MM_PREC_MIXED64. */
@@ -322,6 +328,13 @@ typedef struct sp3mm_batch sp3mm_batch;
 int sp3mm_batch_create(int32_t count, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P,
                        const sp3mm_opts *opts, int32_t use_graph, sp3mm_batch **out);
 int sp3mm_batch_run(sp3mm_batch *b, int32_t use_graph, double *ms);
+/* Copy the current entries of the operands into the batch (the batch works on
+ * packed copies made by create()): the same count, shapes and nonzero counts
+ * as at creation, values and patterns as they are now.  The next run()
+ * recomputes every product from them; a pattern change that alters a
+ * product's sizes from those the batch was built for is reported by run(). */
+int sp3mm_batch_repack(sp3mm_batch *b, int32_t count, const sp3mm_dmat *R, const sp3mm_dmat *A,
+                       const sp3mm_dmat *P);
 int sp3mm_batch_result(sp3mm_batch *b, int32_t i, sp3mm_dmat *out);
 int sp3mm_batch_info(sp3mm_batch *b, double *ms_pack, double *ms_build, int64_t *nnz, int32_t *has_graph);
 void sp3mm_batch_free(sp3mm_batch *b);

@@ -109,14 +109,32 @@ def set_threads(n: int) -> None:
     _gomp.omp_set_num_threads(int(n))
 
 
-def config(threads: int, grid_rows: int = 0, grid_cols: int = 0, mode: int = IDXMAP) -> Config:
+def chunks_fair_folded():
+    """Address of the reference's chunksFairFolded (code/src/include/ompChunksDivide.h:77),
+    the chunk distribution its test driver installs for a non-static OpenMP
+    schedule (code/test/Sp3MM_test.c:80, :304-308); exported by oracle/ref_sched.c.
+    None if the reference build predates that export."""
+    L = lib()
+    try:
+        f = L.sp3mm_ref_chunks_fair_folded
+    except AttributeError:
+        return None
+    f.restype = C.c_void_p
+    return f()
+
+
+def config(threads: int, grid_rows: int = 0, grid_cols: int = 0, mode: int = IDXMAP,
+           fair_folded: bool = False) -> Config:
     """CONFIG as the reference test prepares it (code/test/Sp3MM_test.c:82-86,
-    :175-207 grid adaptation: 1D blocks use 2*threads row blocks)."""
+    :175-207 grid adaptation: 1D blocks use 2*threads row blocks).  With
+    fair_folded the chunk distribution function is the driver's
+    chunksFairFolded (else one that leaves the OpenMP schedule alone)."""
     lib()
     set_threads(threads)
     gr = grid_rows or max(1, 2 * threads)
     gc = grid_cols or 1
-    return Config(gr, gc, mode, threads, C.cast(_NOOP, C.c_void_p).value)
+    fn = chunks_fair_folded() if fair_folded else None
+    return Config(gr, gc, mode, threads, fn or C.cast(_NOOP, C.c_void_p).value)
 
 
 _libc = C.CDLL(None)
@@ -153,17 +171,17 @@ def spmm_fn(name: str, base: int = 0):
 
 
 def sp3mm_pair(R, A, P, spmm: str = "spmmRowByRow_SymbNum_", base: int = 0, threads: int = 1,
-               mode: int = IDXMAP, grid=(0, 0)) -> CSR:
+               mode: int = IDXMAP, grid=(0, 0), fair_folded: bool = False) -> CSR:
     r, a, p = Held(R, base), Held(A, base), Held(P, base)
-    cfg = config(threads, grid[0], grid[1], mode)
+    cfg = config(threads, grid[0], grid[1], mode, fair_folded)
     f = getattr(lib(), f"sp3mmRowByRowPair_{base}")
     out = f(C.byref(r.s), C.byref(a.s), C.byref(p.s), C.byref(cfg), spmm_fn(spmm, base))
     return take(out, base)
 
 
-def sp3mm_merged(R, A, P, base: int = 0, threads: int = 1) -> CSR:
+def sp3mm_merged(R, A, P, base: int = 0, threads: int = 1, fair_folded: bool = False) -> CSR:
     r, a, p = Held(R, base), Held(A, base), Held(P, base)
-    cfg = config(threads)
+    cfg = config(threads, fair_folded=fair_folded)
     f = getattr(lib(), f"sp3mmRowByRowMerged_{base}")
     return take(f(C.byref(r.s), C.byref(a.s), C.byref(p.s), C.byref(cfg), None), base)
 

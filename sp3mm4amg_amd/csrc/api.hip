@@ -295,7 +295,7 @@ int sp3mm_rap(const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P, sp3
             set_error("rap: the mixed-precision product runs on the direct path only");
             return -1;
         }
-        return rap_direct_mixed(*R, *A, *P, *out, stats, o.precision, o.guard, o.rowsum);
+        return rap_direct_mixed(*R, *A, *P, *out, stats, o.precision, o.guard, o.rowsum, o.timing != 0);
     }
     if (o.precision != SP3MM_PREC_FP64) {
         set_error("rap: unknown precision %d", o.precision);
@@ -393,6 +393,17 @@ int sp3mm_batch_run(sp3mm_batch *b, int32_t use_graph, double *ms)
 {
     clear_error();
     return batch_run(b, use_graph != 0, ms);
+}
+
+int sp3mm_batch_repack(sp3mm_batch *b, int32_t count, const sp3mm_dmat *R, const sp3mm_dmat *A,
+                       const sp3mm_dmat *P)
+{
+    clear_error();
+    if (!b || !R || !A || !P) {
+        set_error("batch: null argument");
+        return -1;
+    }
+    return batch_repack(b, count, R, A, P);
 }
 
 int sp3mm_batch_result(sp3mm_batch *b, int32_t i, sp3mm_dmat *out)

@@ -168,6 +168,17 @@ int init_device(int dev)
     SP3MM_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
     c->stream = c->own_stream;
     SP3MM_HIP(hipHostMalloc((void **)&c->host_small, 4096, hipHostMallocDefault));
+    // a coherent block the host polls for counters exported mid-stream (null:
+    // the plans read their counters back with a stream synchronisation)
+    if (hipHostMalloc((void **)&c->poll_host, 4096, hipHostMallocCoherent | hipHostMallocMapped) == hipSuccess &&
+        hipHostGetDevicePointer((void **)&c->poll_dev, c->poll_host, 0) == hipSuccess) {
+        memset(c->poll_host, 0, 4096);
+    } else {
+        (void)hipGetLastError();
+        if (c->poll_host) (void)hipHostFree(c->poll_host);
+        c->poll_host = nullptr;
+        c->poll_dev = nullptr;
+    }
     // this device's copy of the fixed scratch array (sp3mm_rap_noalloc reports
     // an error if it cannot be resolved; the other entry points do not use it)
     void *sp = nullptr;
@@ -373,6 +384,7 @@ static int sub_log(const sp3mm_dmat &B, int G)
 struct Ctl {
     int counts[16];
     int counts2[16];
+    int rawc[16];  // bin populations before small bins are merged (direct plans)
     int nretry;
     int err;
     int nspill;  // long rows whose column windows exceed the LDS window kernel
@@ -449,13 +461,23 @@ struct Timer {
     explicit Timer(hipStream_t st, bool enabled) : s(st), on(enabled) {}
     ~Timer()
     {
-        for (auto e : ev) (void)hipEventDestroy(e);
+        // events go back to the context's pool (creating one costs host time
+        // in the middle of a product's launch sequence)
+        for (auto e : ev) {
+            if (g_ctx && g_ctx->ev_pool.size() < 64) g_ctx->ev_pool.push_back(e);
+            else (void)hipEventDestroy(e);
+        }
     }
     int mark()
     {
         if (!on) return -1;  // disabled: no events are created
         hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return -1;
+        if (g_ctx && !g_ctx->ev_pool.empty()) {
+            e = g_ctx->ev_pool.back();
+            g_ctx->ev_pool.pop_back();
+        } else if (hipEventCreate(&e) != hipSuccess) {
+            return -1;
+        }
         (void)hipEventRecord(e, s);
         ev.push_back(e);
         return (int)ev.size() - 1;
@@ -468,6 +490,61 @@ struct Timer {
         return (double)t;
     }
 };
+
+constexpr size_t kPollFlagOff = 4088;
+
+// Reads the control block back without draining the stream: the block and a
+// new sequence number are exported into the coherent host block, and the host
+// spins on the flag (work queued behind the export keeps the GPU busy
+// meanwhile).  A stream that ends without the flag (a failed launch or a
+// fault) is reported.  Falls back to read_ctl without the coherent block.
+static int read_ctl(Context *c, const Ctl *d, Ctl &h);
+
+// Queues the export of the control block into the coherent host block and
+// returns its sequence number (0: no coherent block; wait with read_ctl).
+static unsigned long long export_ctl(Context *c, const Ctl *d)
+{
+    static_assert(sizeof(Ctl) <= kPollFlagOff, "control block fits before the flag");
+    if (!c->poll_host) return 0;
+    const unsigned long long seq = ++c->poll_seq;
+    auto *flag_dev = reinterpret_cast<unsigned long long *>(c->poll_dev + kPollFlagOff);
+    launch_export_flag(d, c->poll_dev, (int)(sizeof(Ctl) / 4), flag_dev, seq, c->stream);
+    return seq;
+}
+
+// Waits for the export `seq` (queued by export_ctl, possibly with more work
+// queued behind it) and copies the block out; seq 0 reads back with a stream
+// synchronisation instead.
+static int wait_ctl(Context *c, const Ctl *d, unsigned long long seq, Ctl &h)
+{
+    if (seq == 0) return read_ctl(c, d, h);
+    SP3MM_HIP(hipGetLastError());
+    auto *flag_host = reinterpret_cast<unsigned long long *>(c->poll_host + kPollFlagOff);
+    for (unsigned spins = 0;; spins++) {
+        if (__atomic_load_n(flag_host, __ATOMIC_ACQUIRE) == seq) break;
+        if ((spins & 1023) == 1023) {
+            const hipError_t q = hipStreamQuery(c->stream);
+            if (q == hipSuccess) {
+                if (__atomic_load_n(flag_host, __ATOMIC_ACQUIRE) == seq) break;
+                set_error("control block export did not arrive (sequence %llu)", seq);
+                return -1;
+            }
+            if (q != hipErrorNotReady) {
+                set_error("stream failed while waiting for the control block: %s", hipGetErrorString(q));
+                return -1;
+            }
+        }
+        __builtin_ia32_pause();
+    }
+    memcpy(&h, c->poll_host, sizeof(Ctl));
+    return 0;
+}
+
+static int read_ctl_poll(Context *c, const Ctl *d, Ctl &h)
+{
+    SP3MM_HIP(hipGetLastError());
+    return wait_ctl(c, d, export_ctl(c, d), h);
+}
 
 static int read_ctl(Context *c, const Ctl *d, Ctl &h)
 {
@@ -689,15 +766,20 @@ static int spgemm_onepass(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &
     const int nwB = dense_words(B.N);
     Buf dbits, dvals;
     const int wb = dense_blocks(c, nlong);
-    if (nlong > 0) {  // long rows: dense bitmap (+ values) in HBM
-        dbits = Buf((size_t)wb * nwB * 4);
-        dvals = Buf((size_t)wb * nwB * 32 * 8);
-        if (!dbits.p || !dvals.p) {
+    if (nlong > 0) {
+        // long rows: dense bitmap (+ values) in LDS when B's columns fit, else
+        // in an HBM workspace; the numeric pass of rows too wide for LDS
+        // values runs in LDS column windows when one fits (no value workspace)
+        const bool bits_lds = spgemm_dense_in_lds(false, nwB, c->max_lds);
+        const bool vals_ws = !spgemm_dense_in_lds(true, nwB, c->max_lds) && spgemm_window_cols(c->max_lds) == 0;
+        if (!bits_lds) dbits = Buf((size_t)wb * nwB * 4);
+        if (vals_ws) dvals = Buf((size_t)wb * nwB * 32 * 8);
+        if ((!bits_lds && !dbits.p) || (vals_ws && !dvals.p)) {
             set_error("dense workspace allocation failed");
             return -1;
         }
-        SP3MM_HIP(hipMemsetAsync(dbits.p, 0, dbits.bytes, s));
-        SP3MM_HIP(hipMemsetAsync(dvals.p, 0, dvals.bytes, s));
+        if (dbits.p) SP3MM_HIP(hipMemsetAsync(dbits.p, 0, dbits.bytes, s));
+        if (dvals.p) SP3MM_HIP(hipMemsetAsync(dvals.p, 0, dvals.bytes, s));
         launch_dense_spgemm(false, wb, a, b, lists.as<int32_t>() + (int64_t)thr.nb * M, d->counts + thr.nb,
                             dbits.as<unsigned>(), nullptr, nwB, sub_log(B, kDenseThreads), rown.as<int32_t>(),
                             nullptr, nullptr, nullptr, &d->err, s, c->max_lds);
@@ -1040,12 +1122,24 @@ struct FusedPlan {
     int nlong = 0, nretry = 0, nspill = 0;
     bool built = false;  // build() has read the retry and spill counts
     bool spec = false;   // tail launched before the retry count is known
+    // the speculative tail leaves rows whose tables filled for later: their
+    // window passes (count, then numeric) run only if some row was retried
+    bool defer_retry = false;
     int64_t ccap = 0;    // entries allocated for C.col / C.val
     // Output arrays supplied by the caller (C.rp, C.col, C.val set before
     // build(), ext_cap entries in C.col / C.val); the plan then allocates no
     // output and keeps no long-row slots (scratch comes from an Arena).
     bool ext_out = false;
     int64_t ext_cap = 0;
+    // per-phase device times (ms_ub, ms_num, ms_scan): HIP events between the
+    // phases, which add gaps to the stream; off unless asked for
+    bool detail = false;
+    int mk(Timer &tm) const { return detail ? tm.mark() : -1; }
+    // One-off FP64 products may be reassociated: build() stops after the
+    // binning and sets `reassoc` when most rows are beyond the one-wavefront
+    // tables and the output is wider than an LDS window (the caller then runs
+    // R*(A*P) as two SpGEMMs; see wants_reassoc)
+    bool reassoc_ok = false, reassoc = false;
 
     // (re)allocates the output arrays for `cap` entries
     int alloc_out(Context *c, int64_t cap)
@@ -1088,6 +1182,31 @@ struct FusedPlan {
     SpOperand a() const { return operand(A); }
     SpOperand p() const { return operand(P); }
 
+    // A bin launched before its population is known (`pred`, the bin the mean
+    // row bound falls in) owns scratch slots for all M rows; the others share
+    // scol/sval (offsets soff) and slotn (offsets nbase), sized by population.
+    int pred = -1;
+    Buf pcol, pval, pslotn;
+    int32_t *slot_col(int i) const { return i == pred ? pcol.as<int32_t>() : scol.as<int32_t>() + soff[i]; }
+    double *slot_val(int i) const { return i == pred ? pval.as<double>() : sval.as<double>() + soff[i]; }
+    int32_t *slot_n(int i) const { return i == pred ? pslotn.as<int32_t>() : slotn.as<int32_t>() + nbase[i]; }
+    int slot_rows(int i) const { return i == pred ? (int)R.M : cnt[i]; }
+
+    // The bin whose launch can go ahead of the read-back of the populations:
+    // the one the mean row bound nnz(R)/M * nnz(A)/A.M falls in, for FP64
+    // plans with their own output; never a multi-wavefront bin of a product
+    // that may be reassociated (its rows would be computed for nothing).
+    int predict_bin(Context *c) const
+    {
+        if (mixed || ext_out || R.M == 0 || A.M == 0 || R.M >= INT32_MAX) return -1;
+        const double est = (double)R.nnz / (double)R.M * ((double)A.nnz / (double)A.M);
+        int b = 0;
+        while (b < thr.nb && (double)thr.v[b] < est) b++;
+        if (b >= thr.nb) return -1;
+        if (cfg[b].G > kWave && reassoc_ok && P.N > (int64_t)window_cap(c->max_lds - 1024)) return -1;
+        return b;
+    }
+
     void launch_main(Context *c, hipStream_t s, int i)
     {
         const FusedCfg &f = cfg[i];
@@ -1100,9 +1219,8 @@ struct FusedPlan {
                                 fix.as<int32_t>() + nbase[i], d()->counts2 + i, &d()->err, max_blocks(c), s, tu);
         else
             launch_direct_fused(f.G, f.H1, f.H2, slog1, slog2, r(), a(), p(), lists.as<int32_t>() + (int64_t)i * M,
-                                d()->counts + i, cnt[i], rown.as<int32_t>(), slotn.as<int32_t>() + nbase[i],
-                                scol.as<int32_t>() + soff[i], sval.as<double>() + soff[i], retry.as<int32_t>(),
-                                &d()->nretry, &d()->err, max_blocks(c), s, f32, tu);
+                                d()->counts + i, slot_rows(i), rown.as<int32_t>(), slot_n(i), slot_col(i), slot_val(i),
+                                retry.as<int32_t>(), &d()->nretry, &d()->err, max_blocks(c), s, f32, tu);
     }
 
     void launch_tail(Context *c, hipStream_t s)
@@ -1110,9 +1228,8 @@ struct FusedPlan {
         const int64_t M = R.M;
         for (int i = 0; i < thr.nb; i++) {
             if (cnt[i] == 0) continue;
-            launch_gather_rows(lists.as<int32_t>() + (int64_t)i * M, d()->counts + i, cnt[i],
-                               slotn.as<int32_t>() + nbase[i], cfg[i].H2, scol.as<int32_t>() + soff[i],
-                               sval.as<double>() + soff[i], C.rp, C.col, static_cast<double *>(C.val), ccap,
+            launch_gather_rows(lists.as<int32_t>() + (int64_t)i * M, d()->counts + i, cnt[i], slot_n(i), cfg[i].H2,
+                               slot_col(i), slot_val(i), C.rp, C.col, static_cast<double *>(C.val), ccap,
                                &d()->err, max_blocks(c), s);
         }
         launch_long(c, s, true);
@@ -1147,7 +1264,7 @@ struct FusedPlan {
                                  dlog1, spill.as<int32_t>(), &d()->nspill, rn, num ? C.rp : nullptr,
                                  num ? C.col : nullptr, cv, rs, cap, &d()->err, s, f32);
         }
-        if (nretry > 0 || (!num && !built) || (num && spec))
+        if (!defer_retry && (nretry > 0 || (!num && !built) || (num && spec)))
             launch_window_direct(num, c->num_cu, wl, r(), a(), p(), retry.as<int32_t>(), &d()->nretry, dlog1,
                                  spill.as<int32_t>(), &d()->nspill, rn, num ? C.rp : nullptr, num ? C.col : nullptr,
                                  cv, rs, cap, &d()->err, s, f32);
@@ -1171,27 +1288,40 @@ struct FusedPlan {
     }
 
     // Bins of at most kMergeRows rows are folded into the next larger non-empty
-    // bin (its tables hold their upper bounds too; chains of small bins move up
-    // together): a launch of a handful of rows costs a whole row's latency,
-    // while the appended rows fill groups of the larger launch that hold one
-    // row fewer than the rest.  Updates cnt; the device lists and counts follow
-    // with launch_merge_bin.
-    std::vector<std::pair<int, int>> merges;
-    void plan_merges()
-    {
-        merges.clear();
-        for (int b = 0; b < thr.nb; b++) {
-            if (cnt[b] == 0 || cnt[b] > kMergeRows) continue;
-            int to = b + 1;
-            while (to < thr.nb && cnt[to] == 0) to++;
-            if (to >= thr.nb) continue;
-            merges.push_back({b, to});
-            cnt[to] += cnt[b];
-            cnt[b] = 0;
-        }
-    }
+    // bin while the binning writes its lists (its tables hold their upper
+    // bounds too): a launch of a handful of rows costs a whole row's latency,
+    // while the appended rows fill groups of the larger launch.  d()->counts
+    // holds the merged populations, d()->rawc the populations before.
     static constexpr int kMergeRows = 1024;
 
+    // seq > 0: the scatter kernel also exports the control block to the
+    // coherent host block with that sequence number (see export_ctl)
+    void launch_binning(Context *c, hipStream_t s, unsigned long long seq = 0)
+    {
+        void *host = seq ? c->poll_dev : nullptr;
+        auto *flag = seq ? reinterpret_cast<unsigned long long *>(c->poll_dev + kPollFlagOff) : nullptr;
+        launch_ub_bin(r(), a(), ub.as<int64_t>(), rown.as<int32_t>(), lists.as<int32_t>(), d()->rawc, thr,
+                      d()->totals, binscr.as<int>(), s, kMergeRows, d()->counts, d(), host,
+                      (int)(sizeof(Ctl) / 4), flag, seq);
+    }
+
+    // R*(A*P) instead of the direct forwarding: when at least half of R's
+    // rows are long rows or rows of the multi-wavefront bins, and P's columns
+    // do not fit one LDS accumulator of the long-row window kernel.  Such
+    // rows forward thousands of R*A entries into short rows of P, and past one
+    // LDS window every window re-walks all of them; in R*(A*P) a row walks
+    // nnz(R_r) rows of A*P instead, each a long contiguous run per window
+    // (k_spgemm_window), and the rows of A*P fit the hash tables.  Same
+    // pattern (structural products are associative), FP64 sums in another
+    // order.
+    bool wants_reassoc(Context *c) const
+    {
+        if (mixed || f32) return false;
+        int64_t big = nlong;
+        for (int i = 0; i < thr.nb; i++)
+            if (cfg[i].G > kWave) big += cnt[i];
+        return 2 * big >= R.M && R.M > 0 && P.N > (int64_t)window_cap(c->max_lds - 1024);
+    }
 
     int build(Context *c, Timer &tm)
     {
@@ -1214,29 +1344,60 @@ struct FusedPlan {
             return -1;
         }
         Ctl h;
-        const int e0 = tm.mark();
+        const int e0 = mk(tm);
         SP3MM_HIP(hipMemsetAsync(d(), 0, sizeof(Ctl), s));
-        launch_ub_bin(r(), a(), ub.as<int64_t>(), rown.as<int32_t>(), lists.as<int32_t>(), d()->counts, thr,
-                      d()->totals, binscr.as<int>(), s);
-        const int eu = tm.mark();
-        if (read_ctl(c, d(), h)) return -1;
+        // the populations go to the host from the scatter kernel, while the
+        // likely bin already runs behind it
+        SP3MM_HIP(hipGetLastError());
+        const unsigned long long seq = c->poll_host ? ++c->poll_seq : 0;
+        launch_binning(c, s, seq);
+        const int eu = mk(tm);
+        slog1 = sub_log(A, 64);
+        slog2 = sub_log(P, 64);
+        dlog1 = sub_log(A, kDenseThreads);
+        dlog2 = sub_log(P, kDenseThreads);
+        // the likely bin starts before the populations are read back, so the
+        // GPU does not idle through the read-back
+        pred = predict_bin(c);
+        int en0 = -1;
+        if (pred >= 0) {
+            const size_t cap = (size_t)M * cfg[pred].H2;
+            pcol = Buf(cap * 4);
+            pval = Buf(cap * 8);
+            pslotn = Buf((size_t)M * 4);
+            if (!pcol.p || !pval.p || !pslotn.p) {
+                (void)hipGetLastError();
+                pcol = Buf();
+                pval = Buf();
+                pslotn = Buf();
+                pred = -1;
+            } else {
+                en0 = mk(tm);
+                launch_main(c, s, pred);
+            }
+        }
+        if (wait_ctl(c, d(), seq, h)) return -1;
         mac1 = (long long)h.totals[0];
         cnt.assign(h.counts, h.counts + thr.nb + 1);
         nlong = cnt[thr.nb];
-        plan_merges();
-        for (const auto &m : merges) launch_merge_bin(lists.as<int32_t>(), d()->counts, m.first, m.second, M, s);
+        if (reassoc_ok && wants_reassoc(c)) {
+            reassoc = true;
+            return 0;
+        }
         soff.assign(thr.nb + 1, 0);
         nbase.assign(thr.nb + 1, 0);
-        int64_t ntot = 0;
+        int64_t ntot = 0, sown = 0;
         stot = 0;
         for (int i = 0; i < thr.nb; i++) {
-            soff[i] = stot;
+            stot += (int64_t)cnt[i] * cfg[i].H2;  // slot entries of every row (bounds nnz)
+            if (i == pred) continue;
+            soff[i] = sown;
             nbase[i] = ntot;
-            stot += (int64_t)cnt[i] * cfg[i].H2;
+            sown += (int64_t)cnt[i] * cfg[i].H2;
             ntot += cnt[i];
         }
-        scol = Buf((size_t)(stot ? stot : 1) * 4);
-        sval = Buf((size_t)(stot ? stot : 1) * 8);
+        scol = Buf((size_t)(sown ? sown : 1) * 4);
+        sval = Buf((size_t)(sown ? sown : 1) * 8);
         if (!scol.p || !sval.p) {
             set_error("scratch allocation (%lld entries) failed", (long long)stot);
             return -1;
@@ -1250,16 +1411,20 @@ struct FusedPlan {
             lval = Buf((size_t)nlong * lcap * 8);
             lslots = lslotn.p && lcol.p && lval.p;
         }
-        slog1 = sub_log(A, 64);
-        slog2 = sub_log(P, 64);
-        dlog1 = sub_log(A, kDenseThreads);
-        dlog2 = sub_log(P, kDenseThreads);
-        const int en0 = tm.mark();
+        if (en0 < 0) en0 = mk(tm);
         for (int i = 0; i < thr.nb; i++)
-            if (cnt[i]) launch_main(c, s, i);
-        const int en1 = tm.mark();
+            if (cnt[i] && i != pred) launch_main(c, s, i);
+        const int en1 = mk(tm);
         // long and retried rows: count pass of the window kernel (it lists the
         // rows whose windows exceed LDS), then the global dense count for those
+        // Without long rows every row that stayed on the hash tables fits its
+        // scratch slot, so the slot total is a likely output size: allocate it
+        // and run the tail without reading the counters back.  Every writer
+        // checks the capacity; if a row was retried, a retried row outgrew the
+        // output, or a row spilled to the global dense kernel, the exact path
+        // below redoes the tail.
+        const bool speculate = nlong == 0 && !mixed && stot > 0 && stot < INT32_MAX;
+        defer_retry = speculate;
         launch_long(c, s, false);
         if (ext_out) {
             C.M = M;
@@ -1271,12 +1436,7 @@ struct FusedPlan {
             return -1;
         }
         launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d()->nnz, s);
-        // Without long rows every row that stayed on the hash tables fits its
-        // scratch slot, so the slot total is a likely output size: allocate it
-        // and run the tail without reading the counters back.  Every writer
-        // checks the capacity; if a retried row outgrew it, or a row spilled
-        // to the global dense kernel, the exact path below redoes the tail.
-        if (nlong == 0 && !mixed && stot > 0 && stot < INT32_MAX) {
+        if (speculate) {
             if (ext_out)
                 ccap = ext_cap;  // the caller's arrays; a product that outgrows them is caught below
             else if (alloc_out(c, stot))
@@ -1284,15 +1444,16 @@ struct FusedPlan {
             spec = true;
             launch_tail(c, s);
             spec = false;
+            defer_retry = false;
             if (read_ctl(c, d(), h)) return -1;
-            if (!h.err && h.nspill == 0 && h.nnz <= ccap) {
+            if (!h.err && h.nspill == 0 && h.nretry == 0 && h.nnz <= ccap) {
                 nretry = h.nretry;
                 nspill = 0;
                 built = true;
                 fcnt.assign(h.counts2, h.counts2 + thr.nb + 1);
                 nnz = h.nnz;
                 C.nnz = nnz;
-                const int es = tm.mark();
+                const int es = mk(tm);
                 ms_ub = tm.ms(e0, eu);
                 ms_num = tm.ms(en0, en1);
                 ms_scan = tm.ms(en1, es);
@@ -1303,6 +1464,12 @@ struct FusedPlan {
                 return -1;
             }
             SP3MM_HIP(hipMemsetAsync(&d()->err, 0, sizeof(int), s));
+            if (h.nretry > 0) {
+                // the retried rows' lengths (window count pass), then the scan again
+                nretry = h.nretry;
+                launch_long(c, s, false);
+                launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d()->nnz, s);
+            }
         }
         if (read_ctl(c, d(), h)) return -1;
         nretry = h.nretry;
@@ -1316,7 +1483,7 @@ struct FusedPlan {
             launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d()->nnz, s);
             if (read_ctl(c, d(), h)) return -1;
         }
-        const int es = tm.mark();
+        const int es = mk(tm);
         if (h.err) {
             set_error("direct product: %s", err_text(h.err));
             return -1;
@@ -1347,16 +1514,18 @@ struct FusedPlan {
     {
         const int64_t M = R.M;
         launch_fill_i32(reinterpret_cast<int32_t *>(d()), (int64_t)(sizeof(Ctl) / 4), 0, s);
-        launch_ub_bin(r(), a(), ub.as<int64_t>(), rown.as<int32_t>(), lists.as<int32_t>(), d()->counts, thr,
-                      d()->totals, binscr.as<int>(), s);
-        for (const auto &m : merges) launch_merge_bin(lists.as<int32_t>(), d()->counts, m.first, m.second, M, s);
+        launch_binning(c, s);
         for (int i = 0; i < thr.nb; i++)
             if (cnt[i]) launch_main(c, s, i);
         launch_long(c, s, false);
         launch_scan(rown.as<int32_t>(), M, C.rp, C.rp32, tiles.as<long long>(), &d()->nnz, s);
         launch_tail(c, s);
         launch_fixups(c, s, true);
-        launch_check_nnz(&d()->nnz, nnz, &d()->nretry, nretry, &d()->nspill, nspill, &d()->err, s);
+        BinCounts want{};
+        want.n = thr.nb + 1;
+        for (int i = 0; i <= thr.nb; i++) want.c[i] = cnt[i];
+        launch_check_nnz(&d()->nnz, nnz, &d()->nretry, nretry, &d()->nspill, nspill, &d()->err, s, d()->counts,
+                         &want);
     }
 
     // after a replay: did the device see the sizes this plan was built for?
@@ -1378,7 +1547,7 @@ struct FusedPlan {
 };
 
 static int rap_direct_fused(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, sp3mm_dmat &C,
-                            sp3mm_stats *st, int prec, double guard, double *rowsum)
+                            sp3mm_stats *st, int prec, double guard, double *rowsum, bool timing)
 {
     Context *c = ctx();
     Timer tm(c->stream, true);
@@ -1392,9 +1561,44 @@ static int rap_direct_fused(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3m
     plan.tu = c->tune;
     plan.guard = guard;
     plan.rowsum = rowsum;
+    plan.reassoc_ok = prec == SP3MM_PREC_FP64 && !rowsum;
+    plan.detail = timing;
     if (plan.build(c, tm)) {
         plan.release_all();
         return -1;
+    }
+    if (plan.reassoc) {
+        // R*(A*P): the one-pass SpGEMM twice, A*P kept in HBM
+        sp3mm_stats s1{}, s2{};
+        sp3mm_dmat AP;
+        memset(&AP, 0, sizeof AP);
+        const int en0 = tm.mark();
+        int rc = spgemm_any(A, P, AP, &s1, nullptr, nullptr);
+        if (!rc) rc = spgemm_any(R, AP, C, &s2, nullptr, nullptr);
+        const int64_t mid = AP.nnz;
+        dmat_release(AP);
+        if (rc) {
+            dmat_release(C);
+            return -1;
+        }
+        const int e1 = tm.mark();
+        (void)hipEventSynchronize(tm.ev[e1]);
+        if (st) {
+            memset(st, 0, sizeof *st);
+            st->ms_total = tm.ms(e0, e1);
+            st->ms_ub = tm.ms(e0, en0);
+            st->ms_numeric = tm.ms(en0, e1);
+            st->ms_main = st->ms_numeric;
+            st->mac1 = plan.mac1;
+            st->nnz_mid = mid;
+            st->nnz_out = C.nnz;
+            st->main_kernel_launches = 2;
+            st->guarded_entries = plan.nlong;
+            for (int i = 0; i < plan.thr.nb; i++)
+                if (plan.cfg[i].G > kWave) st->guarded_entries += plan.cnt[i];
+            st->reassociated = 1;
+        }
+        return 0;
     }
     const int e1 = tm.mark();
     (void)hipEventSynchronize(tm.ev[e1]);
@@ -1432,11 +1636,11 @@ int rap_direct(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, sp
     if (check_pair(R, A) || check_pair(A, P)) return -1;
     static const bool twopass = getenv("SP3MM_DIRECT_TWOPASS") != nullptr;
     if (symbolic_only || twopass) return rap_direct_twopass(R, A, P, C, st, timing, symbolic_only);
-    return rap_direct_fused(R, A, P, C, st, SP3MM_PREC_FP64, 0.0, nullptr);
+    return rap_direct_fused(R, A, P, C, st, SP3MM_PREC_FP64, 0.0, nullptr, timing);
 }
 
 int rap_direct_mixed(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, sp3mm_dmat &C, sp3mm_stats *st,
-                     int prec, double guard, double *rowsum)
+                     int prec, double guard, double *rowsum, bool timing)
 {
     Context *c = ctx();
     if (!c) return -1;
@@ -1457,7 +1661,7 @@ int rap_direct_mixed(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat 
         set_error("row sums are produced by the FP32-pair form only");
         return -1;
     }
-    return rap_direct_fused(R, A, P, C, st, prec, guard > 0 ? guard : 1e-2, rowsum);
+    return rap_direct_fused(R, A, P, C, st, prec, guard > 0 ? guard : 1e-2, rowsum, timing);
 }
 
 // The direct FP64 product of rap_direct without a single device allocation:
@@ -1869,7 +2073,7 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
         for (int k = 0; k <= reps; k++) {
             sp3mm_dmat C{};
             sp3mm_stats st{};
-            const int rc = rap_direct_fused(R, A, P, C, &st, prec, guard > 0 ? guard : 1e-2, nullptr);
+            const int rc = rap_direct_fused(R, A, P, C, &st, prec, guard > 0 ? guard : 1e-2, nullptr, false);
             dmat_release(C);
             if (rc) return -1;
             if (k > 0) v.push_back(st.ms_total);
@@ -1888,7 +2092,7 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
                 c->tune = w ? u : t;
                 sp3mm_dmat C{};
                 sp3mm_stats st{};
-                const int rc = rap_direct_fused(R, A, P, C, &st, prec, guard > 0 ? guard : 1e-2, nullptr);
+                const int rc = rap_direct_fused(R, A, P, C, &st, prec, guard > 0 ? guard : 1e-2, nullptr, false);
                 dmat_release(C);
                 if (rc) return -1;
                 (w ? vu : vt).push_back(st.ms_total);
@@ -1998,8 +2202,12 @@ static int batch_build(Context *c, Batch *b, bool graph)
     return 0;
 }
 
+// Packs `count` matrices block-diagonally into `out` (row and entry offsets
+// added on the GPU, column offsets coloff).  reuse: `out` already holds a
+// packing of matrices of the same shapes and nonzero counts; its arrays are
+// overwritten in place.
 static int pack(Context *c, int count, const sp3mm_dmat *const *ms, int vtype, sp3mm_dmat &out,
-                const std::vector<int64_t> &coloff)
+                const std::vector<int64_t> &coloff, bool reuse = false)
 {
     int64_t M = 0, N = 0, nnz = 0;
     std::vector<PackItem> items((size_t)count);
@@ -2015,7 +2223,17 @@ static int pack(Context *c, int count, const sp3mm_dmat *const *ms, int vtype, s
                   (long long)nnz);
         return -1;
     }
-    if (dmat_alloc(out, M, N, nnz, vtype)) return -1;
+    if (reuse) {
+        if (out.M != M || out.N != N || out.nnz != nnz) {
+            set_error("batch: operands differ in shape or nonzero count from the batch's (%lld x %lld, %lld nnz "
+                      "packed; %lld x %lld, %lld nnz given)",
+                      (long long)out.M, (long long)out.N, (long long)out.nnz, (long long)M, (long long)N,
+                      (long long)nnz);
+            return -1;
+        }
+    } else if (dmat_alloc(out, M, N, nnz, vtype)) {
+        return -1;
+    }
     Buf it(items.size() * sizeof(PackItem));
     if (!it.p) return -1;
     SP3MM_HIP(hipMemcpyAsync(it.p, items.data(), items.size() * sizeof(PackItem), hipMemcpyHostToDevice, c->stream));
@@ -2114,6 +2332,50 @@ int batch_autotune(void *h, int reps, Tuning &best, double &ms_best, double &ms_
     b->plan.release_all();
     b->plan = FusedPlan();
     return batch_build(c, b, graph);
+}
+
+// Copies the current entries of the batch's operands (the same count, shapes
+// and nonzero counts as at creation) into its packed operands; the next run()
+// recomputes every product from them.  A pattern change that moves a product
+// off the sizes the plan was built for is reported by run().
+int batch_repack(void *h, int count, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P)
+{
+    Context *c = ctx();
+    if (!c || !h) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    Batch *b = static_cast<Batch *>(h);
+    if (count != b->count) {
+        set_error("batch: %d problems given, the batch holds %d", count, b->count);
+        return -1;
+    }
+    const int vt = b->mixed ? SP3MM_F32 : SP3MM_F64;
+    std::vector<const sp3mm_dmat *> rs, as, ps;
+    std::vector<int64_t> offR, offA, offP;
+    int64_t kr = 0, ka = 0, kp = 0;
+    for (int i = 0; i < count; i++) {
+        if (R[i].M != b->M[i] || R[i].N != A[i].M || A[i].N != P[i].M) {
+            set_error("batch: problem %d has shapes that differ from the batch's", i);
+            return -1;
+        }
+        if (R[i].vtype != vt || A[i].vtype != vt || P[i].vtype != vt) {
+            set_error("batch: problem %d operands must be %s", i, b->mixed ? "FP32" : "FP64");
+            return -1;
+        }
+        rs.push_back(R + i);
+        as.push_back(A + i);
+        ps.push_back(P + i);
+        offR.push_back(kr);
+        offA.push_back(ka);
+        offP.push_back(kp);
+        kr += R[i].N;
+        ka += A[i].N;
+        kp += P[i].N;
+    }
+    SP3MM_HIP(hipStreamSynchronize(c->stream));
+    if (pack(c, count, rs.data(), vt, b->Rb, offR, true) || pack(c, count, as.data(), vt, b->Ab, offA, true) ||
+        pack(c, count, ps.data(), vt, b->Pb, offP, true))
+        return -1;
+    return 0;
 }
 
 int batch_run(void *h, bool use_graph, double *ms)

@@ -56,6 +56,14 @@ struct Context {
     // device address of the fixed scratch array of the allocation-free entry
     // point on this device (null if it could not be resolved)
     unsigned char *noalloc_base = nullptr;
+    // coherent pinned host block for counters the host polls while the
+    // stream goes on: the control block, then a flag at byte kPollFlagOff
+    // that k_export_flag sets to the sequence number of the export
+    unsigned char *poll_host = nullptr;
+    unsigned char *poll_dev = nullptr;
+    unsigned long long poll_seq = 0;
+    // HIP events kept for reuse by the timers of later calls
+    std::vector<hipEvent_t> ev_pool;
     // small pinned host mirror for counters read back between phases
     long long *host_small = nullptr;
     long long *host_small_dev = nullptr;  // its device-side address (kernels write it directly)
@@ -122,7 +130,7 @@ int spgemm(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &C, sp3mm_stats 
 int rap_direct(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, sp3mm_dmat &C,
                sp3mm_stats *st, bool timing, bool symbolic_only = false);
 int rap_direct_mixed(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, sp3mm_dmat &C, sp3mm_stats *st,
-                     int prec, double guard, double *rowsum);
+                     int prec, double guard, double *rowsum, bool timing = false);
 // R*A*P (FP64, direct path) into caller-owned device arrays, every scratch
 // buffer carved from the fixed device scratch array: no device allocation
 int rap_direct_noalloc(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int64_t *crp, int32_t *ccol,
@@ -131,6 +139,7 @@ void dmat_release(sp3mm_dmat &m);
 void *batch_create(int count, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P, int prec,
                    double guard, bool graph);
 int batch_run(void *b, bool use_graph, double *ms);
+int batch_repack(void *b, int count, const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P);
 // times candidate schedules on the batch's packed product, installs the
 // fastest and rebuilds the batch (plan and graph) with it
 int batch_autotune(void *b, int reps, Tuning &best, double &ms_best, double &ms_default);

@@ -67,7 +67,7 @@ __device__ __forceinline__ long long row_measure(int64_t r, const int64_t *__res
     // batches of kB entries: all their columns, then all their row bounds, are
     // loaded back to back (two memory round trips per batch); entries past
     // the row read its first entry and count nothing
-    constexpr int kB = 16;
+    constexpr int kB = 32;
     for (int64_t e = a0; e < a1; e += kB) {
         int k[kB];
 #pragma unroll
@@ -174,15 +174,86 @@ __global__ void __launch_bounds__(1024) k_bin_offsets(int *__restrict__ hist, in
     }
 }
 
+// Bins of at most `lim` rows fold into the next larger non-empty hash bin
+// (a launch of a handful of rows costs a whole row's latency, while appended
+// rows fill groups of the larger launch; chains of small bins move up
+// together).  From the bin populations raw[0..nb]: dest[b] is the bin whose
+// list takes bin b's rows, starting at position start[b]; cnt[b] the merged
+// populations.  The long bin nb never merges.
+__device__ __forceinline__ void merge_plan(const int *raw, int nb, int lim, int *dest, int *start, int *cnt)
+{
+    for (int b = 0; b <= nb; b++) {
+        cnt[b] = raw[b];
+        dest[b] = b;
+        start[b] = 0;
+    }
+    for (int b = 0; b < nb; b++) {
+        if (cnt[b] == 0 || cnt[b] > lim) continue;
+        int to = b + 1;
+        while (to < nb && cnt[to] == 0) to++;
+        if (to >= nb) continue;
+        for (int c = 0; c <= b; c++)
+            if (dest[c] == b) {
+                dest[c] = to;
+                start[c] += cnt[to];
+            }
+        cnt[to] += cnt[b];
+        cnt[b] = 0;
+    }
+}
+
+// merge_lim > 0: rows land in their merged bins (merge_plan over the
+// populations `raw` that k_bin_offsets produced), and workgroup 0 writes the
+// merged populations to `mcounts`
+// ex (optional): workgroup 0 also exports the control block ex.src (bin
+// populations final, merged ones included) into coherent host memory and
+// then raises the host flag, so the host learns the populations while the
+// scatter and the work queued behind it still run.
+struct CtlExport {
+    const uint32_t *src;
+    uint32_t *host;
+    int words;
+    unsigned long long *flag;
+    unsigned long long seq;
+};
+
 __global__ void __launch_bounds__(kBinThreads) k_bin_scatter(const int64_t *__restrict__ ub, const int32_t *__restrict__ meas1,
                                                               const int32_t *__restrict__ meas2, int64_t M,
                                                               const int *__restrict__ hist, int32_t *__restrict__ lists,
-                                                              BinThresholds thr)
+                                                              BinThresholds thr, const int *__restrict__ raw,
+                                                              int merge_lim, int *__restrict__ mcounts, CtlExport ex)
 {
     __shared__ int h[kMaxBins + 1];
+    __shared__ int mdest[kMaxBins + 1], mstart[kMaxBins + 1];
     const int nbin = thr.nb + 1;
     if ((int)threadIdx.x < nbin) h[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        int dest[kMaxBins + 1], start[kMaxBins + 1], cnt[kMaxBins + 1];
+        if (merge_lim > 0) {
+            merge_plan(raw, thr.nb, merge_lim, dest, start, cnt);
+        } else {
+            for (int b = 0; b < nbin; b++) {
+                dest[b] = b;
+                start[b] = 0;
+            }
+        }
+        for (int b = 0; b < nbin; b++) {
+            mdest[b] = dest[b];
+            mstart[b] = start[b];
+        }
+        if (merge_lim > 0 && blockIdx.x == 0)
+            for (int b = 0; b < nbin; b++) mcounts[b] = cnt[b];
+    }
     __syncthreads();
+    if (ex.host && blockIdx.x == 0) {
+        for (int i = threadIdx.x; i < ex.words; i += blockDim.x)
+            __hip_atomic_store(ex.host + i, ex.src[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __threadfence_system();
+            __hip_atomic_store(ex.flag, ex.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
     const int64_t r = (int64_t)blockIdx.x * kBinThreads + threadIdx.x;
     if (r >= M) return;
     long long u;
@@ -196,7 +267,7 @@ __global__ void __launch_bounds__(kBinThreads) k_bin_scatter(const int64_t *__re
     const int b = bin_of(u, thr);
     if (b < 0) return;
     const int rank = atomicAdd(h + b, 1);
-    lists[(int64_t)b * M + hist[(int64_t)b * gridDim.x + blockIdx.x] + rank] = (int32_t)r;
+    lists[(int64_t)mdest[b] * M + mstart[b] + hist[(int64_t)b * gridDim.x + blockIdx.x] + rank] = (int32_t)r;
 }
 
 // Upper bound split over column ranges of B: boff[k*G+g] .. boff[k*G+g+1] are
@@ -894,32 +965,62 @@ __device__ __forceinline__ void dense_expand(const int32_t *lcol, const LV *lval
     }
 }
 
-// sorted emission of the set bits (and values) of a workspace; returns the
-// count.  Entries beyond `cap` are counted but not written.
+// index (0..31) of the k-th set bit of x (k < popcount(x))
+__device__ __forceinline__ int nth_set_bit(unsigned x, int k)
+{
+    int b = 0;
+    for (int s = 16; s > 0; s >>= 1) {
+        const unsigned low = (b + s) >= 32 ? x : (x & ((1u << (b + s)) - 1u));
+        if (__popc(low) <= k) b += s;
+    }
+    return b;
+}
+
+// Sorted emission of the set bits (and values) of a workspace of nw words;
+// returns the count.  Entries beyond `cap` are counted but not written.
+// Every thread owns one word per pass; a wavefront's 64 words hold a run of
+// consecutive outputs, which the wavefront writes 64 entries per store
+// instruction: output j of the run goes to lane j % 64, which finds the word
+// holding it (binary search over the lanes' exclusive counts) and the bit
+// inside the word.  Emitted bits and values are cleared.
 __device__ __forceinline__ int dense_emit(unsigned *bits, double *vals, int nw, int32_t *outc, double *outv,
                                           long long cap, long long *sh, int c0 = 0)
 {
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, l = lane();
     long long base = 0;
     for (int w0 = 0; w0 < nw; w0 += kDense) {
         const int w = w0 + t;
-        unsigned x = w < nw ? ld_word(bits + w) : 0u;
+        const unsigned x = w < nw ? ld_word(bits + w) : 0u;
+        const int pc = __popc(x);
         long long total;
-        long long pos = base + block_exclusive_scan(__popc(x), sh, total);
-        if (x) bits[w] = 0u;
-        while (x) {
-            const int b = __ffs(x) - 1;
-            x &= x - 1;
-            const int c = w * 32 + b;
-            const bool in = pos < cap;
-            if (outc && in) outc[pos] = c0 + c;
-            if (vals) {
-                const double v = ld_val(vals + c);
-                vals[c] = 0.0;
-                if (outv && in) outv[pos] = v;
+        const long long pos = base + block_exclusive_scan(pc, sh, total);
+        const long long wfirst = __shfl(pos, 0);
+        const int wcount = (int)(__shfl(pos + pc, 63) - wfirst);
+        const int ex = (int)(pos - wfirst);  // this lane's first output within the wavefront's run
+        const int wbase = w - l;             // word of lane 0
+        for (int j0 = 0; j0 < wcount; j0 += 64) {
+            const int j = j0 + l;
+            // the last lane m with ex(m) <= j holds output j
+            int m = 0;
+            for (int st = 32; st > 0; st >>= 1) {
+                const int e = __shfl(ex, m + st);
+                if (m + st < 64 && e <= j) m += st;
             }
-            pos++;
+            const unsigned xm = __shfl(x, m);
+            const int em = __shfl(ex, m);
+            if (j < wcount) {
+                const int c = (wbase + m) * 32 + nth_set_bit(xm, j - em);
+                const long long o = wfirst + j;
+                const bool in = o < cap;
+                if (outc && in) outc[o] = c0 + c;
+                if (vals) {
+                    const double v = ld_val(vals + c);
+                    vals[c] = 0.0;
+                    if (outv && in) outv[o] = v;
+                }
+            }
         }
+        if (x) bits[w] = 0u;
         base += total;
     }
     __threadfence();
@@ -982,6 +1083,201 @@ __global__ void __launch_bounds__(kDense) k_dense_spgemm(SpOperand A, SpOperand 
             const int got = dense_emit(bits, vals, nw, ccol + off, cval + off, crp[r + 1] - off, sh);
             if (threadIdx.x == 0 && got != crp[r + 1] - off) atomicOr(err, kErrOverflow);
         }
+    }
+}
+
+// Long rows of pairwise SpMM C = A*B, walked window by window over B's
+// columns with every left entry's position kept between windows.
+//
+// Per row, the left entries are staged once in LDS (for the first
+// kWinEntries of a row): cursor into the right row, its end, and (numeric
+// pass) the left value; an entry beyond that re-derives its cursor by binary
+// search and its row bounds from global memory.  A window then costs one
+// round of loads per kWinStep positions of every entry: sub-groups of
+// kWinSub lanes each own one entry, every lane loads kWinVec positions
+// (cursor + sub + v*kWinSub) at once, the positions whose column falls inside
+// the window form a prefix of the sub-group's kWinVec*kWinSub positions (the
+// right row is sorted), and their count advances the cursor; a step with a
+// position past the window ends the entry's slice.
+//
+// Count pass (NUM false): presence bits only, one window spanning all of B's
+// columns (they fit in LDS as bits), the row length to rownnz[r].  Numeric
+// pass: windows of `wcols` columns with FP64 values (native ds_add_f64) and
+// presence bits in LDS, each window's part emitted in column order at
+// crp[r] + (entries of the earlier windows).  One 1024-thread workgroup per
+// row, persistent over the row list.
+constexpr int kWinEntries = 1024;
+constexpr int kWinSub = 16;
+constexpr int kWinVec = 4;
+constexpr int kWinStep = kWinSub * kWinVec;
+
+__device__ __forceinline__ int lower_col(const int32_t *__restrict__ col, int lo, int hi, int c)
+{
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (col[mid] < c) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// LDS bytes of k_spgemm_window<NUM> for windows of wcols columns
+__host__ __device__ constexpr size_t spgemm_window_lds(bool num, int wcols)
+{
+    return (num ? (size_t)wcols * 8 : 0) + (size_t)(wcols + 31) / 32 * 4 + (size_t)kWinEntries * (num ? 16 : 8);
+}
+
+template <bool NUM>
+__global__ void __launch_bounds__(kDense) k_spgemm_window(SpOperand A, SpOperand B, const int32_t *__restrict__ rows,
+                                                           const int *__restrict__ count, int wcols,
+                                                           int32_t *__restrict__ rownnz,
+                                                           const int64_t *__restrict__ crp,
+                                                           int32_t *__restrict__ ccol, double *__restrict__ cval,
+                                                           int *__restrict__ err)
+{
+    extern __shared__ __align__(16) unsigned char dsm[];
+    __shared__ long long sh[kDense / 64];
+    const int nwords = (wcols + 31) / 32;
+    double *wv = reinterpret_cast<double *>(dsm);                                    // NUM: wcols values
+    unsigned *wbits = reinterpret_cast<unsigned *>(dsm + (NUM ? (size_t)wcols * 8 : 0));  // nwords words
+    unsigned char *ep = dsm + (NUM ? (size_t)wcols * 8 : 0) + (size_t)nwords * 4;
+    double *elv = reinterpret_cast<double *>(ep);                                    // NUM: kWinEntries values
+    int *ecur = reinterpret_cast<int *>(ep + (NUM ? (size_t)kWinEntries * 8 : 0));   // kWinEntries cursors
+    int *eend = ecur + kWinEntries;                                                  // kWinEntries row ends
+    {
+        unsigned *z = reinterpret_cast<unsigned *>(dsm);
+        const int nz = (NUM ? wcols * 2 : 0) + nwords;
+        for (int w = threadIdx.x; w < nz; w += kDense) z[w] = 0u;
+        __syncthreads();
+    }
+    const int t = threadIdx.x;
+    const int sub = t & (kWinSub - 1);
+    constexpr int ng = kDense / kWinSub;
+    const int sh0 = lane() & ~(kWinSub - 1);  // this sub-group's bits of a wavefront ballot
+    const double *aval = static_cast<const double *>(A.val);
+    const double *bval = static_cast<const double *>(B.val);
+    const int n = *count;
+    const int ncol = (int)B.N;
+    const int span = NUM ? wcols : ncol;
+    __shared__ int lohi[2 * (kDense / 64)];
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const int r = rows[i];
+        const int64_t e0 = A.rp[r];
+        const int na = (int)(A.rp[r + 1] - e0);
+        // stage the entries; the numeric pass also finds the row's column
+        // range [lo, hi] from the first and last column of every right row
+        // and runs its windows over that range only
+        int lo = INT_MAX, hi = -1;
+        for (int e = t; e < na; e += kDense) {
+            const int k = A.col[e0 + e];
+            const int b0 = B.rp32[k], b1 = B.rp32[k + 1];
+            if (e < kWinEntries) {
+                ecur[e] = b0;
+                eend[e] = b1;
+                if constexpr (NUM) elv[e] = aval[e0 + e];
+            }
+            if (NUM && b1 > b0) {
+                lo = min(lo, B.col[b0]);
+                hi = max(hi, B.col[b1 - 1]);
+            }
+        }
+        if constexpr (NUM) {
+            for (int o = 32; o > 0; o >>= 1) {
+                lo = min(lo, __shfl_xor(lo, o));
+                hi = max(hi, __shfl_xor(hi, o));
+            }
+            if (lane() == 0) {
+                lohi[t / 64] = lo;
+                lohi[kDense / 64 + t / 64] = hi;
+            }
+        }
+        __syncthreads();
+        int c0 = 0, c1 = ncol;  // columns [c0, c1) in windows of `span`
+        if constexpr (NUM) {
+            lo = lohi[0];
+            hi = lohi[kDense / 64];
+            for (int w = 1; w < kDense / 64; w++) {
+                lo = min(lo, lohi[w]);
+                hi = max(hi, lohi[kDense / 64 + w]);
+            }
+            c0 = hi >= lo ? lo : 0;
+            c1 = hi >= lo ? hi + 1 : 0;
+        }
+        int64_t off = 0;
+        long long cap = 0;
+        if constexpr (NUM) {
+            off = crp[r];
+            cap = crp[r + 1] - off;
+        }
+        long long base = 0;
+        for (int w0 = c0; w0 < c1; w0 += span) {
+            const int w1 = min(c1, w0 + span);
+            // every sub-group runs the same number of entry rounds, so the
+            // ballots below see whole sub-groups
+            for (int eb = 0; eb < na; eb += ng) {
+                const int e = eb + t / kWinSub;
+                const bool ok = e < na;
+                int q = 0, qe = 0;
+                double lv = 0.0;
+                if (ok) {
+                    if (e < kWinEntries) {
+                        q = ecur[e];
+                        qe = eend[e];
+                        if constexpr (NUM) lv = elv[e];
+                    } else {
+                        const int k = A.col[e0 + e];
+                        qe = B.rp32[k + 1];
+                        q = w0 == c0 ? B.rp32[k] : lower_col(B.col, B.rp32[k], qe, w0);
+                        if constexpr (NUM) lv = aval[e0 + e];
+                    }
+                }
+                bool more = ok && q < qe;
+                while (ballot(more)) {
+                    int c[kWinVec];
+                    double bv[kWinVec];
+#pragma unroll
+                    for (int v = 0; v < kWinVec; v++) {
+                        const int qq = q + v * kWinSub + sub;
+                        const bool valid = more && qq < qe;
+                        c[v] = valid ? B.col[qq] : INT_MAX;
+                        if constexpr (NUM) bv[v] = valid ? bval[qq] : 0.0;
+                    }
+                    int cnt = 0;
+#pragma unroll
+                    for (int v = 0; v < kWinVec; v++) {
+                        const bool in = c[v] < w1;
+                        if (in) {
+                            const int cw = c[v] - w0;
+                            atomicOr(wbits + (cw >> 5), 1u << (cw & 31));
+                            if constexpr (NUM) atomicAdd(wv + cw, lv * bv[v]);
+                        }
+                        cnt += __popcll((ballot(in) >> sh0) & ((1ull << kWinSub) - 1));
+                    }
+                    q += cnt;
+                    more = more && cnt == kWinStep;
+                }
+                if (ok && sub == 0 && e < kWinEntries) ecur[e] = q;
+            }
+            __syncthreads();
+            if constexpr (NUM) {
+                base += dense_emit(wbits, wv, (w1 - w0 + 31) / 32, ccol + off + base, cval + off + base,
+                                   cap - base, sh, w0);
+            } else {
+                long long pc = 0;
+                for (int w = t; w < nwords; w += kDense) {
+                    pc += __popc(wbits[w]);
+                    wbits[w] = 0u;
+                }
+                base += block_sum_dense(pc, sh);
+                __syncthreads();
+            }
+        }
+        if constexpr (NUM) {
+            if (t == 0 && base != cap) atomicOr(err, kErrOverflow);
+        } else {
+            if (t == 0) rownnz[r] = (int32_t)base;
+        }
+        __syncthreads();
     }
 }
 
@@ -1236,6 +1532,21 @@ __global__ void k_export_words(const uint32_t *__restrict__ src, uint32_t *dst, 
     for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
 
+// The same into coherent pinned host memory, then `seq` into *flag with a
+// system-scope release: a host thread polling the flag may read the block as
+// soon as the flag shows seq, while later work of the stream goes on.
+__global__ void k_export_flag(const uint32_t *__restrict__ src, uint32_t *dst, int n, unsigned long long *flag,
+                              unsigned long long seq)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        __hip_atomic_store(dst + i, src[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence_system();
+        __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1258,7 +1569,8 @@ int64_t bin_scratch_ints(int64_t M) { return (kMaxBins + 1) * ((M + kBinThreads 
 
 static void launch_bins(const SpOperand *A, const SpOperand *B, const int32_t *m1, const int32_t *m2, int64_t M,
                         int64_t *ub, int32_t *rownnz, int32_t *lists, int *counts, const BinThresholds &thr,
-                        unsigned long long *totals, int *scratch, hipStream_t st)
+                        unsigned long long *totals, int *scratch, hipStream_t st, int merge_lim = 0,
+                        int *mcounts = nullptr, const CtlExport *ex = nullptr)
 {
     if (M == 0) return;
     const int nblk = (int)((M + kBinThreads - 1) / kBinThreads);
@@ -1269,14 +1581,20 @@ static void launch_bins(const SpOperand *A, const SpOperand *B, const int32_t *m
                        B ? B->rp32 : nullptr, m1, m2, M, ub, rownnz, hist, part, thr);
     hipLaunchKernelGGL(k_bin_offsets, dim3(thr.nb + 2), dim3(1024), 0, st, hist, nblk, thr.nb + 1, part, counts,
                        totals);
-    hipLaunchKernelGGL(k_bin_scatter, dim3(nblk), dim3(kBinThreads), 0, st, ub, m1, m2, M, hist, lists, thr);
+    CtlExport e{};
+    if (ex) e = *ex;
+    hipLaunchKernelGGL(k_bin_scatter, dim3(nblk), dim3(kBinThreads), 0, st, ub, m1, m2, M, hist, lists, thr, counts,
+                       merge_lim, mcounts, e);
 }
 
 void launch_ub_bin(const SpOperand &A, const SpOperand &B, int64_t *ub, int32_t *rownnz,
                    int32_t *lists, int *counts, const BinThresholds &thr,
-                   unsigned long long *totals, int *scratch, hipStream_t st)
+                   unsigned long long *totals, int *scratch, hipStream_t st, int merge_lim, int *mcounts,
+                   const void *ctl, void *ctl_host, int ctl_words, unsigned long long *flag, unsigned long long seq)
 {
-    launch_bins(&A, &B, nullptr, nullptr, A.M, ub, rownnz, lists, counts, thr, totals, scratch, st);
+    CtlExport e{static_cast<const uint32_t *>(ctl), static_cast<uint32_t *>(ctl_host), ctl_words, flag, seq};
+    launch_bins(&A, &B, nullptr, nullptr, A.M, ub, rownnz, lists, counts, thr, totals, scratch, st, merge_lim,
+                mcounts, ctl_host ? &e : nullptr);
 }
 
 void launch_ub_colparts(const int64_t *arp, const int32_t *acol, int64_t M, const int64_t *boff, int G,
@@ -1367,6 +1685,20 @@ void launch_direct(int G, bool num, int H1, int H2, int slog1, int slog2, const 
 
 int dense_words(int64_t ncols) { return (int)((ncols + 31) / 32) + 1; }
 
+// widest power-of-two column window whose values and presence bits, with the
+// staged entries of k_spgemm_window, fit in `max_lds` bytes (0: none)
+int spgemm_window_cols(size_t max_lds)
+{
+    for (int w = 1 << 20; w >= 1024; w >>= 1)
+        if (spgemm_window_lds(true, w) <= max_lds) return w;
+    return 0;
+}
+
+bool spgemm_dense_in_lds(bool num, int nw, size_t max_lds)
+{
+    return (size_t)nw * 4 + (num ? (size_t)nw * 32 * 8 : 0) <= max_lds;
+}
+
 void launch_dense_spgemm(bool num, int blocks, const SpOperand &A, const SpOperand &B, const int32_t *rows,
                          const int *count, unsigned *bits, double *vals, int nw, int slog, int32_t *rownnz,
                          const int64_t *crp, int32_t *ccol, double *cval, int *err, hipStream_t st,
@@ -1374,6 +1706,21 @@ void launch_dense_spgemm(bool num, int blocks, const SpOperand &A, const SpOpera
 {
     const size_t lds = (size_t)nw * 4 + (num ? (size_t)nw * 32 * 8 : 0);
     const bool in_lds = lds <= max_lds;
+    if (!num && spgemm_window_lds(false, (int)B.N) <= max_lds) {
+        // count pass: presence bits over all of B's columns, staged entries
+        hipLaunchKernelGGL(k_spgemm_window<false>, dim3(blocks), dim3(kDense), spgemm_window_lds(false, (int)B.N), st,
+                           A, B, rows, count, (int)B.N, rownnz, crp, ccol, cval, err);
+        return;
+    }
+    if (num && !in_lds) {
+        // values beyond LDS: windows of B's columns in LDS, when one fits
+        const int wcols = spgemm_window_cols(max_lds);
+        if (wcols > 0) {
+            hipLaunchKernelGGL(k_spgemm_window<true>, dim3(blocks), dim3(kDense), spgemm_window_lds(true, wcols), st,
+                               A, B, rows, count, wcols, rownnz, crp, ccol, cval, err);
+            return;
+        }
+    }
 #define SP3MM_DSPG_ARGS A, B, rows, count, bits, vals, nw, slog, rownnz, crp, ccol, cval, err
     if (num && in_lds)
         hipLaunchKernelGGL((k_dense_spgemm<true, true>), dim3(blocks), dim3(kDense), lds, st, SP3MM_DSPG_ARGS);
@@ -1807,39 +2154,28 @@ void launch_gather_rows(const int32_t *rows, const int *count, int rows_host, co
                            ccol, cval, out_cap, err, rows_host);
 }
 
+// A replay computed the sizes its plan was built for: nnz, retried and
+// spilled rows, and (counts != null) every bin's population, whose launches
+// and scratch slots the plan fixed.  Anything else is reported.
 __global__ void k_check_nnz(const int64_t *nnz, int64_t want, const int *nretry, int want_retry,
-                            const int *nspill, int want_spill, int *err)
+                            const int *nspill, int want_spill, const int *counts, BinCounts want_counts, int *err)
 {
-    if (threadIdx.x == 0 && (*nnz != want || *nretry != want_retry || *nspill != want_spill))
-        atomicOr(err, kErrOverflow);
+    if (threadIdx.x != 0) return;
+    bool bad = *nnz != want || *nretry != want_retry || *nspill != want_spill;
+    if (counts)
+        for (int b = 0; b < want_counts.n; b++) bad = bad || counts[b] != want_counts.c[b];
+    if (bad) atomicOr(err, kErrOverflow);
 }
 
 void launch_check_nnz(const int64_t *nnz, int64_t want, const int *nretry, int want_retry, const int *nspill,
-                      int want_spill, int *err, hipStream_t st)
+                      int want_spill, int *err, hipStream_t st, const int *counts, const BinCounts *want_counts)
 {
+    BinCounts wc{};
+    if (want_counts) wc = *want_counts;
     hipLaunchKernelGGL(k_check_nnz, dim3(1), dim3(64), 0, st, nnz, want, nretry, want_retry, nspill, want_spill,
-                       err);
+                       want_counts ? counts : nullptr, wc, err);
 }
 
-// Appends bin `from`'s row list to bin `to`'s (lists of M entries per bin) and
-// moves its count: the rows then run in `to`'s launch, whose tables are larger.
-__global__ void k_merge_bin(int32_t *__restrict__ lists, int *__restrict__ counts, int from, int to, int64_t M)
-{
-    const int n = counts[from], base = counts[to];
-    const int32_t *src = lists + (int64_t)from * M;
-    int32_t *dst = lists + (int64_t)to * M + base;
-    for (int j = threadIdx.x; j < n; j += blockDim.x) dst[j] = src[j];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        counts[to] = base + n;
-        counts[from] = 0;
-    }
-}
-
-void launch_merge_bin(int32_t *lists, int *counts, int from, int to, int64_t M, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_merge_bin, dim3(1), dim3(1024), 0, st, lists, counts, from, to, M);
-}
 
 // ---------------------------------------------------------------------------
 // batches: block-diagonal packing of many operands, and extraction of one block
@@ -1954,6 +2290,13 @@ void launch_export_words(const void *src, void *host_dst, int words, hipStream_t
 {
     hipLaunchKernelGGL(k_export_words, dim3(1), dim3(64), 0, st, static_cast<const uint32_t *>(src),
                        static_cast<uint32_t *>(host_dst), words);
+}
+
+void launch_export_flag(const void *src, void *host_dst, int words, unsigned long long *flag, unsigned long long seq,
+                        hipStream_t st)
+{
+    hipLaunchKernelGGL(k_export_flag, dim3(1), dim3(64), 0, st, static_cast<const uint32_t *>(src),
+                       static_cast<uint32_t *>(host_dst), words, flag, seq);
 }
 
 void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st)

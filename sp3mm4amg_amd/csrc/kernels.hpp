@@ -26,9 +26,18 @@ struct BinThresholds {
 
 // binning scratch: bin_scratch_ints(M) ints of device memory
 int64_t bin_scratch_ints(int64_t M);
+// Upper bounds of A*B's rows and bin lists.  counts[] accumulates the bin
+// populations.  merge_lim > 0: bins of at most merge_lim rows fold into the
+// next larger non-empty hash bin while the lists are written, and mcounts[]
+// receives the merged populations (the lists are laid out by them).
+// ctl_host != null: the scatter kernel also copies ctl_words words of the
+// control block `ctl` (populations and totals final by then) into coherent
+// host memory at ctl_host and then writes seq to *flag (host-mapped).
 void launch_ub_bin(const SpOperand &A, const SpOperand &B, int64_t *ub, int32_t *rownnz,
                    int32_t *lists, int *counts, const BinThresholds &thr,
-                   unsigned long long *totals, int *scratch, hipStream_t st);
+                   unsigned long long *totals, int *scratch, hipStream_t st, int merge_lim = 0,
+                   int *mcounts = nullptr, const void *ctl = nullptr, void *ctl_host = nullptr, int ctl_words = 0,
+                   unsigned long long *flag = nullptr, unsigned long long seq = 0);
 void launch_ub_colparts(const int64_t *arp, const int32_t *acol, int64_t M, const int64_t *boff, int G,
                         int64_t *out, unsigned long long *total, hipStream_t st);
 void launch_rebin(const int32_t *m1, const int32_t *m2, int64_t M, int32_t *lists, int *counts,
@@ -64,6 +73,12 @@ struct DenseWs {
 };
 
 int dense_words(int64_t ncols);
+// Long rows of A*B: one 1024-thread workgroup per row.  Bitmap (and, in the
+// numeric pass, values) over B's columns in LDS when they fit in max_lds, else
+// in the HBM workspace `bits`/`vals`; a numeric pass whose values do not fit
+// runs in LDS column windows instead (k_spgemm_window) and needs no workspace.
+bool spgemm_dense_in_lds(bool num, int nw, size_t max_lds);
+int spgemm_window_cols(size_t max_lds);
 void launch_dense_spgemm(bool num, int blocks, const SpOperand &A, const SpOperand &B, const int32_t *rows,
                          const int *count, unsigned *bits, double *vals, int nw, int slog, int32_t *rownnz,
                          const int64_t *crp, int32_t *ccol, double *cval, int *err, hipStream_t st,
@@ -112,8 +127,14 @@ void launch_spgemm_fused(int G, int H, int slog, const SpOperand &A, const SpOpe
 void launch_gather_rows(const int32_t *rows, const int *count, int rows_host, const int32_t *slotn, int cap,
                         const int32_t *scol, const double *sval, const int64_t *crp, int32_t *ccol, double *cval,
                         int64_t out_cap, int *err, int max_blocks, hipStream_t st);
+// populations of the bins a plan launched (bins 0..n-1, n <= kMaxBins + 1)
+struct BinCounts {
+    int n;
+    int c[kMaxBins + 1];
+};
 void launch_check_nnz(const int64_t *nnz, int64_t want, const int *nretry, int want_retry, const int *nspill,
-                      int want_spill, int *err, hipStream_t st);
+                      int want_spill, int *err, hipStream_t st, const int *counts = nullptr,
+                      const BinCounts *want_counts = nullptr);
 
 size_t mixed_lds(int G, int H1, int H2);
 void launch_direct_mixed(int G, int H1, int H2, int slog1, int slog2, float guard, const SpOperand &R,
@@ -152,8 +173,6 @@ void launch_fingerprint(const int64_t *rp, int64_t M, const int32_t *col, int64_
                         hipStream_t st);
 // *out = OR of the n device words *words[i] (one wavefront)
 void launch_or_words(const int *const *words, int n, int *out, hipStream_t st);
-// appends bin `from`'s row list to bin `to`'s and moves its count (one workgroup)
-void launch_merge_bin(int32_t *lists, int *counts, int from, int to, int64_t M, hipStream_t st);
 // setup.hip: transpose and smoothed-aggregation prolongator
 void launch_col_count(const int32_t *col, int64_t nnz, int32_t *cnt, hipStream_t st);
 void launch_transpose_scatter(const int64_t *rp, const int32_t *col, const void *val, bool f32, int64_t M,
@@ -181,6 +200,10 @@ int dot_parts();
 void launch_jacobi(const int64_t *rp, const int32_t *col, const double *val, int64_t M, int64_t nnz, const double *b,
                    const double *xi, double *xo, double omega, int *err, hipStream_t st);
 void launch_export_words(const void *src, void *host_dst, int words, hipStream_t st);
+// copies `words` words into coherent pinned host memory, then writes `seq` to
+// *flag (host-mapped) with a system-scope release
+void launch_export_flag(const void *src, void *host_dst, int words, unsigned long long *flag, unsigned long long seq,
+                        hipStream_t st);
 void launch_copy_probe(const void *in, void *out, int64_t bytes, hipStream_t st);
 
 }  // namespace sp3mm

@@ -61,10 +61,10 @@ class Stats(C.Structure):
                 ("mac1", C.c_int64), ("mac2", C.c_int64), ("nnz_mid", C.c_int64),
                 ("nnz_out", C.c_int64), ("guarded_rows", C.c_int64),
                 ("guarded_entries", C.c_int64), ("main_kernel_launches", C.c_int32),
-                ("spilled_rows", C.c_int32)]
+                ("spilled_rows", C.c_int32), ("reassociated", C.c_int32), ("reserved2", C.c_int32)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
 
 
 class Tuning(C.Structure):
@@ -144,6 +144,7 @@ def lib():
         L.sp3mm_batch_create.argtypes = [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, P(Opts), C.c_int32,
                                          P(C.c_void_p)]
         L.sp3mm_batch_run.argtypes = [C.c_void_p, C.c_int32, P(C.c_double)]
+        L.sp3mm_batch_repack.argtypes = [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.sp3mm_batch_result.argtypes = [C.c_void_p, C.c_int32, P(DMat)]
         L.sp3mm_batch_info.argtypes = [C.c_void_p, P(C.c_double), P(C.c_double), P(C.c_int64), P(C.c_int32)]
         L.sp3mm_batch_free.argtypes = [C.c_void_p]
@@ -376,6 +377,17 @@ class Batch:
         ms = C.c_double()
         _check(lib().sp3mm_batch_run(self.h, int(graph), C.byref(ms)), "sp3mm_batch_run")
         return ms.value
+
+    def repack(self, Rs=None, As=None, Ps=None) -> None:
+        """Copy the operands' current entries into the batch (by default the
+        matrices given at creation, whose device arrays the caller may have
+        rewritten); shapes and nonzero counts must be unchanged."""
+        n = self.count
+        arr = DMat * n
+        R = arr(*[m.d for m in Rs]) if Rs is not None else self._R
+        A = arr(*[m.d for m in As]) if As is not None else self._A
+        P = arr(*[m.d for m in Ps]) if Ps is not None else self._P
+        _check(lib().sp3mm_batch_repack(self.h, n, R, A, P), "sp3mm_batch_repack")
 
     def result(self, i: int) -> DeviceCSR:
         d = DMat()

@@ -453,7 +453,7 @@ def _row_subset(R: CSR, rows: np.ndarray) -> CSR:
     return CSR(len(rows), R.N, np.array(irp, dtype=np.int64), np.concatenate(ja), np.concatenate(va))
 
 
-def _irregular_sampled(n):
+def _irregular_sampled(n, reassociated=None):
     """Lognormal irregular operator with random aggregation (the irregular
     workload): full product on the GPU, 40 rows (first, last, random) against
     the oracle's R[rows]*A*P, patterns exact, values within the error model."""
@@ -462,7 +462,10 @@ def _irregular_sampled(n):
     print(f"irregular {n}: R {R.shape} nnz {R.nnz}", flush=True)
     d = [L.DeviceCSR.upload(m) for m in (R, A, P)]
     out, st = L.rap(*d, path=L.DIRECT)
-    print(f"  nnz {st['nnz_out']} off-table rows {st['guarded_entries']} spilled {st['spilled_rows']}", flush=True)
+    print(f"  nnz {st['nnz_out']} off-table rows {st['guarded_entries']} spilled {st['spilled_rows']} "
+          f"reassociated {st['reassociated']} ({st['ms_total']:.1f} ms)", flush=True)
+    if reassociated is not None:
+        assert st["reassociated"] == reassociated
     rows = np.unique(np.concatenate([[0, R.M - 1], np.random.default_rng(7).choice(R.M, 38, replace=False)]))
     got = _sampled_rows(out, rows)
     out.free()
@@ -474,21 +477,57 @@ def _irregular_sampled(n):
 
 
 def test_irregular_100k_sampled():
-    """P's ~12.5K columns fit in LDS: the dense rows' second stage runs there."""
-    _irregular_sampled(100_000)
+    """P's ~12.5K columns fit in LDS: the dense rows' second stage runs there
+    (direct forwarding, no reassociation)."""
+    _irregular_sampled(100_000, reassociated=0)
 
 
 def test_irregular_200k_sampled():
-    """P's ~25K columns exceed LDS: the dense rows' second stage runs in two
-    column windows (k_dense_window)."""
-    _irregular_sampled(200_000)
+    """P's ~25K columns exceed an LDS accumulator and nearly every row is
+    beyond the one-wavefront tables: the direct product runs as R*(A*P), its
+    long rows in LDS column windows (k_spgemm_window)."""
+    _irregular_sampled(200_000, reassociated=1)
+
+
+def _random_csr(rng, M, N, lens):
+    rows = np.repeat(np.arange(M), lens)
+    cols = np.concatenate([np.sort(rng.choice(N, int(l), replace=False)) for l in lens])
+    return from_coo(M, N, rows, cols, rng.uniform(-1.0, 1.0, rows.size))
+
+
+def test_spgemm_window_long_wide_rows():
+    """SpGEMM rows beyond the hash tables whose output is wider than one LDS
+    accumulator: the numeric pass runs in LDS column windows of 16K columns
+    (k_spgemm_window; 60,000 columns = 4 windows), including a row of 5,000
+    left entries, whose cursors beyond the 4,096 kept in LDS are found by
+    binary search.  Pattern exact, values within the rounding-error model."""
+    rng = np.random.default_rng(21)
+    M, K, N = 48, 8000, 60000
+    la = rng.integers(150, 400, M)
+    la[3] = 5000
+    A = _random_csr(rng, M, K, la)
+    B = _random_csr(rng, K, N, rng.integers(20, 60, K))
+    dA, dB = L.DeviceCSR.upload(A), L.DeviceCSR.upload(B)
+    try:
+        out, st = L.spgemm(dA, dB)
+        got = out.download()
+        out.free()
+    finally:
+        dA.free()
+        dB.free()
+    want = O.spmm(A, B)
+    scale = O.spmm(A.with_vals(np.abs(A.vals)), B.with_vals(np.abs(B.vals)))
+    T = O.ub(A, B)[:-1][want.row_ids()].astype(np.float64)
+    bound = 1e-12 * np.abs(want.vals) + 4.0 * T * 2.0 ** -53 * scale.vals
+    check(got, want, None, bound=bound)
 
 
 @pytest.mark.slow
 def test_irregular_1m_sampled():
     """The full-size irregular workload: 1,000,000 fine rows, ~1.1e10 output
-    nonzeros (about 140 GB of HBM for the output and workspaces)."""
-    _irregular_sampled(1_000_000)
+    nonzeros (about 140 GB of HBM for the output and workspaces), run as
+    R*(A*P) with LDS column windows."""
+    _irregular_sampled(1_000_000, reassociated=1)
 
 
 def test_autotune_mixed():
@@ -604,15 +643,53 @@ def test_batch_autotune_rebuilds(prec):
                 m.free()
 
 
-def test_batch_rejects_changed_pattern():
-    """A replay whose pattern no longer matches the plan is reported, not silently wrong."""
+@pytest.mark.parametrize("graph", [False, True])
+def test_batch_repack_values_and_changed_pattern(graph):
+    """repack() refreshes the batch's packed operands from the caller's
+    matrices: new values give the new product; a pattern change that alters
+    the product's size is reported by run(), not returned as a wrong result."""
     R, A, P = triple("poisson2d64")
-    dR, dA, dP = (L.DeviceCSR.upload(m) for m in (R, A, P))
-    b = L.Batch([dR], [dA], [dP], graph=False)
-    b.run(graph=False)
-    b.free()
-    for m in (dR, dA, dP):
-        m.free()
+    R2, A2, P2 = triple("poisson3d32")
+    d1 = [L.DeviceCSR.upload(m) for m in (R, A, P)]
+    d2 = [L.DeviceCSR.upload(m) for m in (R2, A2, P2)]
+    b = L.Batch([d1[0], d2[0]], [d1[1], d2[1]], [d1[2], d2[2]], graph=graph)
+    try:
+        b.run(graph=graph)
+        # new values of A in the first problem (same pattern)
+        rng = np.random.default_rng(11)
+        A_new = A.with_vals(A.vals * rng.uniform(0.5, 1.5, A.nnz))
+        dA_new = L.DeviceCSR.upload(A_new)
+        b.repack(As=[dA_new, d2[1]])
+        b.run(graph=graph)
+        for i, (r, a, p) in enumerate(((R, A_new, P), (R2, A2, P2))):
+            out = b.result(i)
+            check(out.download(), O.sp3mm_merged(r, a, p), O.abs_product(r, a, p))
+            out.free()
+        # a changed pattern: the last entry of row 65 of A (an interior point
+        # of the 64x64 grid, columns 1, 64, 65, 66, 129) moves to column 4000,
+        # far across the grid, keeping the row sorted and the nonzero count;
+        # R*A*P gains entries (12,676 -> 12,685 nonzeros)
+        e1 = int(A.irp[66])
+        ja = A.ja.copy()
+        assert ja[e1 - 1] == 129
+        ja[e1 - 1] = 4000
+        A_moved = CSR(A.M, A.N, A.irp, ja, A.vals)
+        A_moved.check()
+        want = O.sp3mm_merged(R, A_moved, P)
+        assert want.nnz != O.sp3mm_merged(R, A, P).nnz
+        dA_moved = L.DeviceCSR.upload(A_moved)
+        b.repack(As=[dA_moved, d2[1]])
+        with pytest.raises(RuntimeError, match="replayed product"):
+            b.run(graph=graph)
+        # shapes or nonzero counts that differ from the batch's are refused
+        with pytest.raises(RuntimeError, match="differ"):
+            b.repack(As=[d2[1], d1[1]])
+        dA_new.free()
+        dA_moved.free()
+    finally:
+        b.free()
+        for m in d1 + d2:
+            m.free()
 
 
 # ---------------------------------------------------------------------------
