@@ -385,11 +385,13 @@ struct Ctl {
     int counts[16];
     int counts2[16];
     int rawc[16];  // bin populations before small bins are merged (direct plans)
+    int pred_count;  // population seen by the launch queued before the read-back (0: abandoned)
+    int pad3;
     int nretry;
     int err;
     int nspill;  // long rows whose column windows exceed the LDS window kernel
     int pad;
-    unsigned long long totals[2];
+    unsigned long long totals[3];  // bound sum, bound maximum, rows of unstructured spread
     int64_t nnz;
     int64_t pad2;
 };
@@ -493,27 +495,14 @@ struct Timer {
 
 constexpr size_t kPollFlagOff = 4088;
 
-// Reads the control block back without draining the stream: the block and a
-// new sequence number are exported into the coherent host block, and the host
-// spins on the flag (work queued behind the export keeps the GPU busy
-// meanwhile).  A stream that ends without the flag (a failed launch or a
-// fault) is reported.  Falls back to read_ctl without the coherent block.
+// Reads the control block back with a stream synchronisation.
 static int read_ctl(Context *c, const Ctl *d, Ctl &h);
 
-// Queues the export of the control block into the coherent host block and
-// returns its sequence number (0: no coherent block; wait with read_ctl).
-static unsigned long long export_ctl(Context *c, const Ctl *d)
-{
-    static_assert(sizeof(Ctl) <= kPollFlagOff, "control block fits before the flag");
-    if (!c->poll_host) return 0;
-    const unsigned long long seq = ++c->poll_seq;
-    auto *flag_dev = reinterpret_cast<unsigned long long *>(c->poll_dev + kPollFlagOff);
-    launch_export_flag(d, c->poll_dev, (int)(sizeof(Ctl) / 4), flag_dev, seq, c->stream);
-    return seq;
-}
-
-// Waits for the export `seq` (queued by export_ctl, possibly with more work
-// queued behind it) and copies the block out; seq 0 reads back with a stream
+// Waits for the export `seq` of the control block into the coherent host
+// block (the binning's scatter kernel writes it and then raises the flag at
+// kPollFlagOff to seq; work queued behind it keeps the GPU busy meanwhile) and
+// copies the block out.  A stream that ends without the flag (a failed launch
+// or a fault) is reported.  seq 0 (no coherent block) reads back with a stream
 // synchronisation instead.
 static int wait_ctl(Context *c, const Ctl *d, unsigned long long seq, Ctl &h)
 {
@@ -538,12 +527,6 @@ static int wait_ctl(Context *c, const Ctl *d, unsigned long long seq, Ctl &h)
     }
     memcpy(&h, c->poll_host, sizeof(Ctl));
     return 0;
-}
-
-static int read_ctl_poll(Context *c, const Ctl *d, Ctl &h)
-{
-    SP3MM_HIP(hipGetLastError());
-    return wait_ctl(c, d, export_ctl(c, d), h);
 }
 
 static int read_ctl(Context *c, const Ctl *d, Ctl &h)
@@ -1194,8 +1177,9 @@ struct FusedPlan {
 
     // The bin whose launch can go ahead of the read-back of the populations:
     // the one the mean row bound nnz(R)/M * nnz(A)/A.M falls in, for FP64
-    // plans with their own output; never a multi-wavefront bin of a product
-    // that may be reassociated (its rows would be computed for nothing).
+    // plans with their own output.  Its launch reads its population from
+    // Ctl::pred_count, which the binning sets to 0 when the product is to be
+    // reassociated, so that launch then ends at once.
     int predict_bin(Context *c) const
     {
         if (mixed || ext_out || R.M == 0 || A.M == 0 || R.M >= INT32_MAX) return -1;
@@ -1203,14 +1187,15 @@ struct FusedPlan {
         int b = 0;
         while (b < thr.nb && (double)thr.v[b] < est) b++;
         if (b >= thr.nb) return -1;
-        if (cfg[b].G > kWave && reassoc_ok && P.N > (int64_t)window_cap(c->max_lds - 1024)) return -1;
+        (void)c;
         return b;
     }
 
-    void launch_main(Context *c, hipStream_t s, int i)
+    void launch_main(Context *c, hipStream_t s, int i, bool early = false)
     {
         const FusedCfg &f = cfg[i];
         const int64_t M = R.M;
+        const int *count = early ? &d()->pred_count : d()->counts + i;
         if (mixed)
             launch_direct_mixed(f.G, f.H1, f.H2, slog1, slog2, (float)guard, r(), a(), p(),
                                 lists.as<int32_t>() + (int64_t)i * M, d()->counts + i, cnt[i], rown.as<int32_t>(),
@@ -1219,7 +1204,7 @@ struct FusedPlan {
                                 fix.as<int32_t>() + nbase[i], d()->counts2 + i, &d()->err, max_blocks(c), s, tu);
         else
             launch_direct_fused(f.G, f.H1, f.H2, slog1, slog2, r(), a(), p(), lists.as<int32_t>() + (int64_t)i * M,
-                                d()->counts + i, slot_rows(i), rown.as<int32_t>(), slot_n(i), slot_col(i), slot_val(i),
+                                count, slot_rows(i), rown.as<int32_t>(), slot_n(i), slot_col(i), slot_val(i),
                                 retry.as<int32_t>(), &d()->nretry, &d()->err, max_blocks(c), s, f32, tu);
     }
 
@@ -1294,33 +1279,53 @@ struct FusedPlan {
     // holds the merged populations, d()->rawc the populations before.
     static constexpr int kMergeRows = 1024;
 
-    // seq > 0: the scatter kernel also exports the control block to the
-    // coherent host block with that sequence number (see export_ctl)
+    // The binning clears the control block first; with seq > 0 its scatter
+    // kernel also exports the block to the coherent host block with that
+    // sequence number (see wait_ctl).
     void launch_binning(Context *c, hipStream_t s, unsigned long long seq = 0)
     {
         void *host = seq ? c->poll_dev : nullptr;
         auto *flag = seq ? reinterpret_cast<unsigned long long *>(c->poll_dev + kPollFlagOff) : nullptr;
+        // rows of R whose columns spread over more than a quarter of A's rows
+        // count as unstructured (see wants_reassoc)
+        const int span_lim = reassoc_ok ? (int)std::max<int64_t>(1, R.N / 4) : 0;
+        int big_from = thr.nb;
+        for (int i = 0; i < thr.nb; i++)
+            if (cfg[i].G > kWave) {
+                big_from = i;
+                break;
+            }
         launch_ub_bin(r(), a(), ub.as<int64_t>(), rown.as<int32_t>(), lists.as<int32_t>(), d()->rawc, thr,
                       d()->totals, binscr.as<int>(), s, kMergeRows, d()->counts, d(), host,
-                      (int)(sizeof(Ctl) / 4), flag, seq);
+                      (int)(sizeof(Ctl) / 4), flag, seq, span_lim, &d()->pred_count, pred, big_from,
+                      reassoc_possible(c));
     }
 
     // R*(A*P) instead of the direct forwarding: when at least half of R's
-    // rows are long rows or rows of the multi-wavefront bins, and P's columns
-    // do not fit one LDS accumulator of the long-row window kernel.  Such
-    // rows forward thousands of R*A entries into short rows of P, and past one
-    // LDS window every window re-walks all of them; in R*(A*P) a row walks
-    // nnz(R_r) rows of A*P instead, each a long contiguous run per window
-    // (k_spgemm_window), and the rows of A*P fit the hash tables.  Same
-    // pattern (structural products are associative), FP64 sums in another
-    // order.
+    // rows are long rows or rows of the multi-wavefront bins, at least half
+    // of R's rows are unstructured (their columns spread over more than a
+    // quarter of A's rows: random aggregates rather than mesh neighbourhoods)
+    // and P's columns do not fit one LDS accumulator of the long-row window
+    // kernel.  Such rows forward thousands of R*A entries into short rows of
+    // P that land all over the output, and past one LDS window every window
+    // re-walks all of them; in R*(A*P) a row walks nnz(R_r) rows of A*P
+    // instead, each a long contiguous run per window (k_spgemm_window), and
+    // the rows of A*P fit the hash tables.  Mesh-structured products (whose
+    // output rows stay narrow) keep the direct forwarding.  Same pattern
+    // (structural products are associative), FP64 sums in another order.
+    unsigned long long wide = 0;  // unstructured rows of R (binning totals[2])
+    bool reassoc_possible(Context *c) const
+    {
+        return reassoc_ok && !mixed && !f32 && P.N > (int64_t)window_cap(c->max_lds - 1024);
+    }
     bool wants_reassoc(Context *c) const
     {
         if (mixed || f32) return false;
         int64_t big = nlong;
         for (int i = 0; i < thr.nb; i++)
             if (cfg[i].G > kWave) big += cnt[i];
-        return 2 * big >= R.M && R.M > 0 && P.N > (int64_t)window_cap(c->max_lds - 1024);
+        return 2 * big >= R.M && 2 * (int64_t)wide >= R.M && R.M > 0 &&
+               P.N > (int64_t)window_cap(c->max_lds - 1024);
     }
 
     int build(Context *c, Timer &tm)
@@ -1344,22 +1349,15 @@ struct FusedPlan {
             return -1;
         }
         Ctl h;
-        const int e0 = mk(tm);
-        SP3MM_HIP(hipMemsetAsync(d(), 0, sizeof(Ctl), s));
-        // the populations go to the host from the scatter kernel, while the
-        // likely bin already runs behind it
-        SP3MM_HIP(hipGetLastError());
-        const unsigned long long seq = c->poll_host ? ++c->poll_seq : 0;
-        launch_binning(c, s, seq);
-        const int eu = mk(tm);
         slog1 = sub_log(A, 64);
         slog2 = sub_log(P, 64);
         dlog1 = sub_log(A, kDenseThreads);
         dlog2 = sub_log(P, kDenseThreads);
         // the likely bin starts before the populations are read back, so the
-        // GPU does not idle through the read-back
+        // GPU does not idle through the read-back; its slots are allocated
+        // first, so that nothing but launches sits between the binning
+        // kernels and its launch
         pred = predict_bin(c);
-        int en0 = -1;
         if (pred >= 0) {
             const size_t cap = (size_t)M * cfg[pred].H2;
             pcol = Buf(cap * 4);
@@ -1371,13 +1369,23 @@ struct FusedPlan {
                 pval = Buf();
                 pslotn = Buf();
                 pred = -1;
-            } else {
-                en0 = mk(tm);
-                launch_main(c, s, pred);
             }
+        }
+        const int e0 = mk(tm);
+        // the binning clears the control block, and its scatter kernel sends
+        // the populations to the host while the likely bin already runs
+        SP3MM_HIP(hipGetLastError());
+        const unsigned long long seq = c->poll_host ? ++c->poll_seq : 0;
+        launch_binning(c, s, seq);
+        const int eu = mk(tm);
+        int en0 = -1;
+        if (pred >= 0) {
+            en0 = mk(tm);
+            launch_main(c, s, pred, true);
         }
         if (wait_ctl(c, d(), seq, h)) return -1;
         mac1 = (long long)h.totals[0];
+        wide = h.totals[2];
         cnt.assign(h.counts, h.counts + thr.nb + 1);
         nlong = cnt[thr.nb];
         if (reassoc_ok && wants_reassoc(c)) {
@@ -1513,8 +1521,7 @@ struct FusedPlan {
     void record(Context *c, hipStream_t s)
     {
         const int64_t M = R.M;
-        launch_fill_i32(reinterpret_cast<int32_t *>(d()), (int64_t)(sizeof(Ctl) / 4), 0, s);
-        launch_binning(c, s);
+        launch_binning(c, s);  // clears the control block first
         for (int i = 0; i < thr.nb; i++)
             if (cnt[i]) launch_main(c, s, i);
         launch_long(c, s, false);

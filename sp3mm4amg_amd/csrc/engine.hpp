@@ -58,7 +58,7 @@ struct Context {
     unsigned char *noalloc_base = nullptr;
     // coherent pinned host block for counters the host polls while the
     // stream goes on: the control block, then a flag at byte kPollFlagOff
-    // that k_export_flag sets to the sequence number of the export
+    // that the binning's scatter kernel sets to the sequence number of the export
     unsigned char *poll_host = nullptr;
     unsigned char *poll_dev = nullptr;
     unsigned long long poll_seq = 0;

@@ -53,16 +53,21 @@ __device__ __forceinline__ int bin_of(long long u, const BinThresholds &thr)
     return thr.nb;
 }
 
+// kspan receives the spread (last - first) of the row's columns (0 for the
+// given measures)
 __device__ __forceinline__ long long row_measure(int64_t r, const int64_t *__restrict__ arp,
                                                  const int32_t *__restrict__ acol, const int32_t *__restrict__ brp,
-                                                 const int32_t *__restrict__ meas1, const int32_t *__restrict__ meas2)
+                                                 const int32_t *__restrict__ meas1, const int32_t *__restrict__ meas2,
+                                                 int &kspan)
 {
+    kspan = 0;
     if (meas1) {
         int u = meas1[r];
         if (meas2) u = max(u, meas2[r]);
         return u;
     }
     const int64_t a0 = arp[r], a1 = arp[r + 1];
+    if (a1 > a0) kspan = acol[a1 - 1] - acol[a0];
     long long u = 0;
     // batches of kB entries: all their columns, then all their row bounds, are
     // loaded back to back (two memory round trips per batch); entries past
@@ -86,42 +91,54 @@ __global__ void __launch_bounds__(kBinThreads) k_bin_hist(const int64_t *__restr
                                                            const int32_t *__restrict__ meas2, int64_t M,
                                                            int64_t *__restrict__ ub, int32_t *__restrict__ rownnz,
                                                            int *__restrict__ hist, unsigned long long *__restrict__ part,
-                                                           BinThresholds thr)
+                                                           BinThresholds thr, uint32_t *__restrict__ zero, int zero_words,
+                                                           int span_lim)
 {
     __shared__ int h[kMaxBins + 1];
-    __shared__ unsigned long long red[2][kBinThreads / 64];
+    // workgroup 0 clears the caller's control block (zero_words words), which
+    // the kernels after this one accumulate into
+    if (zero && blockIdx.x == 0)
+        for (int i = threadIdx.x; i < zero_words; i += blockDim.x) zero[i] = 0u;
+    __shared__ unsigned long long red[3][kBinThreads / 64];
     const int nbin = thr.nb + 1;
     if ((int)threadIdx.x < nbin) h[threadIdx.x] = 0;
     __syncthreads();
     const int64_t r = (int64_t)blockIdx.x * kBinThreads + threadIdx.x;
     long long u = 0;
+    int kspan = 0;
     if (r < M) {
-        u = row_measure(r, arp, acol, brp, meas1, meas2);
+        u = row_measure(r, arp, acol, brp, meas1, meas2, kspan);
         if (ub) ub[r] = u;
         if (u <= 0 && rownnz) rownnz[r] = 0;
         const int b = bin_of(u, thr);
         if (b >= 0) atomicAdd(h + b, 1);
     }
+    // rows whose columns spread over more than span_lim (unstructured rows)
     unsigned long long s = (unsigned long long)(u > 0 ? u : 0), mx = s;
+    unsigned long long wide = span_lim > 0 && kspan > span_lim ? 1 : 0;
     for (int o = 32; o > 0; o >>= 1) {
         s += __shfl_xor(s, o);
+        wide += __shfl_xor(wide, o);
         const unsigned long long t = __shfl_xor(mx, o);
         mx = t > mx ? t : mx;
     }
     if (lane() == 0) {
         red[0][threadIdx.x / 64] = s;
         red[1][threadIdx.x / 64] = mx;
+        red[2][threadIdx.x / 64] = wide;
     }
     __syncthreads();
     if ((int)threadIdx.x < nbin) hist[(int64_t)threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
     if (threadIdx.x == 0) {
-        unsigned long long ts = 0, tm = 0;
+        unsigned long long ts = 0, tm = 0, tw = 0;
         for (int i = 0; i < kBinThreads / 64; i++) {
             ts += red[0][i];
             tm = red[1][i] > tm ? red[1][i] : tm;
+            tw += red[2][i];
         }
-        part[2 * (int64_t)blockIdx.x] = ts;
-        part[2 * (int64_t)blockIdx.x + 1] = tm;
+        part[3 * (int64_t)blockIdx.x] = ts;
+        part[3 * (int64_t)blockIdx.x + 1] = tm;
+        part[3 * (int64_t)blockIdx.x + 2] = tw;
     }
 }
 
@@ -146,30 +163,35 @@ __global__ void __launch_bounds__(1024) k_bin_offsets(int *__restrict__ hist, in
         }
         if (threadIdx.x == 0) counts[b] += (int)carry;
     } else if (totals) {
-        unsigned long long s = 0, m = 0;
+        unsigned long long s = 0, m = 0, wd = 0;
         for (int i = threadIdx.x; i < nblk; i += 1024) {
-            s += part[2 * (int64_t)i];
-            m = part[2 * (int64_t)i + 1] > m ? part[2 * (int64_t)i + 1] : m;
+            s += part[3 * (int64_t)i];
+            m = part[3 * (int64_t)i + 1] > m ? part[3 * (int64_t)i + 1] : m;
+            wd += part[3 * (int64_t)i + 2];
         }
         for (int o = 32; o > 0; o >>= 1) {
             s += __shfl_xor(s, o);
+            wd += __shfl_xor(wd, o);
             const unsigned long long t = __shfl_xor(m, o);
             m = t > m ? t : m;
         }
-        __shared__ unsigned long long rs[16], rm[16];
+        __shared__ unsigned long long rs[16], rm[16], rw[16];
         if (lane() == 0) {
             rs[threadIdx.x / 64] = s;
             rm[threadIdx.x / 64] = m;
+            rw[threadIdx.x / 64] = wd;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            unsigned long long ts = 0, tm = 0;
+            unsigned long long ts = 0, tm = 0, tw = 0;
             for (int w = 0; w < 16; w++) {
                 ts += rs[w];
                 tm = rm[w] > tm ? rm[w] : tm;
+                tw += rw[w];
             }
             totals[0] += ts;
             totals[1] = tm > totals[1] ? tm : totals[1];
+            totals[2] += tw;
         }
     }
 }
@@ -215,6 +237,16 @@ struct CtlExport {
     int words;
     unsigned long long *flag;
     unsigned long long seq;
+    bool clear;    // k_bin_hist clears src (the control block) first
+    int span_lim;  // > 0: totals[2] counts the rows whose columns spread wider
+    // the count a launch queued before the read-back (bin `pred`) reads:
+    // *pred_count = the merged population of bin pred, or 0 when the product
+    // is to be reassociated (reassoc_possible, and at least half of the rows
+    // in bins >= big_from or long, and at least half of them wide)
+    int *pred_count;
+    int pred, big_from;
+    bool reassoc_possible;
+    const unsigned long long *totals;
 };
 
 __global__ void __launch_bounds__(kBinThreads) k_bin_scatter(const int64_t *__restrict__ ub, const int32_t *__restrict__ meas1,
@@ -243,6 +275,15 @@ __global__ void __launch_bounds__(kBinThreads) k_bin_scatter(const int64_t *__re
         }
         if (merge_lim > 0 && blockIdx.x == 0)
             for (int b = 0; b < nbin; b++) mcounts[b] = cnt[b];
+        if (ex.pred_count && blockIdx.x == 0) {
+            bool reassoc = false;
+            if (ex.reassoc_possible) {
+                long long big = 0;
+                for (int b = ex.big_from; b < nbin; b++) big += merge_lim > 0 ? cnt[b] : raw[b];
+                reassoc = 2 * big >= M && 2 * (long long)ex.totals[2] >= M;
+            }
+            *ex.pred_count = reassoc || ex.pred < 0 ? 0 : (merge_lim > 0 ? cnt[ex.pred] : raw[ex.pred]);
+        }
     }
     __syncthreads();
     if (ex.host && blockIdx.x == 0) {
@@ -983,6 +1024,9 @@ __device__ __forceinline__ int nth_set_bit(unsigned x, int k)
 // instruction: output j of the run goes to lane j % 64, which finds the word
 // holding it (binary search over the lanes' exclusive counts) and the bit
 // inside the word.  Emitted bits and values are cleared.
+// A workspace (LDS, or global memory) belongs to one workgroup, so the
+// fences between its phases are workgroup-scope: a device-scope fence
+// would also write back the XCD's L2 at every emission.
 __device__ __forceinline__ int dense_emit(unsigned *bits, double *vals, int nw, int32_t *outc, double *outv,
                                           long long cap, long long *sh, int c0 = 0)
 {
@@ -1023,7 +1067,7 @@ __device__ __forceinline__ int dense_emit(unsigned *bits, double *vals, int nw, 
         if (x) bits[w] = 0u;
         base += total;
     }
-    __threadfence();
+    __threadfence_block();
     __syncthreads();
     return (int)base;
 }
@@ -1070,13 +1114,13 @@ __global__ void __launch_bounds__(kDense) k_dense_spgemm(SpOperand A, SpOperand 
         long long fresh = 0;
         dense_expand<NUM, double>(A.col, aval, A.rp[r], A.rp[r + 1], B.rp32, B.col, bval, slog, bits, vals,
                                   fresh);
-        __threadfence();
+        __threadfence_block();
         __syncthreads();
         if constexpr (!NUM) {
             const long long tot = block_sum_dense(fresh, sh);
             for (int w = threadIdx.x; w < nw; w += kDense) bits[w] = 0u;
             if (threadIdx.x == 0) rownnz[r] = (int32_t)tot;
-            __threadfence();
+            __threadfence_block();
             __syncthreads();
         } else {
             const int64_t off = crp[r];
@@ -1107,6 +1151,7 @@ __global__ void __launch_bounds__(kDense) k_dense_spgemm(SpOperand A, SpOperand 
 // crp[r] + (entries of the earlier windows).  One 1024-thread workgroup per
 // row, persistent over the row list.
 constexpr int kWinEntries = 1024;
+constexpr int kWinPiece = 256;
 constexpr int kWinSub = 16;
 constexpr int kWinVec = 4;
 constexpr int kWinStep = kWinSub * kWinVec;
@@ -1164,23 +1209,53 @@ __global__ void __launch_bounds__(kDense) k_spgemm_window(SpOperand A, SpOperand
         const int r = rows[i];
         const int64_t e0 = A.rp[r];
         const int na = (int)(A.rp[r + 1] - e0);
-        // stage the entries; the numeric pass also finds the row's column
-        // range [lo, hi] from the first and last column of every right row
-        // and runs its windows over that range only
-        int lo = INT_MAX, hi = -1;
+        // Stage the entries.  A right row longer than kWinPiece positions is
+        // cut into pieces of at most kWinPiece, each staged with its own
+        // cursor, so that no sub-group walks a long row alone while the others
+        // wait at the window's barrier -- if all pieces of the row fit the
+        // staging area; else the entries are staged whole (and those beyond
+        // it read from global memory).  The numeric pass also finds the row's
+        // column range [lo, hi] from the first and last column of every right
+        // row and runs its windows over that range only.
+        int pieces = 0;
         for (int e = t; e < na; e += kDense) {
             const int k = A.col[e0 + e];
-            const int b0 = B.rp32[k], b1 = B.rp32[k + 1];
-            if (e < kWinEntries) {
-                ecur[e] = b0;
-                eend[e] = b1;
-                if constexpr (NUM) elv[e] = aval[e0 + e];
-            }
-            if (NUM && b1 > b0) {
-                lo = min(lo, B.col[b0]);
-                hi = max(hi, B.col[b1 - 1]);
-            }
+            pieces += (B.rp32[k + 1] - B.rp32[k] + kWinPiece - 1) / kWinPiece;
         }
+        const bool split = block_sum_dense(pieces, sh) <= kWinEntries;
+        int lo = INT_MAX, hi = -1;
+        long long pbase = 0;
+        for (int e0c = 0; e0c < na; e0c += kDense) {
+            const int e = e0c + t;
+            int b0 = 0, b1 = 0, np = 0;
+            double x = 0.0;
+            if (e < na) {
+                const int k = A.col[e0 + e];
+                b0 = B.rp32[k];
+                b1 = B.rp32[k + 1];
+                if constexpr (NUM) x = aval[e0 + e];
+                np = split ? (b1 - b0 + kWinPiece - 1) / kWinPiece : 1;
+                if (NUM && b1 > b0) {
+                    lo = min(lo, B.col[b0]);
+                    hi = max(hi, B.col[b1 - 1]);
+                }
+            }
+            long long tot;
+            const long long first = pbase + (split ? block_exclusive_scan(np, sh, tot) : (long long)e0c + t);
+            if (!split) tot = min(kDense, na - e0c);
+            for (int j = 0; j < np; j++) {
+                const long long v = first + j;
+                if (v < kWinEntries) {
+                    ecur[v] = b0 + j * kWinPiece;
+                    eend[v] = split ? min(b1, b0 + (j + 1) * kWinPiece) : b1;
+                    if constexpr (NUM) elv[v] = x;
+                }
+            }
+            pbase += tot;
+        }
+        // staged units: pieces (split) or entries; with !split, entries at
+        // and past kWinEntries come from global memory
+        const int nunit = split ? (int)pbase : na;
         if constexpr (NUM) {
             for (int o = 32; o > 0; o >>= 1) {
                 lo = min(lo, __shfl_xor(lo, o));
@@ -1212,11 +1287,11 @@ __global__ void __launch_bounds__(kDense) k_spgemm_window(SpOperand A, SpOperand
         long long base = 0;
         for (int w0 = c0; w0 < c1; w0 += span) {
             const int w1 = min(c1, w0 + span);
-            // every sub-group runs the same number of entry rounds, so the
-            // ballots below see whole sub-groups
-            for (int eb = 0; eb < na; eb += ng) {
+            // every sub-group runs the same number of rounds, so the ballots
+            // below see whole sub-groups
+            for (int eb = 0; eb < nunit; eb += ng) {
                 const int e = eb + t / kWinSub;
-                const bool ok = e < na;
+                const bool ok = e < nunit;
                 int q = 0, qe = 0;
                 double lv = 0.0;
                 if (ok) {
@@ -1334,12 +1409,12 @@ __global__ void __launch_bounds__(kDense) k_dense_direct(SpOperand R, SpOperand 
         long long f1 = 0, f2 = 0;
         dense_expand<NUM, VT, VT>(R.col, rval, R.rp[r], R.rp[r + 1], A.rp32, A.col, aval, slog1, bitsA, valsA,
                                   f1);
-        __threadfence();
+        __threadfence_block();
         __syncthreads();
         const int n1 = dense_emit(bitsA, valsA, ws.nwA, listc, listv, (long long)ws.nwA * 32, sh);
         double ps = 0.0;
         dense_expand<NUM, double, VT>(listc, listv, 0, n1, P.rp32, P.col, pval, slog2, bitsP, valsP, f2, &ps);
-        __threadfence();
+        __threadfence_block();
         __syncthreads();
         if (NUM && rowsum) {
             for (int o = 32; o > 0; o >>= 1) ps += __shfl_xor(ps, o);
@@ -1358,7 +1433,7 @@ __global__ void __launch_bounds__(kDense) k_dense_direct(SpOperand R, SpOperand 
                 rownnz[r] = (int32_t)tot;
                 if (n1out) n1out[r] = n1;
             }
-            __threadfence();
+            __threadfence_block();
             __syncthreads();
         } else {
             const int64_t off = crp[r];
@@ -1411,11 +1486,11 @@ __global__ void __launch_bounds__(kDense) k_dense_window(SpOperand R, SpOperand 
         const int r = rows[i];
         long long f1 = 0;
         dense_expand<true, VT, VT>(R.col, rval, R.rp[r], R.rp[r + 1], A.rp32, A.col, aval, slog1, bitsA, valsA, f1);
-        __threadfence();
+        __threadfence_block();
         __syncthreads();
         const int n1 = dense_emit(bitsA, valsA, ws.nwA, listc, listv, (long long)ws.nwA * 32, sh);
         for (int e = threadIdx.x; e < n1; e += kDense) cur[e] = P.rp32[listc[e]];
-        __threadfence();
+        __threadfence_block();
         __syncthreads();
         const int64_t off = crp[r];
         const long long cap = crp[r + 1] - off;
@@ -1532,21 +1607,6 @@ __global__ void k_export_words(const uint32_t *__restrict__ src, uint32_t *dst, 
     for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
 
-// The same into coherent pinned host memory, then `seq` into *flag with a
-// system-scope release: a host thread polling the flag may read the block as
-// soon as the flag shows seq, while later work of the stream goes on.
-__global__ void k_export_flag(const uint32_t *__restrict__ src, uint32_t *dst, int n, unsigned long long *flag,
-                              unsigned long long seq)
-{
-    for (int i = threadIdx.x; i < n; i += blockDim.x)
-        __hip_atomic_store(dst + i, src[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence_system();
-        __hip_atomic_store(flag, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
-}
-
 __global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1565,7 +1625,7 @@ __global__ void k_copy_probe(const double4 *__restrict__ in, double4 *__restrict
 // ---------------------------------------------------------------------------
 static inline unsigned blocks_for(int64_t n, int per) { return (unsigned)((n + per - 1) / per); }
 
-int64_t bin_scratch_ints(int64_t M) { return (kMaxBins + 1) * ((M + kBinThreads - 1) / kBinThreads + 1) + 4 * ((M + kBinThreads - 1) / kBinThreads + 1); }
+int64_t bin_scratch_ints(int64_t M) { return (kMaxBins + 1) * ((M + kBinThreads - 1) / kBinThreads + 1) + 6 * ((M + kBinThreads - 1) / kBinThreads + 1) + 2; }
 
 static void launch_bins(const SpOperand *A, const SpOperand *B, const int32_t *m1, const int32_t *m2, int64_t M,
                         int64_t *ub, int32_t *rownnz, int32_t *lists, int *counts, const BinThresholds &thr,
@@ -1577,8 +1637,10 @@ static void launch_bins(const SpOperand *A, const SpOperand *B, const int32_t *m
     int *hist = scratch;
     unsigned long long *part = reinterpret_cast<unsigned long long *>(scratch + (int64_t)(kMaxBins + 1) * (nblk + 1) + 1) ;
     part = reinterpret_cast<unsigned long long *>((reinterpret_cast<uintptr_t>(part) + 7) & ~(uintptr_t)7);
+    uint32_t *zero = ex && ex->clear ? const_cast<uint32_t *>(ex->src) : nullptr;
     hipLaunchKernelGGL(k_bin_hist, dim3(nblk), dim3(kBinThreads), 0, st, A ? A->rp : nullptr, A ? A->col : nullptr,
-                       B ? B->rp32 : nullptr, m1, m2, M, ub, rownnz, hist, part, thr);
+                       B ? B->rp32 : nullptr, m1, m2, M, ub, rownnz, hist, part, thr, zero, zero ? ex->words : 0,
+                       ex ? ex->span_lim : 0);
     hipLaunchKernelGGL(k_bin_offsets, dim3(thr.nb + 2), dim3(1024), 0, st, hist, nblk, thr.nb + 1, part, counts,
                        totals);
     CtlExport e{};
@@ -1590,11 +1652,13 @@ static void launch_bins(const SpOperand *A, const SpOperand *B, const int32_t *m
 void launch_ub_bin(const SpOperand &A, const SpOperand &B, int64_t *ub, int32_t *rownnz,
                    int32_t *lists, int *counts, const BinThresholds &thr,
                    unsigned long long *totals, int *scratch, hipStream_t st, int merge_lim, int *mcounts,
-                   const void *ctl, void *ctl_host, int ctl_words, unsigned long long *flag, unsigned long long seq)
+                   const void *ctl, void *ctl_host, int ctl_words, unsigned long long *flag, unsigned long long seq,
+                   int span_lim, int *pred_count, int pred, int big_from, bool reassoc_possible)
 {
-    CtlExport e{static_cast<const uint32_t *>(ctl), static_cast<uint32_t *>(ctl_host), ctl_words, flag, seq};
+    CtlExport e{static_cast<const uint32_t *>(ctl), static_cast<uint32_t *>(ctl_host), ctl_words, flag, seq,
+                ctl != nullptr, span_lim, pred_count, pred, big_from, reassoc_possible, totals};
     launch_bins(&A, &B, nullptr, nullptr, A.M, ub, rownnz, lists, counts, thr, totals, scratch, st, merge_lim,
-                mcounts, ctl_host ? &e : nullptr);
+                mcounts, ctl ? &e : nullptr);
 }
 
 void launch_ub_colparts(const int64_t *arp, const int32_t *acol, int64_t M, const int64_t *boff, int G,
@@ -2290,13 +2354,6 @@ void launch_export_words(const void *src, void *host_dst, int words, hipStream_t
 {
     hipLaunchKernelGGL(k_export_words, dim3(1), dim3(64), 0, st, static_cast<const uint32_t *>(src),
                        static_cast<uint32_t *>(host_dst), words);
-}
-
-void launch_export_flag(const void *src, void *host_dst, int words, unsigned long long *flag, unsigned long long seq,
-                        hipStream_t st)
-{
-    hipLaunchKernelGGL(k_export_flag, dim3(1), dim3(64), 0, st, static_cast<const uint32_t *>(src),
-                       static_cast<uint32_t *>(host_dst), words, flag, seq);
 }
 
 void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st)

@@ -30,14 +30,22 @@ int64_t bin_scratch_ints(int64_t M);
 // populations.  merge_lim > 0: bins of at most merge_lim rows fold into the
 // next larger non-empty hash bin while the lists are written, and mcounts[]
 // receives the merged populations (the lists are laid out by them).
-// ctl_host != null: the scatter kernel also copies ctl_words words of the
-// control block `ctl` (populations and totals final by then) into coherent
-// host memory at ctl_host and then writes seq to *flag (host-mapped).
+// ctl != null: the first kernel clears ctl_words words of the control block
+// `ctl` (the caller need not), and with ctl_host != null the scatter kernel
+// also copies them (populations and totals final by then) into coherent host
+// memory at ctl_host and then writes seq to *flag (host-mapped).  totals[0]
+// receives the sum of the bounds, totals[1] their maximum and, with
+// span_lim > 0 (ctl given), totals[2] the number of rows of A whose columns
+// spread over more than span_lim (last column minus first).  pred_count !=
+// null: the scatter also writes the (merged) population of bin `pred` there,
+// or 0 if reassoc_possible and at least half of the rows are in bins >=
+// big_from (or long) and at least half are spread wider than span_lim.
 void launch_ub_bin(const SpOperand &A, const SpOperand &B, int64_t *ub, int32_t *rownnz,
                    int32_t *lists, int *counts, const BinThresholds &thr,
                    unsigned long long *totals, int *scratch, hipStream_t st, int merge_lim = 0,
                    int *mcounts = nullptr, const void *ctl = nullptr, void *ctl_host = nullptr, int ctl_words = 0,
-                   unsigned long long *flag = nullptr, unsigned long long seq = 0);
+                   unsigned long long *flag = nullptr, unsigned long long seq = 0, int span_lim = 0,
+                   int *pred_count = nullptr, int pred = -1, int big_from = 0, bool reassoc_possible = false);
 void launch_ub_colparts(const int64_t *arp, const int32_t *acol, int64_t M, const int64_t *boff, int G,
                         int64_t *out, unsigned long long *total, hipStream_t st);
 void launch_rebin(const int32_t *m1, const int32_t *m2, int64_t M, int32_t *lists, int *counts,
@@ -200,10 +208,6 @@ int dot_parts();
 void launch_jacobi(const int64_t *rp, const int32_t *col, const double *val, int64_t M, int64_t nnz, const double *b,
                    const double *xi, double *xo, double omega, int *err, hipStream_t st);
 void launch_export_words(const void *src, void *host_dst, int words, hipStream_t st);
-// copies `words` words into coherent pinned host memory, then writes `seq` to
-// *flag (host-mapped) with a system-scope release
-void launch_export_flag(const void *src, void *host_dst, int words, unsigned long long *flag, unsigned long long seq,
-                        hipStream_t st);
 void launch_copy_probe(const void *in, void *out, int64_t bytes, hipStream_t st);
 
 }  // namespace sp3mm
