@@ -421,8 +421,9 @@ def irregular_leg(lib, problems, reps=2, n=1_000_000):
     R, A, P = p["R"], p["A"], p["P"]
     log(f"irregular inputs generated in {time.perf_counter() - t0:.1f}s: nnz R/A/P {R.nnz}/{A.nnz}/{P.nnz}")
     dR, dA, dP = (lib.DeviceCSR.upload(m) for m in (R, A, P))
-    RA, _ = lib.spgemm(dR, dA)
+    RA, st_ra = lib.spgemm(dR, dA)
     nnz_ra = RA.nnz
+    mac1 = int(st_ra["mac1"])
     mac2 = int(lib.ub(RA, dP)[-1])
     RA.free()
     lib.set_tuning(None)
@@ -438,7 +439,6 @@ def irregular_leg(lib, problems, reps=2, n=1_000_000):
         dev.append(st["ms_total"])
         nnz_out = st["nnz_out"]
         out.free()
-    mac1 = st["mac1"]
     for m in (dR, dA, dP):
         m.free()
     ms = float(np.median(dev))

@@ -225,7 +225,10 @@ typedef struct {
     double ms_scan;        /* row-offset scans                                  */
     double ms_numeric;     /* numeric phase(s)                                  */
     double ms_main;        /* the dominant kernel (numeric of the last product) */
-    int64_t mac1, mac2;    /* multiply-adds of R*A and of (RA)*P                */
+    int64_t mac1, mac2;    /* multiply-adds of R*A and of (RA)*P; -1: not
+                              counted (a direct product whose rows all fit one
+                              bin by their longest-row bound is not measured
+                              row by row)                                      */
     int64_t nnz_mid;       /* nnz of R*A (pair path) or its symbolic count      */
     int64_t nnz_out;
     int64_t guarded_rows;  /* MIXED: rows recomputed in FP64                    */

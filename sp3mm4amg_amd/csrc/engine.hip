@@ -386,7 +386,7 @@ struct Ctl {
     int counts2[16];
     int rawc[16];  // bin populations before small bins are merged (direct plans)
     int pred_count;  // population seen by the launch queued before the read-back (0: abandoned)
-    int pad3;
+    int onebin;      // 1: the binning put every row in one bin without measuring it
     int nretry;
     int err;
     int nspill;  // long rows whose column windows exceed the LDS window kernel
@@ -1170,6 +1170,8 @@ struct FusedPlan {
     // scol/sval (offsets soff) and slotn (offsets nbase), sized by population.
     int pred = -1;
     Buf pcol, pval, pslotn;
+    Buf mlen;             // per-workgroup longest rows of R and A (one-bin test)
+    bool onebin = false;  // the binning put every row in bin pred
     int32_t *slot_col(int i) const { return i == pred ? pcol.as<int32_t>() : scol.as<int32_t>() + soff[i]; }
     double *slot_val(int i) const { return i == pred ? pval.as<double>() : sval.as<double>() + soff[i]; }
     int32_t *slot_n(int i) const { return i == pred ? pslotn.as<int32_t>() : slotn.as<int32_t>() + nbase[i]; }
@@ -1295,10 +1297,21 @@ struct FusedPlan {
                 big_from = i;
                 break;
             }
+        // one-bin mode: when the longest row of R times the longest row of A
+        // fits the likely bin's bound, every row goes to that bin unmeasured
+        OneBin ob{};
+        const bool one = pred >= 0 && cfg[pred].G <= kWave && mlen.p;
+        if (one) {
+            ob.part = mlen.as<int>();
+            ob.nblk = kMaxLenBlocks;
+            ob.bound = thr.v[pred];
+            ob.u = thr.v[pred];
+            ob.word = (int)(offsetof(Ctl, onebin) / 4);
+        }
         launch_ub_bin(r(), a(), ub.as<int64_t>(), rown.as<int32_t>(), lists.as<int32_t>(), d()->rawc, thr,
                       d()->totals, binscr.as<int>(), s, kMergeRows, d()->counts, d(), host,
                       (int)(sizeof(Ctl) / 4), flag, seq, span_lim, &d()->pred_count, pred, big_from,
-                      reassoc_possible(c));
+                      reassoc_possible(c), one ? &ob : nullptr);
     }
 
     // R*(A*P) instead of the direct forwarding: when at least half of R's
@@ -1371,6 +1384,10 @@ struct FusedPlan {
                 pred = -1;
             }
         }
+        if (pred >= 0) {
+            mlen = Buf((size_t)kMaxLenBlocks * 2 * 4);
+            if (!mlen.p) (void)hipGetLastError();  // binning measures every row then
+        }
         const int e0 = mk(tm);
         // the binning clears the control block, and its scatter kernel sends
         // the populations to the host while the likely bin already runs
@@ -1384,7 +1401,8 @@ struct FusedPlan {
             launch_main(c, s, pred, true);
         }
         if (wait_ctl(c, d(), seq, h)) return -1;
-        mac1 = (long long)h.totals[0];
+        onebin = h.onebin != 0;
+        mac1 = onebin ? -1 : (long long)h.totals[0];  // one-bin mode: the bounds were not summed
         wide = h.totals[2];
         cnt.assign(h.counts, h.counts + thr.nb + 1);
         nlong = cnt[thr.nb];

@@ -86,19 +86,73 @@ __device__ __forceinline__ long long row_measure(int64_t r, const int64_t *__res
     return u;
 }
 
+// Longest row of A (rows 0..M-1) and of B: per-workgroup maxima into
+// part[2b] and part[2b + 1] (kMaxLenBlocks workgroups), reduced by every
+// workgroup of k_bin_hist.  Their product bounds every row's ub.
+__global__ void __launch_bounds__(256) k_row_maxlen(const int64_t *__restrict__ arp, int64_t M,
+                                                     const int64_t *__restrict__ brp, int64_t BM,
+                                                     int *__restrict__ part)
+{
+    __shared__ long long red[2][256 / 64];
+    long long ma = 0, mb = 0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < M; i += stride)
+        ma = max(ma, (long long)(arp[i + 1] - arp[i]));
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < BM; i += stride)
+        mb = max(mb, (long long)(brp[i + 1] - brp[i]));
+    for (int o = 32; o > 0; o >>= 1) {
+        ma = max(ma, (long long)__shfl_xor(ma, o));
+        mb = max(mb, (long long)__shfl_xor(mb, o));
+    }
+    if (lane() == 0) {
+        red[0][threadIdx.x / 64] = ma;
+        red[1][threadIdx.x / 64] = mb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 256 / 64; w++) {
+            ma = max(ma, red[0][w]);
+            mb = max(mb, red[1][w]);
+        }
+        part[2 * blockIdx.x] = (int)min(ma, (long long)INT32_MAX);
+        part[2 * blockIdx.x + 1] = (int)min(mb, (long long)INT32_MAX);
+    }
+}
+
 __global__ void __launch_bounds__(kBinThreads) k_bin_hist(const int64_t *__restrict__ arp, const int32_t *__restrict__ acol,
                                                            const int32_t *__restrict__ brp, const int32_t *__restrict__ meas1,
                                                            const int32_t *__restrict__ meas2, int64_t M,
                                                            int64_t *__restrict__ ub, int32_t *__restrict__ rownnz,
                                                            int *__restrict__ hist, unsigned long long *__restrict__ part,
                                                            BinThresholds thr, uint32_t *__restrict__ zero, int zero_words,
-                                                           int span_lim)
+                                                           int span_lim, OneBin ob)
 {
     __shared__ int h[kMaxBins + 1];
+    // one-bin mode: the longest rows bound every row's ub by ob.bound, so
+    // every row goes to the bin of ob.u without measuring it
+    bool one = false;
+    if (ob.part) {
+        __shared__ int dec;
+        if (threadIdx.x < 64) {
+            long long ma = 0, mb = 0;
+            for (int i = threadIdx.x; i < ob.nblk; i += 64) {
+                ma = max(ma, (long long)ob.part[2 * i]);
+                mb = max(mb, (long long)ob.part[2 * i + 1]);
+            }
+            for (int o = 32; o > 0; o >>= 1) {
+                ma = max(ma, (long long)__shfl_xor(ma, o));
+                mb = max(mb, (long long)__shfl_xor(mb, o));
+            }
+            if (threadIdx.x == 0) dec = ma * mb <= ob.bound;
+        }
+        __syncthreads();
+        one = dec != 0;
+    }
     // workgroup 0 clears the caller's control block (zero_words words), which
-    // the kernels after this one accumulate into
+    // the kernels after this one accumulate into; word ob.word (if any)
+    // receives the one-bin decision
     if (zero && blockIdx.x == 0)
-        for (int i = threadIdx.x; i < zero_words; i += blockDim.x) zero[i] = 0u;
+        for (int i = threadIdx.x; i < zero_words; i += blockDim.x) zero[i] = i == ob.word && one ? 1u : 0u;
     __shared__ unsigned long long red[3][kBinThreads / 64];
     const int nbin = thr.nb + 1;
     if ((int)threadIdx.x < nbin) h[threadIdx.x] = 0;
@@ -107,14 +161,15 @@ __global__ void __launch_bounds__(kBinThreads) k_bin_hist(const int64_t *__restr
     long long u = 0;
     int kspan = 0;
     if (r < M) {
-        u = row_measure(r, arp, acol, brp, meas1, meas2, kspan);
+        u = one ? ob.u : row_measure(r, arp, acol, brp, meas1, meas2, kspan);
         if (ub) ub[r] = u;
         if (u <= 0 && rownnz) rownnz[r] = 0;
         const int b = bin_of(u, thr);
         if (b >= 0) atomicAdd(h + b, 1);
     }
-    // rows whose columns spread over more than span_lim (unstructured rows)
-    unsigned long long s = (unsigned long long)(u > 0 ? u : 0), mx = s;
+    // rows whose columns spread over more than span_lim (unstructured rows);
+    // in one-bin mode the bounds are not known and not summed
+    unsigned long long s = (unsigned long long)(u > 0 && !one ? u : 0), mx = s;
     unsigned long long wide = span_lim > 0 && kspan > span_lim ? 1 : 0;
     for (int o = 32; o > 0; o >>= 1) {
         s += __shfl_xor(s, o);
@@ -1630,9 +1685,16 @@ int64_t bin_scratch_ints(int64_t M) { return (kMaxBins + 1) * ((M + kBinThreads 
 static void launch_bins(const SpOperand *A, const SpOperand *B, const int32_t *m1, const int32_t *m2, int64_t M,
                         int64_t *ub, int32_t *rownnz, int32_t *lists, int *counts, const BinThresholds &thr,
                         unsigned long long *totals, int *scratch, hipStream_t st, int merge_lim = 0,
-                        int *mcounts = nullptr, const CtlExport *ex = nullptr)
+                        int *mcounts = nullptr, const CtlExport *ex = nullptr, const OneBin *one = nullptr)
 {
     if (M == 0) return;
+    OneBin ob{};
+    ob.word = -1;
+    if (one && A && B) {
+        ob = *one;
+        hipLaunchKernelGGL(k_row_maxlen, dim3(kMaxLenBlocks), dim3(256), 0, st, A->rp, M, B->rp, B->M,
+                           const_cast<int *>(ob.part));
+    }
     const int nblk = (int)((M + kBinThreads - 1) / kBinThreads);
     int *hist = scratch;
     unsigned long long *part = reinterpret_cast<unsigned long long *>(scratch + (int64_t)(kMaxBins + 1) * (nblk + 1) + 1) ;
@@ -1640,7 +1702,7 @@ static void launch_bins(const SpOperand *A, const SpOperand *B, const int32_t *m
     uint32_t *zero = ex && ex->clear ? const_cast<uint32_t *>(ex->src) : nullptr;
     hipLaunchKernelGGL(k_bin_hist, dim3(nblk), dim3(kBinThreads), 0, st, A ? A->rp : nullptr, A ? A->col : nullptr,
                        B ? B->rp32 : nullptr, m1, m2, M, ub, rownnz, hist, part, thr, zero, zero ? ex->words : 0,
-                       ex ? ex->span_lim : 0);
+                       ex ? ex->span_lim : 0, ob);
     hipLaunchKernelGGL(k_bin_offsets, dim3(thr.nb + 2), dim3(1024), 0, st, hist, nblk, thr.nb + 1, part, counts,
                        totals);
     CtlExport e{};
@@ -1653,12 +1715,12 @@ void launch_ub_bin(const SpOperand &A, const SpOperand &B, int64_t *ub, int32_t 
                    int32_t *lists, int *counts, const BinThresholds &thr,
                    unsigned long long *totals, int *scratch, hipStream_t st, int merge_lim, int *mcounts,
                    const void *ctl, void *ctl_host, int ctl_words, unsigned long long *flag, unsigned long long seq,
-                   int span_lim, int *pred_count, int pred, int big_from, bool reassoc_possible)
+                   int span_lim, int *pred_count, int pred, int big_from, bool reassoc_possible, const OneBin *one)
 {
     CtlExport e{static_cast<const uint32_t *>(ctl), static_cast<uint32_t *>(ctl_host), ctl_words, flag, seq,
                 ctl != nullptr, span_lim, pred_count, pred, big_from, reassoc_possible, totals};
     launch_bins(&A, &B, nullptr, nullptr, A.M, ub, rownnz, lists, counts, thr, totals, scratch, st, merge_lim,
-                mcounts, ctl ? &e : nullptr);
+                mcounts, ctl ? &e : nullptr, ctl ? one : nullptr);
 }
 
 void launch_ub_colparts(const int64_t *arp, const int32_t *acol, int64_t M, const int64_t *boff, int G,

@@ -40,12 +40,25 @@ int64_t bin_scratch_ints(int64_t M);
 // null: the scatter also writes the (merged) population of bin `pred` there,
 // or 0 if reassoc_possible and at least half of the rows are in bins >=
 // big_from (or long) and at least half are spread wider than span_lim.
+// One-bin mode of the binning (direct plans): k_row_maxlen writes the
+// longest row of A and of B per workgroup into part (2 * kMaxLenBlocks ints);
+// when their product is at most `bound`, every row is binned with measure `u`
+// without measuring it, the per-row bounds are not summed (totals[0] stays
+// 0), and word `word` of the control block is 1.
+constexpr int kMaxLenBlocks = 256;
+struct OneBin {
+    const int *part;
+    int nblk;  // kMaxLenBlocks
+    long long bound, u;
+    int word;
+};
 void launch_ub_bin(const SpOperand &A, const SpOperand &B, int64_t *ub, int32_t *rownnz,
                    int32_t *lists, int *counts, const BinThresholds &thr,
                    unsigned long long *totals, int *scratch, hipStream_t st, int merge_lim = 0,
                    int *mcounts = nullptr, const void *ctl = nullptr, void *ctl_host = nullptr, int ctl_words = 0,
                    unsigned long long *flag = nullptr, unsigned long long seq = 0, int span_lim = 0,
-                   int *pred_count = nullptr, int pred = -1, int big_from = 0, bool reassoc_possible = false);
+                   int *pred_count = nullptr, int pred = -1, int big_from = 0, bool reassoc_possible = false,
+                   const OneBin *one = nullptr);
 void launch_ub_colparts(const int64_t *arp, const int32_t *acol, int64_t M, const int64_t *boff, int G,
                         int64_t *out, unsigned long long *total, hipStream_t st);
 void launch_rebin(const int32_t *m1, const int32_t *m2, int64_t M, int32_t *lists, int *counts,

@@ -165,7 +165,9 @@ def test_workloads_device_api(name, path):
     else:
         check(got, want, O.abs_product(R, A, P))
     assert st["nnz_out"] == want.nnz
-    assert st["mac1"] == O.ub(R, A)[-1]
+    # the direct path skips measuring rows when they all fit one bin by the
+    # longest rows of R and A (mac1 -1 then)
+    assert st["mac1"] == O.ub(R, A)[-1] or (path == L.DIRECT and st["mac1"] == -1)
     if path == L.PAIR:
         assert st["nnz_mid"] == O.symb(R, A)[-1]
 
@@ -774,7 +776,10 @@ def test_poisson3d128_full_size():
         got = out.download()
         out.free()
         check(got, want, scale)
-        assert st["nnz_out"] == 8382712 and st["mac1"] == 57740288
+        assert st["nnz_out"] == 8382712
+        # R rows of at most 32 entries, A rows of at most 7: every bound fits
+        # the 256-slot bin, so the direct path bins without measuring
+        assert st["mac1"] == (-1 if path == L.DIRECT else 57740288)
     for m in (dR, dA, dP):
         m.free()
 
