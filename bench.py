@@ -410,11 +410,12 @@ def consumer_leg(lib, problems, dA, n, reps=50):
                           "roofline_frac": round(spmv_bytes / (spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}}
 
 
-def irregular_leg(lib, problems, reps=2, n=1_000_000):
+def irregular_leg(lib, problems, reps=2, n=1_000_000, tune=True):
     """Irregular rows at full size: lognormal row lengths on 1M x 1M, random
-    aggregation (PCG64 seed 4), FP64, direct path.  MAC2 (the products of the
-    second stage) comes from the row upper bounds of (R*A)*P, with R*A formed
-    once outside the timed region."""
+    aggregation (PCG64 seed 4), FP64, direct path, under the schedule the
+    autotuner picks for it (untimed; `tune` False keeps the defaults).  MAC2
+    (the products of the second stage) comes from the row upper bounds of
+    (R*A)*P, with R*A formed once outside the timed region."""
     import numpy as np
     t0 = time.perf_counter()
     p = problems.irregular_sa(n, seed=4)
@@ -427,6 +428,12 @@ def irregular_leg(lib, problems, reps=2, n=1_000_000):
     mac2 = int(lib.ub(RA, dP)[-1])
     RA.free()
     lib.set_tuning(None)
+    schedule, tune_s = None, 0.0
+    if tune:
+        t0 = time.perf_counter()
+        schedule, ms_best, ms_default = lib.autotune(dR, dA, dP, reps=1)
+        tune_s = time.perf_counter() - t0
+        log(f"irregular autotune: {schedule} {ms_best:.1f} ms (default {ms_default:.1f} ms) in {tune_s:.1f}s")
     out, st = lib.rap(dR, dA, dP, path=lib.DIRECT)
     out.free()
     lib.sync()
@@ -444,7 +451,9 @@ def irregular_leg(lib, problems, reps=2, n=1_000_000):
     ms = float(np.median(dev))
     ba = alg_bytes(R, A, P, mac1, mac2, nnz_ra, nnz_out)
     bc = compulsory_bytes(R, A, P, nnz_out)
-    return {"workload": f"irregular_lognormal_{n}_seed4 Galerkin R*A*P, FP64, direct path (default schedule)",
+    lib.set_tuning(None)
+    return {"workload": f"irregular_lognormal_{n}_seed4 Galerkin R*A*P, FP64, direct path",
+            "schedule": schedule, "autotune_s": round(tune_s, 1), "reassociated": st["reassociated"],
             "rows_coarse": R.M, "nnz_A": A.nnz, "nnz_RA": nnz_ra, "nnz_RAP": nnz_out, "mac1": mac1, "mac2": mac2,
             "ms_per_product": round(ms, 1), "wall_ms": round(float(np.median(wall)), 1),
             "gflops": round(2 * (mac1 + mac2) / (ms * 1e-3) / 1e9, 1),
@@ -520,7 +529,7 @@ def main():
         elif args.only_leg == "batch":
             leg = batch_leg(lib, problems, max(3, args.steps // 4), tune=not args.no_tune)[0]
         elif args.only_leg == "irregular":
-            leg = irregular_leg(lib, problems, reps=max(1, args.steps))
+            leg = irregular_leg(lib, problems, reps=max(1, args.steps), tune=not args.no_tune)
         else:
             p = problems.poisson3d_sa(args.n)
             dA = lib.DeviceCSR.upload(p["A"])
@@ -640,7 +649,7 @@ def main():
         rec["host_dropin"] = host_dropin_leg(lib, R, A, P, flop)
     if not args.no_irregular:
         log("irregular leg (1M lognormal rows) ...")
-        rec["irregular"] = irregular_leg(lib, problems)
+        rec["irregular"] = irregular_leg(lib, problems, tune=not args.no_tune)
     if not args.no_cpu:
         log("timing the reference CPU path on samples ...")
         p1 = problems.poisson2d_sa(64)

@@ -89,17 +89,36 @@ __device__ __forceinline__ long long row_measure(int64_t r, const int64_t *__res
 // Longest row of A (rows 0..M-1) and of B: per-workgroup maxima into
 // part[2b] and part[2b + 1] (kMaxLenBlocks workgroups), reduced by every
 // workgroup of k_bin_hist.  Their product bounds every row's ub.
-__global__ void __launch_bounds__(256) k_row_maxlen(const int64_t *__restrict__ arp, int64_t M,
-                                                     const int64_t *__restrict__ brp, int64_t BM,
+template <typename IA, typename IB>
+__global__ void __launch_bounds__(256) k_row_maxlen(const IA *__restrict__ arp, int64_t M,
+                                                     const IB *__restrict__ brp, int64_t BM,
                                                      int *__restrict__ part)
 {
     __shared__ long long red[2][256 / 64];
     long long ma = 0, mb = 0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < M; i += stride)
-        ma = max(ma, (long long)(arp[i + 1] - arp[i]));
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < BM; i += stride)
-        mb = max(mb, (long long)(brp[i + 1] - brp[i]));
+    const int64_t i0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // four rows per thread and step, their offsets loaded together
+    for (int64_t i = i0; i < M; i += 4 * stride) {
+        int64_t d[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int64_t k = i + u * stride;
+            d[u] = k < M ? (int64_t)(arp[k + 1] - arp[k]) : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) ma = max(ma, (long long)d[u]);
+    }
+    for (int64_t i = i0; i < BM; i += 4 * stride) {
+        int64_t d[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int64_t k = i + u * stride;
+            d[u] = k < BM ? (int64_t)(brp[k + 1] - brp[k]) : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) mb = max(mb, (long long)d[u]);
+    }
     for (int o = 32; o > 0; o >>= 1) {
         ma = max(ma, (long long)__shfl_xor(ma, o));
         mb = max(mb, (long long)__shfl_xor(mb, o));
@@ -132,21 +151,36 @@ __global__ void __launch_bounds__(kBinThreads) k_bin_hist(const int64_t *__restr
     // every row goes to the bin of ob.u without measuring it
     bool one = false;
     if (ob.part) {
-        __shared__ int dec;
-        if (threadIdx.x < 64) {
-            long long ma = 0, mb = 0;
-            for (int i = threadIdx.x; i < ob.nblk; i += 64) {
-                ma = max(ma, (long long)ob.part[2 * i]);
-                mb = max(mb, (long long)ob.part[2 * i + 1]);
-            }
-            for (int o = 32; o > 0; o >>= 1) {
-                ma = max(ma, (long long)__shfl_xor(ma, o));
-                mb = max(mb, (long long)__shfl_xor(mb, o));
-            }
-            if (threadIdx.x == 0) dec = ma * mb <= ob.bound;
+        static_assert(kMaxLenBlocks % kBinThreads == 0, "whole partials per thread");
+        __shared__ int pm[2][kBinThreads / 64];
+        // every thread loads its kMaxLenBlocks / kBinThreads partials at once
+        int a[kMaxLenBlocks / kBinThreads], b[kMaxLenBlocks / kBinThreads];
+#pragma unroll
+        for (int u = 0; u < kMaxLenBlocks / kBinThreads; u++) {
+            const int i = u * kBinThreads + threadIdx.x;
+            a[u] = ob.part[2 * i];
+            b[u] = ob.part[2 * i + 1];
+        }
+        int ma = 0, mb = 0;
+#pragma unroll
+        for (int u = 0; u < kMaxLenBlocks / kBinThreads; u++) {
+            ma = max(ma, a[u]);
+            mb = max(mb, b[u]);
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            ma = max(ma, __shfl_xor(ma, o));
+            mb = max(mb, __shfl_xor(mb, o));
+        }
+        if (lane() == 0) {
+            pm[0][threadIdx.x / 64] = ma;
+            pm[1][threadIdx.x / 64] = mb;
         }
         __syncthreads();
-        one = dec != 0;
+        for (int w = 0; w < kBinThreads / 64; w++) {
+            ma = max(ma, pm[0][w]);
+            mb = max(mb, pm[1][w]);
+        }
+        one = (long long)ma * mb <= ob.bound;
     }
     // workgroup 0 clears the caller's control block (zero_words words), which
     // the kernels after this one accumulate into; word ob.word (if any)
@@ -160,8 +194,12 @@ __global__ void __launch_bounds__(kBinThreads) k_bin_hist(const int64_t *__restr
     const int64_t r = (int64_t)blockIdx.x * kBinThreads + threadIdx.x;
     long long u = 0;
     int kspan = 0;
-    if (r < M) {
-        u = one ? ob.u : row_measure(r, arp, acol, brp, meas1, meas2, kspan);
+    if (one) {
+        // every row of the workgroup in the bin of ob.u; no bound stored
+        // (the scatter lists the rows in row order without reading them)
+        if (threadIdx.x == 0) h[bin_of(ob.u, thr)] = (int)min((int64_t)kBinThreads, M - blockIdx.x * (int64_t)kBinThreads);
+    } else if (r < M) {
+        u = row_measure(r, arp, acol, brp, meas1, meas2, kspan);
         if (ub) ub[r] = u;
         if (u <= 0 && rownnz) rownnz[r] = 0;
         const int b = bin_of(u, thr);
@@ -302,6 +340,7 @@ struct CtlExport {
     int pred, big_from;
     bool reassoc_possible;
     const unsigned long long *totals;
+    int onebin_word;  // >= 0: src[onebin_word] != 0 lists every row in row order
 };
 
 __global__ void __launch_bounds__(kBinThreads) k_bin_scatter(const int64_t *__restrict__ ub, const int32_t *__restrict__ meas1,
@@ -311,22 +350,26 @@ __global__ void __launch_bounds__(kBinThreads) k_bin_scatter(const int64_t *__re
                                                               int merge_lim, int *__restrict__ mcounts, CtlExport ex)
 {
     __shared__ int h[kMaxBins + 1];
-    __shared__ int mdest[kMaxBins + 1], mstart[kMaxBins + 1];
+    // the merge plan in LDS (indexed by bin: no private-memory arrays)
+    __shared__ int mdest[kMaxBins + 1], mstart[kMaxBins + 1], cnt[kMaxBins + 1];
+    __shared__ int sone_sh;
     const int nbin = thr.nb + 1;
     if ((int)threadIdx.x < nbin) h[threadIdx.x] = 0;
     if (threadIdx.x == 0) {
-        int dest[kMaxBins + 1], start[kMaxBins + 1], cnt[kMaxBins + 1];
+        const bool one = ex.src && ex.onebin_word >= 0 && ex.pred >= 0 && ex.src[ex.onebin_word] != 0u;
+        sone_sh = one;
+    }
+    __syncthreads();
+    // in one-bin mode only workgroup 0 needs the plan (for the populations
+    // it publishes); the others list their rows in row order
+    if (threadIdx.x == 0 && (sone_sh == 0 || blockIdx.x == 0)) {
         if (merge_lim > 0) {
-            merge_plan(raw, thr.nb, merge_lim, dest, start, cnt);
+            merge_plan(raw, thr.nb, merge_lim, mdest, mstart, cnt);
         } else {
             for (int b = 0; b < nbin; b++) {
-                dest[b] = b;
-                start[b] = 0;
+                mdest[b] = b;
+                mstart[b] = 0;
             }
-        }
-        for (int b = 0; b < nbin; b++) {
-            mdest[b] = dest[b];
-            mstart[b] = start[b];
         }
         if (merge_lim > 0 && blockIdx.x == 0)
             for (int b = 0; b < nbin; b++) mcounts[b] = cnt[b];
@@ -350,8 +393,13 @@ __global__ void __launch_bounds__(kBinThreads) k_bin_scatter(const int64_t *__re
             __hip_atomic_store(ex.flag, ex.seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
     }
+    const bool sone = sone_sh != 0;
     const int64_t r = (int64_t)blockIdx.x * kBinThreads + threadIdx.x;
     if (r >= M) return;
+    if (sone) {  // one-bin mode: bin ex.pred (never merged) lists every row in row order
+        lists[(int64_t)ex.pred * M + r] = (int32_t)r;
+        return;
+    }
     long long u;
     if (meas1) {
         int v = meas1[r];
@@ -1692,8 +1740,14 @@ static void launch_bins(const SpOperand *A, const SpOperand *B, const int32_t *m
     ob.word = -1;
     if (one && A && B) {
         ob = *one;
-        hipLaunchKernelGGL(k_row_maxlen, dim3(kMaxLenBlocks), dim3(256), 0, st, A->rp, M, B->rp, B->M,
-                           const_cast<int *>(ob.part));
+        // the int32 offsets when the operands have them (half the bytes)
+        int *part = const_cast<int *>(ob.part);
+        if (A->rp32 && B->rp32)
+            hipLaunchKernelGGL((k_row_maxlen<int32_t, int32_t>), dim3(kMaxLenBlocks), dim3(256), 0, st, A->rp32, M,
+                               B->rp32, B->M, part);
+        else
+            hipLaunchKernelGGL((k_row_maxlen<int64_t, int64_t>), dim3(kMaxLenBlocks), dim3(256), 0, st, A->rp, M,
+                               B->rp, B->M, part);
     }
     const int nblk = (int)((M + kBinThreads - 1) / kBinThreads);
     int *hist = scratch;
@@ -1718,7 +1772,8 @@ void launch_ub_bin(const SpOperand &A, const SpOperand &B, int64_t *ub, int32_t 
                    int span_lim, int *pred_count, int pred, int big_from, bool reassoc_possible, const OneBin *one)
 {
     CtlExport e{static_cast<const uint32_t *>(ctl), static_cast<uint32_t *>(ctl_host), ctl_words, flag, seq,
-                ctl != nullptr, span_lim, pred_count, pred, big_from, reassoc_possible, totals};
+                ctl != nullptr, span_lim, pred_count, pred, big_from, reassoc_possible, totals,
+                ctl && one ? one->word : -1};
     launch_bins(&A, &B, nullptr, nullptr, A.M, ub, rownnz, lists, counts, thr, totals, scratch, st, merge_lim,
                 mcounts, ctl ? &e : nullptr, ctl ? one : nullptr);
 }

@@ -45,10 +45,10 @@ int64_t bin_scratch_ints(int64_t M);
 // when their product is at most `bound`, every row is binned with measure `u`
 // without measuring it, the per-row bounds are not summed (totals[0] stays
 // 0), and word `word` of the control block is 1.
-constexpr int kMaxLenBlocks = 256;
+constexpr int kMaxLenBlocks = 1024;
 struct OneBin {
-    const int *part;
-    int nblk;  // kMaxLenBlocks
+    const int *part;  // 2 * kMaxLenBlocks ints
+    int nblk;         // kMaxLenBlocks
     long long bound, u;
     int word;
 };
