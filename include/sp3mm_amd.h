@@ -67,9 +67,11 @@ extern double Start, End, Elapsed, ElapsedInternal;
 /* --- triple products ---------------------------------------------------- */
 /* code/src/Sp3MM_CSR_OMP_UB_Generic.c:456-494 (declared in
  * code/src/include/Sp3MM_CSR_OMP_UB_Generic.h:44-50): (R*AC)*P as two SpMMs.
- * When spmm is NULL or one of this library's SpMM entry points the whole pair
- * runs on the GPU with the intermediate kept in HBM; any other function pointer
- * is called twice on the host, as in the reference. */
+ * When spmm is NULL or one of this library's SpMM entry points the product
+ * runs on the GPU with the direct kernels (the same pattern, FP64 values
+ * within the summation-order tolerance; sp3mm_rap with SP3MM_PATH_PAIR gives
+ * the two-SpGEMM form); any other function pointer is called twice on the
+ * host, as in the reference. */
 spmat *sp3mmRowByRowPair_0(spmat *R, spmat *AC, spmat *P, CONFIG *cfg, SPMM_INTERF spmm);
 spmat *sp3mmRowByRowPair_1(spmat *R, spmat *AC, spmat *P, CONFIG *cfg, SPMM_INTERF spmm);
 

