@@ -1054,6 +1054,48 @@ __global__ void k_scan_add(int64_t *__restrict__ out, int64_t n, const long long
     }
 }
 
+// The second phase of a short scan (at most kScanShortTiles tiles) without
+// the tile-sum scan kernel: every workgroup sums the raw tile sums before its
+// rows' tile itself (and all of them for the total), then adds.
+constexpr int kScanShortTiles = 2048;
+__global__ void __launch_bounds__(256) k_scan_add_short(int64_t *__restrict__ out, int64_t n,
+                                                         const long long *__restrict__ tile_sums, int ntiles,
+                                                         int64_t *__restrict__ total_out, int32_t *__restrict__ out32)
+{
+    static_assert(kScanTile % 256 == 0, "a workgroup's rows lie in one tile");
+    __shared__ long long sp[256 / 64], st[256 / 64];
+    const int64_t i0 = (int64_t)blockIdx.x * 256;
+    const int tile = (int)(i0 / kScanTile);
+    long long pre = 0, tot = 0;
+    for (int j = threadIdx.x; j < ntiles; j += 256) {
+        const long long v = tile_sums[j];
+        tot += v;
+        pre += j < tile ? v : 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        pre += __shfl_xor(pre, o);
+        tot += __shfl_xor(tot, o);
+    }
+    if (lane() == 0) {
+        sp[threadIdx.x / 64] = pre;
+        st[threadIdx.x / 64] = tot;
+    }
+    __syncthreads();
+    pre = sp[0] + sp[1] + sp[2] + sp[3];
+    tot = st[0] + st[1] + st[2] + st[3];
+    const int64_t i = i0 + threadIdx.x;
+    if (i < n) {
+        const long long v = out[i] + pre;
+        out[i] = v;
+        if (out32 && tot < 2147483647LL) out32[i] = (int32_t)v;
+    }
+    if (i == n) {
+        out[n] = tot;
+        if (out32 && tot < 2147483647LL) out32[n] = (int32_t)tot;
+        *total_out = tot;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // long rows: one workgroup per row on a dense accumulator in HBM
 // ---------------------------------------------------------------------------
@@ -2422,6 +2464,11 @@ void launch_scan(const int32_t *in, int64_t n, int64_t *out, int32_t *out32, lon
     if (n > 0)
         hipLaunchKernelGGL(k_scan_tiles, dim3((unsigned)ntiles), dim3(kScanThreads), 0, st, in, n, out,
                            tile_sums);
+    if (ntiles <= kScanShortTiles) {  // two kernels: the adding one sums the tile sums it needs itself
+        hipLaunchKernelGGL(k_scan_add_short, dim3(blocks_for(n + 1, 256)), dim3(256), 0, st, out, n, tile_sums,
+                           (int)ntiles, total, out32);
+        return;
+    }
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kScanThreads), 0, st, tile_sums, ntiles, total);
     hipLaunchKernelGGL(k_scan_add, dim3(blocks_for(n + 1, 256)), dim3(256), 0, st, out, n, tile_sums, total,
                        out32);
