@@ -211,6 +211,34 @@ __device__ __forceinline__ int key_rank(const int *keys, int n, int k)
     return r;
 }
 
+// Sorts keys[0..N) ascending, moving vals along, by a bitonic network in LDS:
+// N is a power of two, the keys are distinct column indices padded with
+// INT32_MAX, the G lanes of a group share the compare-exchanges of a stage and
+// synchronise between stages.  (log2 N)(log2 N + 1)/2 stages of N/2
+// compare-exchanges, against n^2 comparisons for ranking every key.
+template <int G, typename V>
+__device__ void bitonic_sort(int *keys, V *vals, int N, int t)
+{
+    for (int size = 2; size <= N; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = t; i < N / 2; i += G) {
+                const int a = ((i & ~(stride - 1)) << 1) | (i & (stride - 1));
+                const int b = a + stride;
+                const bool up = (a & size) == 0;
+                const int ka = keys[a], kb = keys[b];
+                if ((ka > kb) == up) {
+                    keys[a] = kb;
+                    keys[b] = ka;
+                    const V va = vals[a];
+                    vals[a] = vals[b];
+                    vals[b] = va;
+                }
+            }
+            Group<G>::sync();
+        }
+    }
+}
+
 // exclusive scan of one value per thread over the workgroup; `sh` holds one
 // long long per wavefront; *total receives the workgroup sum
 __device__ __forceinline__ long long block_exclusive_scan(long long v, long long *sh, long long &total)

@@ -910,11 +910,28 @@ __global__ void __launch_bounds__(256) k_spgemm_fused(SpOperand A, SpOperand B, 
         Group<G>::sync();
         const int nn = compact_table<G, double>(k, v, H, ired);
         const int64_t off = (int64_t)i * H;
-        for (int j = t; j < nn; j += G) {
-            const int key = k[j];
-            const int rk = key_rank(k, nn, key);
-            scol[off + rk] = key;
-            sval[off + rk] = v[j];
+        if (nn > (G > kWave ? 512 : 256)) {
+            // long rows: sort in LDS (padded to a power of two, at most H)
+            // and write in order.  Ranking costs n^2/G comparisons per lane;
+            // the bitonic network (log2 N)^2/2 stages, each a dependent LDS
+            // round trip plus a barrier, which pays from about 256 keys
+            // (one-wavefront groups) or 512 (multi-wavefront groups) on
+            int N = 1;
+            while (N < nn) N <<= 1;
+            for (int j = nn + t; j < N; j += G) k[j] = INT32_MAX;
+            Group<G>::sync();
+            bitonic_sort<G, double>(k, v, N, t);
+            for (int j = t; j < nn; j += G) {
+                scol[off + j] = k[j];
+                sval[off + j] = v[j];
+            }
+        } else {
+            for (int j = t; j < nn; j += G) {
+                const int key = k[j];
+                const int rk = key_rank(k, nn, key);
+                scol[off + rk] = key;
+                sval[off + rk] = v[j];
+            }
         }
         if (t == 0) {
             slotn[i] = nn;
