@@ -557,7 +557,7 @@ def main():
         log(f"schedule installed: {schedule}")
     elif not args.no_tune and path == lib.DIRECT:
         t0 = time.perf_counter()
-        schedule, ms_best, ms_default = lib.autotune(dR, dA, dP, reps=3)
+        schedule, ms_best, ms_default = lib.autotune(dR, dA, dP, reps=9)
         tune_s = time.perf_counter() - t0
         log(f"autotune: {schedule} {ms_best:.4f} ms (default schedule {ms_default:.4f} ms) in {tune_s:.1f}s")
 

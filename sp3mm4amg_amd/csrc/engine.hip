@@ -2080,9 +2080,11 @@ void queue_jacobi(const sp3mm_dmat &A, const double *b, double *x, double *work,
 // --------------------------------------------------------------------------
 // autotuner: times the direct product of the caller's operands under candidate
 // schedules and installs the fastest in the context.  Coordinate descent from
-// the current setting, one parameter at a time; `reps` timed products per
-// candidate after one warm-up (device time by HIP events, median); a candidate
-// is taken only when it is at least 1% faster.
+// the current setting, one parameter at a time, over all parameters twice
+// when the first pass changed anything; `reps` timed products per candidate
+// after one warm-up (device time by HIP events, median).  Every candidate
+// that looks faster at all is timed again interleaved with the current
+// schedule, and taken only when that shows it at least 1% faster.
 // --------------------------------------------------------------------------
 int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int prec, double guard, int reps,
              Tuning &best, double &ms_best, double &ms_default)
@@ -2146,29 +2148,36 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
         {&Tuning::h1_shift, {-1, 0, 1}},            {&Tuning::h2_div, {2, 4, 8}},
         {&Tuning::xcd_map, {1, 0}},
     };
-    for (const Axis &ax : axes) {
-        for (int v : ax.values) {
-            if (cur.*ax.field == v) continue;
-            Tuning t = cur;
-            t.*ax.field = v;
-            if (!tuning_valid(t)) continue;
-            double ms = 0;
-            if (measure(t, ms)) {
-                c->tune = saved;
-                return -1;
-            }
-            if (ms < 0.99 * cur_ms) {
-                double ms_t = 0, ms_cur = 0;
-                if (confirm(t, cur, ms_t, ms_cur)) {
+    for (int pass = 0; pass < 2; pass++) {
+        bool changed = false;
+        for (const Axis &ax : axes) {
+            for (int v : ax.values) {
+                if (cur.*ax.field == v) continue;
+                Tuning t = cur;
+                t.*ax.field = v;
+                if (!tuning_valid(t)) continue;
+                double ms = 0;
+                if (measure(t, ms)) {
                     c->tune = saved;
                     return -1;
                 }
-                if (ms_t < 0.99 * ms_cur) {
-                    cur = t;
-                    cur_ms = ms_t;
+                // a median of a few products is noisy: anything that looks
+                // faster is confirmed against the current schedule
+                if (ms < cur_ms) {
+                    double ms_t = 0, ms_cur = 0;
+                    if (confirm(t, cur, ms_t, ms_cur)) {
+                        c->tune = saved;
+                        return -1;
+                    }
+                    if (ms_t < 0.99 * ms_cur) {
+                        cur = t;
+                        cur_ms = ms_t;
+                        changed = true;
+                    }
                 }
             }
         }
+        if (!changed) break;
     }
     c->tune = cur;
     best = cur;
