@@ -552,7 +552,7 @@ def main():
     flop = 2 * (mac1 + mac2)
     schedule, tune_s = None, 0.0
     if args.schedule:
-        lib.set_tuning({k: int(v) for k, v in (f.split("=") for f in args.schedule.split(","))})
+        lib.set_tuning({k: int(v, 0) for k, v in (f.split("=") for f in args.schedule.split(","))})
         schedule = lib.get_tuning()
         log(f"schedule installed: {schedule}")
     elif not args.no_tune and path == lib.DIRECT:

@@ -623,6 +623,8 @@ static void to_c(const Tuning &t, sp3mm_tuning &o)
     o.h1_shift = t.h1_shift;
     o.h2_div = t.h2_div;
     o.xcd_map = t.xcd_map;
+    o.hash1 = t.hash1;
+    o.hash2 = t.hash2;
 }
 
 int sp3mm_get_tuning(sp3mm_tuning *t)
@@ -651,6 +653,8 @@ int sp3mm_set_tuning(const sp3mm_tuning *t)
         n.h1_shift = t->h1_shift;
         n.h2_div = t->h2_div;
         n.xcd_map = t->xcd_map;
+        n.hash1 = t->hash1 ? t->hash1 : kHashMul;
+        n.hash2 = t->hash2 ? t->hash2 : kHashMul;
         if (!tuning_valid(n)) {
             set_error("set_tuning: parameter out of range");
             return -1;

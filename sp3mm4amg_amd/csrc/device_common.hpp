@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "kernels.hpp"
+
 namespace sp3mm {
 
 constexpr int kWave = 64;
@@ -125,11 +127,13 @@ __device__ __forceinline__ long long group_max(long long v, long long *red)
 // LDS hash table with linear probing.  Keys are column indices (>= 0), the
 // empty slot is -1.  `mask` = H-1 for a power-of-two table of H slots.  A probe
 // sequence is bounded by H: a full table raises kErrHashFull instead of
-// spinning forever.
+// spinning forever.  A key's home slot is the top bits of key * mul (`mul`
+// odd); the direct kernel's two tables take their multipliers from the
+// schedule, which the autotuner fits to the operands (k_hash_score).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ unsigned hslot(int key, int shift)
+__device__ __forceinline__ unsigned hslot(int key, int shift, unsigned mul = kHashMul)
 {
-    return ((unsigned)key * 0x9E3779B1u) >> shift;
+    return ((unsigned)key * mul) >> shift;
 }
 
 // returns slot index, or -1 when the table is full; *fresh = 1 for a new key.

@@ -111,7 +111,8 @@ int tuning_valid(const Tuning &t)
                                             (t.subwave_h & (t.subwave_h - 1)) == 0);
     return (t.u1 == 2 || t.u1 == 4) && (t.u2 >= 2 && t.u2 <= 4) && pow2h && t.slog1_cap >= 0 &&
            t.slog1_cap <= 8 && t.slog2_cap >= 0 && t.slog2_cap <= 8 && t.h1_shift >= -1 && t.h1_shift <= 2 &&
-           (t.h2_div == 1 || t.h2_div == 2 || t.h2_div == 4 || t.h2_div == 8) && (t.xcd_map == 0 || t.xcd_map == 1);
+           (t.h2_div == 1 || t.h2_div == 2 || t.h2_div == 4 || t.h2_div == 8) && (t.xcd_map == 0 || t.xcd_map == 1) &&
+           (t.hash1 & 1u) && (t.hash2 & 1u);
 }
 
 // initial scheduling: defaults, overridden by SP3MM_FUSED_U ("ab", a and b in
@@ -1087,6 +1088,17 @@ static void fused_bins(size_t max_lds, bool mixed, const Tuning &tu, BinThreshol
     cfg.push_back({0, 0, 0});
 }
 
+// The bin the mean row bound nnz(R)/M * nnz(A)/A.M falls in, -1 beyond the
+// bins: in a mesh operator most rows fall in it.
+static int likely_bin(const BinThresholds &thr, const sp3mm_dmat &R, const sp3mm_dmat &A)
+{
+    if (R.M == 0 || A.M == 0) return -1;
+    const double est = (double)R.nnz / (double)R.M * ((double)A.nnz / (double)A.M);
+    int b = 0;
+    while (b < thr.nb && (double)thr.v[b] < est) b++;
+    return b >= thr.nb ? -1 : b;
+}
+
 // FusedPlan: the one-pass direct pipeline.  build() runs it eagerly, reading
 // counters back to size bins, scratch slots and the output; record() enqueues
 // the same kernels for the sizes found by build() with no host
@@ -1184,13 +1196,9 @@ struct FusedPlan {
     // reassociated, so that launch then ends at once.
     int predict_bin(Context *c) const
     {
-        if (mixed || ext_out || R.M == 0 || A.M == 0 || R.M >= INT32_MAX) return -1;
-        const double est = (double)R.nnz / (double)R.M * ((double)A.nnz / (double)A.M);
-        int b = 0;
-        while (b < thr.nb && (double)thr.v[b] < est) b++;
-        if (b >= thr.nb) return -1;
+        if (mixed || ext_out || R.M >= INT32_MAX) return -1;
         (void)c;
-        return b;
+        return likely_bin(thr, R, A);
     }
 
     void launch_main(Context *c, hipStream_t s, int i, bool early = false)
@@ -2078,13 +2086,89 @@ void queue_jacobi(const sp3mm_dmat &A, const double *b, double *x, double *work,
 }
 
 // --------------------------------------------------------------------------
+// hash multipliers of the direct kernel's tables
+// --------------------------------------------------------------------------
+// Candidates: kHashMul, then odd constants from a fixed splitmix64 sequence
+// (the same list in every process).
+static const std::vector<unsigned> &hash_candidates()
+{
+    static const std::vector<unsigned> cand = [] {
+        std::vector<unsigned> v{kHashMul};
+        uint64_t x = 0x243F6A8885A308D3ull;
+        while (v.size() < 256) {
+            x += 0x9E3779B97F4A7C15ull;
+            uint64_t z = x;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            z ^= z >> 31;
+            v.push_back((unsigned)(z >> 32) | 1u);
+        }
+        return v;
+    }();
+    return cand;
+}
+
+// Tables of the likely bin under schedule t (false: no such bin, or tables
+// beyond what k_hash_score takes)
+static bool likely_tables(Context *c, const Tuning &t, const sp3mm_dmat &R, const sp3mm_dmat &A, int &H1, int &H2)
+{
+    BinThresholds thr{};
+    std::vector<FusedCfg> cfg;
+    fused_bins(c->max_lds, false, t, thr, cfg);
+    const int b = likely_bin(thr, R, A);
+    if (b < 0) return false;
+    H1 = cfg[b].H1;
+    H2 = cfg[b].H2;
+    return H1 <= kScoreTable && H2 <= kScoreTable;
+}
+
+// For tables of H1 and H2 slots: the candidates with the smallest summed
+// home-slot distance of the keys of 512 sample rows of R (evenly spaced), at
+// stage 1 (h1) and stage 2 (h2); kHashMul wins ties.
+static int score_hashes(Context *c, const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int H1, int H2,
+                        unsigned &h1, unsigned &h2)
+{
+    const std::vector<unsigned> &cand = hash_candidates();
+    const int nc = (int)cand.size();
+    const int nrows = (int)std::min<int64_t>(R.M, 512);
+    h1 = h2 = kHashMul;
+    if (nrows <= 0 || !A.rp32 || !P.rp32 || H1 > kScoreTable || H2 > kScoreTable) return 0;
+    std::vector<int32_t> rows((size_t)nrows);
+    for (int i = 0; i < nrows; i++) rows[(size_t)i] = (int32_t)(((2 * (int64_t)i + 1) * R.M) / (2 * (int64_t)nrows));
+    Buf drows((size_t)nrows * 4), dcand((size_t)nc * 4), dscore((size_t)nc * 2 * 8);
+    if (!drows.p || !dcand.p || !dscore.p) {
+        set_error("hash scoring: allocation failed");
+        return -1;
+    }
+    hipStream_t s = c->stream;
+    SP3MM_HIP(hipMemcpyAsync(drows.p, rows.data(), (size_t)nrows * 4, hipMemcpyHostToDevice, s));
+    SP3MM_HIP(hipMemcpyAsync(dcand.p, cand.data(), (size_t)nc * 4, hipMemcpyHostToDevice, s));
+    auto *sc = dscore.as<unsigned long long>();
+    launch_hash_score(operand(R), operand(A), operand(P), drows.as<int32_t>(), nrows, dcand.as<unsigned>(), nc, H1,
+                      H2, sc, sc + nc, s);
+    SP3MM_HIP(hipGetLastError());
+    std::vector<unsigned long long> h((size_t)nc * 2);
+    SP3MM_HIP(hipMemcpyAsync(h.data(), sc, (size_t)nc * 2 * 8, hipMemcpyDeviceToHost, s));
+    SP3MM_HIP(hipStreamSynchronize(s));
+    int b1 = 0, b2 = 0;
+    for (int i = 1; i < nc; i++) {
+        if (h[(size_t)i] < h[(size_t)b1]) b1 = i;
+        if (h[(size_t)(nc + i)] < h[(size_t)(nc + b2)]) b2 = i;
+    }
+    h1 = cand[(size_t)b1];
+    h2 = cand[(size_t)b2];
+    return 0;
+}
+
+// --------------------------------------------------------------------------
 // autotuner: times the direct product of the caller's operands under candidate
 // schedules and installs the fastest in the context.  Coordinate descent from
 // the current setting, one parameter at a time, over all parameters twice
 // when the first pass changed anything; `reps` timed products per candidate
 // after one warm-up (device time by HIP events, median).  Every candidate
 // that looks faster at all is timed again interleaved with the current
-// schedule, and taken only when that shows it at least 1% faster.
+// schedule, and taken only when that shows it at least 1% faster.  Each pass
+// first tries the hash multipliers scored on the operands (score_hashes).
 // --------------------------------------------------------------------------
 int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int prec, double guard, int reps,
              Tuning &best, double &ms_best, double &ms_default)
@@ -2148,33 +2232,76 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
         {&Tuning::h1_shift, {-1, 0, 1}},            {&Tuning::h2_div, {2, 4, 8}},
         {&Tuning::xcd_map, {1, 0}},
     };
+    // a candidate is taken when its median looks faster and the interleaved
+    // re-timing confirms it at least 1% faster (a median of a few products
+    // is noisy)
+    auto step = [&](const Tuning &t, bool &took) -> int {
+        took = false;
+        double ms = 0;
+        if (measure(t, ms)) return -1;
+        if (ms < cur_ms) {
+            double ms_t = 0, ms_cur = 0;
+            if (confirm(t, cur, ms_t, ms_cur)) return -1;
+            if (ms_t < 0.99 * ms_cur) {
+                cur = t;
+                cur_ms = ms_t;
+                took = true;
+            }
+        }
+        return 0;
+    };
+    // Hash multipliers scored for a schedule's likely-bin tables
+    // (score_hashes, cached per pair of table sizes).  Once the scored
+    // multipliers are taken (`hashed`), every candidate schedule gets the ones
+    // scored for its own tables.  The float-pair kernel keeps the default.
+    std::map<std::pair<int, int>, std::pair<unsigned, unsigned>> hcache;
+    auto rehash = [&](Tuning &t) -> int {
+        int H1 = 0, H2 = 0;
+        if (prec == SP3MM_PREC_MIXED || !likely_tables(c, t, R, A, H1, H2)) return 0;
+        auto it = hcache.find({H1, H2});
+        if (it == hcache.end()) {
+            unsigned h1 = kHashMul, h2 = kHashMul;
+            if (score_hashes(c, R, A, P, H1, H2, h1, h2)) return -1;
+            it = hcache.emplace(std::make_pair(H1, H2), std::make_pair(h1, h2)).first;
+        }
+        t.hash1 = it->second.first;
+        t.hash2 = it->second.second;
+        return 0;
+    };
+    bool hashed = false;
     for (int pass = 0; pass < 2; pass++) {
-        bool changed = false;
+        bool changed = false, took = false;
+        if (!hashed) {
+            Tuning t = cur;
+            if (rehash(t)) {
+                c->tune = saved;
+                return -1;
+            }
+            if (t.hash1 == cur.hash1 && t.hash2 == cur.hash2) {
+                hashed = true;
+            } else {
+                if (step(t, took)) {
+                    c->tune = saved;
+                    return -1;
+                }
+                hashed = changed = took;
+            }
+        }
         for (const Axis &ax : axes) {
             for (int v : ax.values) {
                 if (cur.*ax.field == v) continue;
                 Tuning t = cur;
                 t.*ax.field = v;
-                if (!tuning_valid(t)) continue;
-                double ms = 0;
-                if (measure(t, ms)) {
+                if (hashed && rehash(t)) {
                     c->tune = saved;
                     return -1;
                 }
-                // a median of a few products is noisy: anything that looks
-                // faster is confirmed against the current schedule
-                if (ms < cur_ms) {
-                    double ms_t = 0, ms_cur = 0;
-                    if (confirm(t, cur, ms_t, ms_cur)) {
-                        c->tune = saved;
-                        return -1;
-                    }
-                    if (ms_t < 0.99 * ms_cur) {
-                        cur = t;
-                        cur_ms = ms_t;
-                        changed = true;
-                    }
+                if (!tuning_valid(t)) continue;
+                if (step(t, took)) {
+                    c->tune = saved;
+                    return -1;
                 }
+                changed = changed || took;
             }
         }
         if (!changed) break;

@@ -113,7 +113,8 @@ template <int G, typename LV, typename BV, int U>
 __device__ __forceinline__ void expand_flat(const int32_t *lcol, const LV *lval, int64_t e0, int64_t e1,
                                             const int32_t *__restrict__ brp, const int32_t *__restrict__ bcol,
                                             const BV *__restrict__ bval, int slog, int *keys, double *vals,
-                                            unsigned mask, int shift, unsigned dslot, bool &full, int *fullflag)
+                                            unsigned mask, int shift, unsigned dslot, bool &full, int *fullflag,
+                                            unsigned hmul = kHashMul)
 {
     constexpr int kLogG = G >= 256 ? 8 : G >= 128 ? 7 : G >= 64 ? 6 : G >= 32 ? 5 : 4;
     if (slog > kLogG) slog = kLogG;
@@ -156,7 +157,7 @@ __device__ __forceinline__ void expand_flat(const int32_t *lcol, const LV *lval,
                 q[u] += S;
             }
 #pragma unroll
-            for (int u = 0; u < U; u++) sl[u] = c[u] != kEmpty ? (hslot(c[u], shift) & mask) : dslot;
+            for (int u = 0; u < U; u++) sl[u] = c[u] != kEmpty ? (hslot(c[u], shift, hmul) & mask) : dslot;
 #pragma unroll
             for (int u = 0; u < U; u++) old[u] = atomicCAS(keys + sl[u], kEmpty, c[u]);
             // keep the U compare-and-swaps in flight together: nothing that
@@ -240,7 +241,7 @@ __device__ __forceinline__ void expand_flat_crowded(const int32_t *lcol, const L
                                                     const int32_t *__restrict__ bcol, const BV *__restrict__ bval,
                                                     int slog, int *keys, double *vals, unsigned mask, int shift,
                                                     unsigned dslot, bool &full, int *fullflag, int *claimed,
-                                                    int limit)
+                                                    int limit, unsigned hmul = kHashMul)
 {
     static_assert(G > kWave, "the claim count is kept per wavefront of a multi-wavefront group");
     constexpr int kLogG = G >= 256 ? 8 : G >= 128 ? 7 : G >= 64 ? 6 : G >= 32 ? 5 : 4;
@@ -288,7 +289,7 @@ __device__ __forceinline__ void expand_flat_crowded(const int32_t *lcol, const L
                 q[u] += S;
             }
 #pragma unroll
-            for (int u = 0; u < U; u++) sl[u] = c[u] != kEmpty ? (hslot(c[u], shift) & mask) : dslot;
+            for (int u = 0; u < U; u++) sl[u] = c[u] != kEmpty ? (hslot(c[u], shift, hmul) & mask) : dslot;
 #pragma unroll
             for (int u = 0; u < U; u++) old[u] = atomicCAS(keys + sl[u], kEmpty, c[u]);
             // keep the U compare-and-swaps in flight together: nothing that

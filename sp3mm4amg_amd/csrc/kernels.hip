@@ -732,7 +732,8 @@ __global__ void __launch_bounds__(256) k_direct(SpOperand R, SpOperand A, SpOper
 // entries in a scratch area (slot i of the bin list -> scratch + i*cap); its
 // length goes to rownnz[r] and slotn[i].  A row whose table 2 fills is marked
 // slotn[i] = -1 and queued on `retry` for the long-row kernels.  After the
-// scan of rownnz, k_gather_rows moves every slot to its final place.
+// scan of rownnz, k_gather_rows moves every slot to its final place.  Table 1
+// hashes with the multiplier hm1, table 2 with hm2.
 template <int G, int U1, int U2, typename VT = double>
 __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand A, SpOperand P,
                                                        const int32_t *__restrict__ rows,
@@ -741,7 +742,8 @@ __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand 
                                                        int32_t *__restrict__ slotn,
                                                        int32_t *__restrict__ scol, double *__restrict__ sval,
                                                        int32_t *__restrict__ retry, int *__restrict__ nretry,
-                                                       int *__restrict__ err, int nxcd, int cap_rows)
+                                                       int *__restrict__ err, int nxcd, int cap_rows, unsigned hm1,
+                                                       unsigned hm2)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int GPB = 256 / G;
@@ -802,10 +804,10 @@ __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand 
         // 7/8 of a table is given up, and the retry path redoes it
         if constexpr (G > kWave)
             expand_flat_crowded<G, VT, VT, U1>(R.col, rval, rb, re, A.rp32, A.col, aval, slog1, k1, v1, m1, sh1,
-                                               (unsigned)(H1 + H2 + t), full1, fullflag, claimed1, lim1);
+                                               (unsigned)(H1 + H2 + t), full1, fullflag, claimed1, lim1, hm1);
         else
             expand_flat<G, VT, VT, U1>(R.col, rval, rb, re, A.rp32, A.col, aval, slog1, k1, v1, m1, sh1,
-                                       (unsigned)(H1 + H2 + t), full1, fullflag);
+                                       (unsigned)(H1 + H2 + t), full1, fullflag, hm1);
         const int64_t rbn = R.rp[rn], ren = R.rp[rn + 1];
         Group<G>::sync();
         if constexpr (G > kWave) full1 = full1 || *claimed1 > lim1;
@@ -813,10 +815,10 @@ __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand 
         if (!full1 && !__hip_atomic_load(fullflag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
             if constexpr (G > kWave)
                 expand_flat_crowded<G, double, VT, U2>(k1, v1, 0, nn1, P.rp32, P.col, pval, slog2, k2, v2, m2, sh2,
-                                                       (unsigned)(H2 + t), full2, fullflag, claimed2, lim2);
+                                                       (unsigned)(H2 + t), full2, fullflag, claimed2, lim2, hm2);
             else
                 expand_flat<G, double, VT, U2>(k1, v1, 0, nn1, P.rp32, P.col, pval, slog2, k2, v2, m2, sh2,
-                                               (unsigned)(H2 + t), full2, fullflag);
+                                               (unsigned)(H2 + t), full2, fullflag, hm2);
         }
         if constexpr (G > kWave) {
             Group<G>::sync();  // every wavefront's claims are counted
@@ -850,6 +852,114 @@ __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand 
         r = rn;
         rb = rbn;
         re = ren;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// hash multipliers for the direct kernel's two tables
+// ---------------------------------------------------------------------------
+// The stage-1 keys of row r of R*A*P are the columns of the rows of A that
+// row r of R selects, the stage-2 keys the columns of the rows of P that
+// those keys select.  In a mesh-ordered operator the key sets of most rows
+// are translates of one stencil, and whether a multiplier sends such a set to
+// distinct slots depends on the multiplier and the strides, not on the row:
+// some multipliers leave no key off its home slot, where others make every
+// pass of the direct kernel probe.  A workgroup takes one sample row: it
+// collects the row's distinct stage-1 and stage-2 keys (dedup tables with the
+// default multiplier, compacted), then each wavefront takes candidates
+// c = w, w + 4, ...: it inserts both key sets by linear probing into tables of
+// H1 and H2 slots, as the direct kernel does, and adds the keys' distances
+// from their home slots to score1[c] and score2[c].  For distinct keys under
+// linear probing that sum does not depend on the insert order, so the scores
+// are deterministic.  A row with more distinct keys at a stage than its
+// dedup table takes (kScoreDedup / 2), or than 7/8 of H1 (H2), is skipped for
+// that stage by every candidate.
+constexpr int kScoreDedup = 2048;
+__global__ void __launch_bounds__(256) k_hash_score(SpOperand R, SpOperand A, SpOperand P,
+                                                   const int32_t *__restrict__ rows, const unsigned *__restrict__ cand,
+                                                   int ncand, int H1, int H2,
+                                                   unsigned long long *__restrict__ score1,
+                                                   unsigned long long *__restrict__ score2)
+{
+    __shared__ int d1[kScoreDedup], d2[kScoreDedup];
+    __shared__ int red[Group<256>::kWaves + 1];
+    __shared__ int over;
+    extern __shared__ int tabs[];  // per wavefront: H1 + H2 slots
+    const int t = threadIdx.x, w = t / kWave, l = t % kWave;
+    const int r = rows[blockIdx.x];
+    constexpr unsigned dmask = kScoreDedup - 1;
+    constexpr int dshift = 32 - 11;  // log2(kScoreDedup) == 11
+    static_assert(kScoreDedup == 2048, "dshift");
+    for (int i = t; i < kScoreDedup; i += 256) {
+        d1[i] = kEmpty;
+        d2[i] = kEmpty;
+    }
+    if (t == 0) over = 0;
+    __syncthreads();
+    // stage-1 keys: every lane takes entries of R's row and walks their rows of A
+    const int64_t rb = R.rp[r], re = R.rp[r + 1];
+    for (int64_t e = rb + t; e < re; e += 256) {
+        const int k = R.col[e];
+        for (int q = A.rp32[k]; q < A.rp32[k + 1]; q++) {
+            int fresh = 0;
+            if (hash_insert(d1, dmask, dshift, A.col[q], &fresh) < 0) over = 1;
+        }
+    }
+    __syncthreads();
+    const int n1 = over ? kScoreDedup : compact_table<256, double>(d1, nullptr, kScoreDedup, red);
+    __syncthreads();
+    // stage-2 keys: the columns of the rows of P that the stage-1 keys select
+    if (n1 <= kScoreDedup / 2) {
+        for (int i = t; i < n1; i += 256) {
+            const int k = d1[i];
+            for (int q = P.rp32[k]; q < P.rp32[k + 1]; q++) {
+                int fresh = 0;
+                if (hash_insert(d2, dmask, dshift, P.col[q], &fresh) < 0) over = 1;
+            }
+        }
+    }
+    __syncthreads();
+    const int n2 = (over || n1 > kScoreDedup / 2) ? kScoreDedup : compact_table<256, double>(d2, nullptr, kScoreDedup, red);
+    const bool use1 = n1 <= kScoreDedup / 2 && n1 <= H1 - H1 / 8;
+    const bool use2 = n2 <= kScoreDedup / 2 && n2 <= H2 - H2 / 8;
+    int *t1 = tabs + w * (H1 + H2), *t2 = t1 + H1;
+    const unsigned m1 = (unsigned)H1 - 1, m2 = (unsigned)H2 - 1;
+    const int sh1 = 32 - ilog2((unsigned)H1), sh2 = 32 - ilog2((unsigned)H2);
+    for (int c = w; c < ncand; c += Group<256>::kWaves) {
+        const unsigned mul = cand[c];
+        for (int i = l; i < H1 + H2; i += kWave) t1[i] = kEmpty;
+        wave_sync();
+        unsigned long long dist1 = 0, dist2 = 0;
+        // keys are distinct: a swap either claims an empty slot or moves on
+        if (use1)
+            for (int i = l; i < n1; i += kWave) {
+                const int k = d1[i];
+                const unsigned home = hslot(k, sh1, mul) & m1;
+                unsigned s = home;
+                for (unsigned probe = 0; probe <= m1; probe++) {
+                    if (atomicCAS(t1 + s, kEmpty, k) == kEmpty) break;
+                    s = (s + 1) & m1;
+                }
+                dist1 += (s - home) & m1;
+            }
+        if (use2)
+            for (int i = l; i < n2; i += kWave) {
+                const int k = d2[i];
+                const unsigned home = hslot(k, sh2, mul) & m2;
+                unsigned s = home;
+                for (unsigned probe = 0; probe <= m2; probe++) {
+                    if (atomicCAS(t2 + s, kEmpty, k) == kEmpty) break;
+                    s = (s + 1) & m2;
+                }
+                dist2 += (s - home) & m2;
+            }
+        dist1 = (unsigned long long)group_sum<kWave>((long long)dist1, nullptr);
+        dist2 = (unsigned long long)group_sum<kWave>((long long)dist2, nullptr);
+        if (l == 0) {
+            if (dist1) atomicAdd(score1 + c, dist1);
+            if (dist2) atomicAdd(score2 + c, dist2);
+        }
+        wave_sync();
     }
 }
 
@@ -2339,7 +2449,8 @@ size_t fused_lds(int G, int H1, int H2)
 }
 
 using FusedFn = void (*)(SpOperand, SpOperand, SpOperand, const int32_t *, const int *, int, int, int, int,
-                        int32_t *, int32_t *, int32_t *, double *, int32_t *, int *, int *, int, int);
+                        int32_t *, int32_t *, int32_t *, double *, int32_t *, int *, int *, int, int, unsigned,
+                        unsigned);
 
 // left entries held per lane: stage 1 U1 in {2, 4}, stage 2 U2 in {2, 3, 4}
 template <int G, typename VT>
@@ -2372,7 +2483,19 @@ void launch_direct_fused(int G, int H1, int H2, int slog1, int slog2, const SpOp
                                           : fused_variant<256, double>(u1, u2));
     const int nxcd = tu.xcd_map && grid >= (unsigned)kXcds ? kXcds : 1;
     hipLaunchKernelGGL(fn, dim3(grid), dim3(256), lds, st, R, A, P, rows, count, H1, H2, slog1, slog2, rownnz,
-                       slotn, scol, sval, retry, nretry, err, nxcd, rows_host);
+                       slotn, scol, sval, retry, nretry, err, nxcd, rows_host, tu.hash1 | 1u, tu.hash2 | 1u);
+}
+
+void launch_hash_score(const SpOperand &R, const SpOperand &A, const SpOperand &P, const int32_t *rows, int nrows,
+                       const unsigned *cand, int ncand, int H1, int H2, unsigned long long *score1,
+                       unsigned long long *score2, hipStream_t st)
+{
+    (void)hipMemsetAsync(score1, 0, sizeof(unsigned long long) * ncand, st);
+    (void)hipMemsetAsync(score2, 0, sizeof(unsigned long long) * ncand, st);
+    if (nrows <= 0) return;
+    const size_t lds = (size_t)Group<256>::kWaves * (H1 + H2) * sizeof(int);
+    hipLaunchKernelGGL(k_hash_score, dim3(nrows), dim3(256), lds, st, R, A, P, rows, cand, ncand, H1, H2, score1,
+                       score2);
 }
 
 void launch_gather_rows(const int32_t *rows, const int *count, int rows_host, const int32_t *slotn, int cap,

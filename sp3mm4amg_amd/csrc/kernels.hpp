@@ -18,6 +18,10 @@ struct SpOperand {
 
 constexpr int kMaxBins = 14;
 
+// Default multiplier of the LDS hash tables (slot = top bits of key * mul):
+// 2^32 / golden ratio, rounded to odd.
+constexpr unsigned kHashMul = 0x9E3779B1u;
+
 // Row goes to the first bin b with measure <= v[b]; above v[nb-1] -> bin nb.
 struct BinThresholds {
     int nb;
@@ -133,6 +137,7 @@ struct Tuning {
     int h2_div = 4;         // stage-2 table = max(64, H1 / h2_div)
     int xcd_map = 1;        // 1: each XCD (block % 8) works through its own contiguous part of
                             // a bin's row list (rows that share A and P rows share one L2)
+    unsigned hash1 = kHashMul, hash2 = kHashMul;  // multipliers of the stage-1 / stage-2 table hashes (odd)
 };
 
 void launch_direct_fused(int G, int H1, int H2, int slog1, int slog2, const SpOperand &R, const SpOperand &A,
@@ -140,6 +145,13 @@ void launch_direct_fused(int G, int H1, int H2, int slog1, int slog2, const SpOp
                          int32_t *rownnz, int32_t *slotn, int32_t *scol, double *sval, int32_t *retry,
                          int *nretry, int *err, int max_blocks, hipStream_t st, bool f32in,
                          const Tuning &tu);
+// Scores of candidate hash multipliers for the direct kernel's tables of H1 and
+// H2 slots (both at most kScoreTable) on sample rows of R (see k_hash_score):
+// score1[c] / score2[c] (zeroed here) receive the summed home-slot distances.
+constexpr int kScoreTable = 1024;
+void launch_hash_score(const SpOperand &R, const SpOperand &A, const SpOperand &P, const int32_t *rows, int nrows,
+                       const unsigned *cand, int ncand, int H1, int H2, unsigned long long *score1,
+                       unsigned long long *score2, hipStream_t st);
 // one-pass SpGEMM rows into scratch slots of H entries (see k_spgemm_fused)
 size_t spgemm_fused_lds(int G, int H);
 void launch_spgemm_fused(int G, int H, int slog, const SpOperand &A, const SpOperand &B, const int32_t *rows,

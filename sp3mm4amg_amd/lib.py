@@ -70,7 +70,8 @@ class Stats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("u1", C.c_int32), ("u2", C.c_int32), ("subwave_h", C.c_int32),
                 ("slog1_cap", C.c_int32), ("slog2_cap", C.c_int32), ("h1_shift", C.c_int32),
-                ("h2_div", C.c_int32), ("xcd_map", C.c_int32)]
+                ("h2_div", C.c_int32), ("xcd_map", C.c_int32), ("hash1", C.c_uint32),
+                ("hash2", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -361,6 +361,9 @@ SCHEDULES = [
     {"h1_shift": 2, "h2_div": 1, "u1": 2, "u2": 4},
     {"xcd_map": 0},                                     # rows interleaved over all workgroups
     {"xcd_map": 0, "subwave_h": 0, "u2": 3},
+    {"hash1": 0xD0B79761, "hash2": 0xCB06E8FB},        # other table hash multipliers
+    {"hash1": 1, "hash2": 1},                           # every key of a row on one home slot
+    {"hash1": 0xFFFFFFFF, "hash2": 0x00010001, "h1_shift": 1},
 ]
 
 
@@ -552,6 +555,7 @@ def test_autotune_installs_a_valid_schedule():
     print(f"autotune: {best} {ms_best:.4f} ms (start {ms_default:.4f} ms)")
     assert ms_best <= ms_default
     assert L.get_tuning() == best
+    assert best["hash1"] % 2 == 1 and best["hash2"] % 2 == 1
     out, _ = L.rap(*d, path=L.DIRECT)
     got = out.download()
     out.free()
@@ -561,6 +565,8 @@ def test_autotune_installs_a_valid_schedule():
     assert L.get_tuning() == default
     with pytest.raises(RuntimeError):
         L.set_tuning({"u1": 3})
+    with pytest.raises(RuntimeError):
+        L.set_tuning({"hash1": 2})  # multipliers are odd
     for m in d:
         m.free()
 

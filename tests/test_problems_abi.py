@@ -61,6 +61,9 @@ def test_struct_layouts_match_reference():
     # CONFIG: ushort, ushort, enum, uint, void*
     assert ctypes.sizeof(L.Config) == 24
     assert L.Config.chunkDistrbFunc.offset == 16
+    # sp3mm_tuning (this library's own): eight int32 fields, two uint32 hash multipliers
+    assert ctypes.sizeof(L.Tuning) == 40
+    assert L.Tuning.hash1.offset == 32 and L.Tuning.hash2.offset == 36
 
 
 @pytest.mark.skipif(not os.path.exists(LIB), reason="library not built")
