@@ -2122,16 +2122,18 @@ static bool likely_tables(Context *c, const Tuning &t, const sp3mm_dmat &R, cons
     return H1 <= kScoreTable && H2 <= kScoreTable;
 }
 
-// For tables of H1 and H2 slots: the candidates with the smallest summed
+// For tables of H1 and H2 slots: the candidates ranked by the summed
 // home-slot distance of the keys of 512 sample rows of R (evenly spaced), at
-// stage 1 (h1) and stage 2 (h2); kHashMul wins ties.
+// stage 1 (rank1) and stage 2 (rank2); equal scores keep the list order, so
+// kHashMul leads a tie.
 static int score_hashes(Context *c, const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int H1, int H2,
-                        unsigned &h1, unsigned &h2)
+                        std::vector<unsigned> &rank1, std::vector<unsigned> &rank2)
 {
     const std::vector<unsigned> &cand = hash_candidates();
     const int nc = (int)cand.size();
     const int nrows = (int)std::min<int64_t>(R.M, 512);
-    h1 = h2 = kHashMul;
+    rank1.assign(1, kHashMul);
+    rank2.assign(1, kHashMul);
     if (nrows <= 0 || !A.rp32 || !P.rp32 || H1 > kScoreTable || H2 > kScoreTable) return 0;
     std::vector<int32_t> rows((size_t)nrows);
     for (int i = 0; i < nrows; i++) rows[(size_t)i] = (int32_t)(((2 * (int64_t)i + 1) * R.M) / (2 * (int64_t)nrows));
@@ -2150,13 +2152,15 @@ static int score_hashes(Context *c, const sp3mm_dmat &R, const sp3mm_dmat &A, co
     std::vector<unsigned long long> h((size_t)nc * 2);
     SP3MM_HIP(hipMemcpyAsync(h.data(), sc, (size_t)nc * 2 * 8, hipMemcpyDeviceToHost, s));
     SP3MM_HIP(hipStreamSynchronize(s));
-    int b1 = 0, b2 = 0;
-    for (int i = 1; i < nc; i++) {
-        if (h[(size_t)i] < h[(size_t)b1]) b1 = i;
-        if (h[(size_t)(nc + i)] < h[(size_t)(nc + b2)]) b2 = i;
+    for (int st = 0; st < 2; st++) {
+        std::vector<int> idx((size_t)nc);
+        for (int i = 0; i < nc; i++) idx[(size_t)i] = i;
+        const unsigned long long *v = h.data() + (size_t)st * nc;
+        std::stable_sort(idx.begin(), idx.end(), [v](int x, int y) { return v[x] < v[y]; });
+        std::vector<unsigned> &out = st ? rank2 : rank1;
+        out.clear();
+        for (int i : idx) out.push_back(cand[(size_t)i]);
     }
-    h1 = cand[(size_t)b1];
-    h2 = cand[(size_t)b2];
     return 0;
 }
 
@@ -2168,7 +2172,7 @@ static int score_hashes(Context *c, const sp3mm_dmat &R, const sp3mm_dmat &A, co
 // after one warm-up (device time by HIP events, median).  Every candidate
 // that looks faster at all is timed again interleaved with the current
 // schedule, and taken only when that shows it at least 1% faster.  Each pass
-// first tries the hash multipliers scored on the operands (score_hashes).
+// first tries the hash multipliers ranked on the operands (score_hashes).
 // --------------------------------------------------------------------------
 int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int prec, double guard, int reps,
              Tuning &best, double &ms_best, double &ms_default)
@@ -2250,22 +2254,36 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
         }
         return 0;
     };
-    // Hash multipliers scored for a schedule's likely-bin tables
-    // (score_hashes, cached per pair of table sizes).  Once the scored
-    // multipliers are taken (`hashed`), every candidate schedule gets the ones
-    // scored for its own tables.  The float-pair kernel keeps the default.
-    std::map<std::pair<int, int>, std::pair<unsigned, unsigned>> hcache;
-    auto rehash = [&](Tuning &t) -> int {
+    // Hash multipliers ranked for a schedule's likely-bin tables
+    // (score_hashes, cached per pair of table sizes).  Once the best-ranked
+    // multipliers are taken (`hashed`), every candidate schedule gets the
+    // best-ranked for its own tables, and each pass times the next kHashTries
+    // of each stage's ranking against the current ones: multipliers that leave
+    // every key on its home slot can still differ in LDS bank conflicts.  The
+    // float-pair kernel keeps the default.
+    constexpr int kHashTries = 3;
+    std::map<std::pair<int, int>, std::pair<std::vector<unsigned>, std::vector<unsigned>>> hcache;
+    auto ranked = [&](const Tuning &t, const std::vector<unsigned> *&r1, const std::vector<unsigned> *&r2) -> int {
+        r1 = r2 = nullptr;
         int H1 = 0, H2 = 0;
         if (prec == SP3MM_PREC_MIXED || !likely_tables(c, t, R, A, H1, H2)) return 0;
         auto it = hcache.find({H1, H2});
         if (it == hcache.end()) {
-            unsigned h1 = kHashMul, h2 = kHashMul;
-            if (score_hashes(c, R, A, P, H1, H2, h1, h2)) return -1;
-            it = hcache.emplace(std::make_pair(H1, H2), std::make_pair(h1, h2)).first;
+            std::vector<unsigned> a, b;
+            if (score_hashes(c, R, A, P, H1, H2, a, b)) return -1;
+            it = hcache.emplace(std::make_pair(H1, H2), std::make_pair(std::move(a), std::move(b))).first;
         }
-        t.hash1 = it->second.first;
-        t.hash2 = it->second.second;
+        r1 = &it->second.first;
+        r2 = &it->second.second;
+        return 0;
+    };
+    auto rehash = [&](Tuning &t) -> int {
+        const std::vector<unsigned> *r1, *r2;
+        if (ranked(t, r1, r2)) return -1;
+        if (r1) {
+            t.hash1 = (*r1)[0];
+            t.hash2 = (*r2)[0];
+        }
         return 0;
     };
     bool hashed = false;
@@ -2285,6 +2303,26 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
                     return -1;
                 }
                 hashed = changed = took;
+            }
+        }
+        if (hashed) {
+            const std::vector<unsigned> *r1, *r2;
+            if (ranked(cur, r1, r2)) {
+                c->tune = saved;
+                return -1;
+            }
+            for (int st = 0; r1 && st < 2; st++) {
+                const std::vector<unsigned> &r = st ? *r2 : *r1;
+                for (int k = 0; k <= kHashTries && k < (int)r.size(); k++) {
+                    Tuning t = cur;
+                    (st ? t.hash2 : t.hash1) = r[(size_t)k];
+                    if (t.hash1 == cur.hash1 && t.hash2 == cur.hash2) continue;
+                    if (step(t, took)) {
+                        c->tune = saved;
+                        return -1;
+                    }
+                    changed = changed || took;
+                }
             }
         }
         for (const Axis &ax : axes) {
