@@ -2134,7 +2134,9 @@ static int score_hashes(Context *c, const sp3mm_dmat &R, const sp3mm_dmat &A, co
     const int nrows = (int)std::min<int64_t>(R.M, 512);
     rank1.assign(1, kHashMul);
     rank2.assign(1, kHashMul);
-    if (nrows <= 0 || !A.rp32 || !P.rp32 || H1 > kScoreTable || H2 > kScoreTable) return 0;
+    if (nrows <= 0 || !A.rp32 || !P.rp32 || H1 > kScoreTable || H2 > kScoreTable ||
+        hash_score_lds(H1, H2) > c->lds_device)
+        return 0;
     std::vector<int32_t> rows((size_t)nrows);
     for (int i = 0; i < nrows; i++) rows[(size_t)i] = (int32_t)(((2 * (int64_t)i + 1) * R.M) / (2 * (int64_t)nrows));
     Buf drows((size_t)nrows * 4), dcand((size_t)nc * 4), dscore((size_t)nc * 2 * 8);
@@ -2164,6 +2166,13 @@ static int score_hashes(Context *c, const sp3mm_dmat &R, const sp3mm_dmat &A, co
     return 0;
 }
 
+static bool same_tuning(const Tuning &x, const Tuning &y)
+{
+    return x.u1 == y.u1 && x.u2 == y.u2 && x.subwave_h == y.subwave_h && x.slog1_cap == y.slog1_cap &&
+           x.slog2_cap == y.slog2_cap && x.h1_shift == y.h1_shift && x.h2_div == y.h2_div && x.xcd_map == y.xcd_map &&
+           x.hash1 == y.hash1 && x.hash2 == y.hash2;
+}
+
 // --------------------------------------------------------------------------
 // autotuner: times the direct product of the caller's operands under candidate
 // schedules and installs the fastest in the context.  Coordinate descent from
@@ -2173,6 +2182,8 @@ static int score_hashes(Context *c, const sp3mm_dmat &R, const sp3mm_dmat &A, co
 // that looks faster at all is timed again interleaved with the current
 // schedule, and taken only when that shows it at least 1% faster.  Each pass
 // first tries the hash multipliers ranked on the operands (score_hashes).
+// The descent runs from the installed schedule and again from a preset for
+// mesh operators; the faster end point wins.
 // --------------------------------------------------------------------------
 int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int prec, double guard, int reps,
              Tuning &best, double &ms_best, double &ms_default)
@@ -2182,6 +2193,7 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     reps = std::min(std::max(reps, 1), 50);
     const Tuning saved = c->tune;
+    bool reassociated = false;  // the product runs as R*(A*P): the direct kernels' fields do not apply
     auto measure = [&](const Tuning &t, double &ms) -> int {
         c->tune = t;
         std::vector<double> v;
@@ -2191,6 +2203,7 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
             const int rc = rap_direct_fused(R, A, P, C, &st, prec, guard > 0 ? guard : 1e-2, nullptr, false);
             dmat_release(C);
             if (rc) return -1;
+            reassociated = st.reassociated != 0;
             if (k > 0) v.push_back(st.ms_total);
         }
         std::sort(v.begin(), v.end());
@@ -2287,62 +2300,101 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
         return 0;
     };
     bool hashed = false;
-    for (int pass = 0; pass < 2; pass++) {
-        bool changed = false, took = false;
-        if (!hashed) {
-            Tuning t = cur;
-            if (rehash(t)) {
-                c->tune = saved;
-                return -1;
+    // coordinate descent from `cur` (measured: cur_ms)
+    auto descend = [&]() -> int {
+        hashed = false;
+        for (int pass = 0; pass < 2; pass++) {
+            bool changed = false, took = false;
+            if (reassociated) hashed = true;  // no direct kernel: no tables to hash
+            if (!hashed) {
+                Tuning t = cur;
+                if (rehash(t)) return -1;
+                if (t.hash1 == cur.hash1 && t.hash2 == cur.hash2) {
+                    hashed = true;
+                } else {
+                    if (step(t, took)) {
+                        return -1;
+                    }
+                    hashed = changed = took;
+                }
             }
-            if (t.hash1 == cur.hash1 && t.hash2 == cur.hash2) {
-                hashed = true;
-            } else {
-                if (step(t, took)) {
-                    c->tune = saved;
+            if (hashed && !reassociated) {
+                const std::vector<unsigned> *r1, *r2;
+                if (ranked(cur, r1, r2)) {
                     return -1;
                 }
-                hashed = changed = took;
+                for (int st = 0; r1 && st < 2; st++) {
+                    const std::vector<unsigned> &r = st ? *r2 : *r1;
+                    for (int k = 0; k <= kHashTries && k < (int)r.size(); k++) {
+                        Tuning t = cur;
+                        (st ? t.hash2 : t.hash1) = r[(size_t)k];
+                        if (t.hash1 == cur.hash1 && t.hash2 == cur.hash2) continue;
+                        if (step(t, took)) {
+                            return -1;
+                        }
+                        changed = changed || took;
+                    }
+                }
             }
-        }
-        if (hashed) {
-            const std::vector<unsigned> *r1, *r2;
-            if (ranked(cur, r1, r2)) {
-                c->tune = saved;
-                return -1;
-            }
-            for (int st = 0; r1 && st < 2; st++) {
-                const std::vector<unsigned> &r = st ? *r2 : *r1;
-                for (int k = 0; k <= kHashTries && k < (int)r.size(); k++) {
+            for (const Axis &ax : axes) {
+                // a reassociated product only walks rows (xcd_map)
+                if (reassociated && ax.field != &Tuning::xcd_map) continue;
+                for (int v : ax.values) {
+                    if (cur.*ax.field == v) continue;
                     Tuning t = cur;
-                    (st ? t.hash2 : t.hash1) = r[(size_t)k];
-                    if (t.hash1 == cur.hash1 && t.hash2 == cur.hash2) continue;
+                    t.*ax.field = v;
+                    if (hashed && !reassociated && rehash(t)) return -1;
+                    if (!tuning_valid(t)) continue;
                     if (step(t, took)) {
-                        c->tune = saved;
                         return -1;
                     }
                     changed = changed || took;
                 }
             }
+            if (!changed) break;
         }
-        for (const Axis &ax : axes) {
-            for (int v : ax.values) {
-                if (cur.*ax.field == v) continue;
-                Tuning t = cur;
-                t.*ax.field = v;
-                if (hashed && rehash(t)) {
-                    c->tune = saved;
-                    return -1;
-                }
-                if (!tuning_valid(t)) continue;
-                if (step(t, took)) {
-                    c->tune = saved;
-                    return -1;
-                }
-                changed = changed || took;
+        return 0;
+    };
+    if (descend()) {
+        c->tune = saved;
+        return -1;
+    }
+    // A second descent from a schedule that suits mesh operators (three-lane
+    // stage-2 batches, four-lane sub-groups, the stage-1 table at half the
+    // bound): on the 128^3 7-point product one descent from the defaults
+    // settled once at 0.63 ms where others reach 0.55 ms (u2 = 3 and
+    // slog2_cap = 2 pay only together).  The better end point is kept after
+    // an interleaved re-timing.
+    Tuning mesh{};
+    mesh.u1 = 4;
+    mesh.u2 = 3;
+    mesh.slog2_cap = 2;
+    mesh.h1_shift = 1;
+    mesh.h2_div = 4;
+    const Tuning end1 = cur;
+    const double end1_ms = cur_ms;
+    if (!reassociated && !same_tuning(mesh, saved)) {
+        cur = mesh;
+        if (measure(cur, cur_ms) || descend()) {
+            c->tune = saved;
+            return -1;
+        }
+        if (!same_tuning(cur, end1)) {
+            double ms2 = 0, ms1 = 0;
+            if (confirm(cur, end1, ms2, ms1)) {
+                c->tune = saved;
+                return -1;
+            }
+            if (ms1 <= ms2) {
+                cur = end1;
+                cur_ms = ms1;
+            } else {
+                cur_ms = ms2;
             }
         }
-        if (!changed) break;
+    } else {
+        cur = end1;
+        cur_ms = end1_ms;
     }
     c->tune = cur;
     best = cur;

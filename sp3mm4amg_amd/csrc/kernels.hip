@@ -2486,6 +2486,12 @@ void launch_direct_fused(int G, int H1, int H2, int slog1, int slog2, const SpOp
                        slotn, scol, sval, retry, nretry, err, nxcd, rows_host, tu.hash1 | 1u, tu.hash2 | 1u);
 }
 
+size_t hash_score_lds(int H1, int H2)
+{
+    // dynamic: one pair of tables per wavefront; static: the two dedup tables and reductions
+    return (size_t)Group<256>::kWaves * (H1 + H2) * sizeof(int) + 2 * kScoreDedup * sizeof(int) + 64;
+}
+
 void launch_hash_score(const SpOperand &R, const SpOperand &A, const SpOperand &P, const int32_t *rows, int nrows,
                        const unsigned *cand, int ncand, int H1, int H2, unsigned long long *score1,
                        unsigned long long *score2, hipStream_t st)
