@@ -426,7 +426,7 @@ typedef struct {
     int32_t slog1_cap;   /* largest lane sub-group (log2) per left entry in stage 1, 0..8
                             (the sub-group never exceeds the row's group) */
     int32_t slog2_cap;   /* the same for stage 2 */
-    int32_t h1_shift;    /* stage-1 table = bin upper bound >> h1_shift, -1..2 (-1: twice
+    int32_t h1_shift;    /* stage-1 table = bin upper bound >> h1_shift, -1..3 (-1: twice
                             the bound; a row whose table fills is redone by the
                             long-row kernels) */
     int32_t h2_div;      /* stage-2 table = max(64, stage-1 table / h2_div): 1, 2, 4, 8 */

@@ -110,7 +110,7 @@ int tuning_valid(const Tuning &t)
     const bool pow2h = t.subwave_h == 0 || (t.subwave_h >= 64 && t.subwave_h <= 4096 &&
                                             (t.subwave_h & (t.subwave_h - 1)) == 0);
     return (t.u1 == 2 || t.u1 == 4) && (t.u2 >= 2 && t.u2 <= 4) && pow2h && t.slog1_cap >= 0 &&
-           t.slog1_cap <= 8 && t.slog2_cap >= 0 && t.slog2_cap <= 8 && t.h1_shift >= -1 && t.h1_shift <= 2 &&
+           t.slog1_cap <= 8 && t.slog2_cap >= 0 && t.slog2_cap <= 8 && t.h1_shift >= -1 && t.h1_shift <= 3 &&
            (t.h2_div == 1 || t.h2_div == 2 || t.h2_div == 4 || t.h2_div == 8) && (t.xcd_map == 0 || t.xcd_map == 1) &&
            (t.hash1 & 1u) && (t.hash2 & 1u);
 }
@@ -2194,7 +2194,10 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
     reps = std::min(std::max(reps, 1), 50);
     const Tuning saved = c->tune;
     bool reassociated = false;  // the product runs as R*(A*P): the direct kernels' fields do not apply
-    auto measure = [&](const Tuning &t, double &ms) -> int {
+    // cutoff > 0: a timed product slower than cutoff ends the candidate at
+    // once with that time (a schedule whose tables fill and send every row
+    // to the long-row kernels costs tens of products' time per product)
+    auto measure = [&](const Tuning &t, double &ms, double cutoff = 0.0) -> int {
         c->tune = t;
         std::vector<double> v;
         for (int k = 0; k <= reps; k++) {
@@ -2205,6 +2208,11 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
             if (rc) return -1;
             reassociated = st.reassociated != 0;
             if (k > 0) v.push_back(st.ms_total);
+            // the warm-up product may allocate: it is cut at four times the bound
+            if (cutoff > 0.0 && st.ms_total > (k > 0 ? cutoff : 4.0 * cutoff)) {
+                ms = st.ms_total;
+                return 0;
+            }
         }
         std::sort(v.begin(), v.end());
         ms = v[v.size() / 2];
@@ -2246,7 +2254,7 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
     const Axis axes[] = {
         {&Tuning::subwave_h, {0, 128, 256, 512}}, {&Tuning::u1, {2, 4}},       {&Tuning::u2, {2, 3, 4}},
         {&Tuning::slog1_cap, {1, 2, 3}},            {&Tuning::slog2_cap, {1, 2}},
-        {&Tuning::h1_shift, {-1, 0, 1}},            {&Tuning::h2_div, {2, 4, 8}},
+        {&Tuning::h1_shift, {-1, 0, 1, 2, 3}},      {&Tuning::h2_div, {2, 4, 8}},
         {&Tuning::xcd_map, {1, 0}},
     };
     // a candidate is taken when its median looks faster and the interleaved
@@ -2255,7 +2263,7 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
     auto step = [&](const Tuning &t, bool &took) -> int {
         took = false;
         double ms = 0;
-        if (measure(t, ms)) return -1;
+        if (measure(t, ms, 2.0 * cur_ms)) return -1;
         if (ms < cur_ms) {
             double ms_t = 0, ms_cur = 0;
             if (confirm(t, cur, ms_t, ms_cur)) return -1;

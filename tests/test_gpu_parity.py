@@ -364,6 +364,7 @@ SCHEDULES = [
     {"hash1": 0xD0B79761, "hash2": 0xCB06E8FB},        # other table hash multipliers
     {"hash1": 1, "hash2": 1},                           # every key of a row on one home slot
     {"hash1": 0xFFFFFFFF, "hash2": 0x00010001, "h1_shift": 1},
+    {"h1_shift": 3, "h2_div": 8, "u1": 2, "u2": 3},     # tables at an eighth of the bound: many rows retried
 ]
 
 
