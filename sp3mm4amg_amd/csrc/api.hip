@@ -275,7 +275,7 @@ void sp3mm_dmat_free(sp3mm_dmat *m)
 {
     if (!m) return;
     Context *c = ctx();
-    if (c) (void)hipStreamSynchronize(c->stream);
+    if (c) (void)spin_stream(c->stream);
     dmat_release(*m);
 }
 

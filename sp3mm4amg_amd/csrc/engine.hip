@@ -5,6 +5,7 @@
 // stream.  The host reads small counters back twice per product (bin
 // populations, then output size) to size grids and the output arrays.
 #include <algorithm>
+#include <chrono>
 #include <cstdarg>
 #include <cstdlib>
 #include <cstdio>
@@ -496,8 +497,30 @@ struct Timer {
 
 constexpr size_t kPollFlagOff = 4088;
 
-// Reads the control block back with a stream synchronisation.
-static int read_ctl(Context *c, const Ctl *d, Ctl &h);
+template <typename Query, typename Block>
+static hipError_t spin_wait(Query q, Block b)
+{
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned k = 0;; k++) {
+        const hipError_t e = q();
+        if (e != hipErrorNotReady) return e;
+        if ((k & 63) == 63 && std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(2)) return b();
+        __builtin_ia32_pause();
+    }
+}
+hipError_t spin_stream(hipStream_t s)
+{
+    return spin_wait([s] { return hipStreamQuery(s); }, [s] { return hipStreamSynchronize(s); });
+}
+hipError_t spin_event(hipEvent_t e)
+{
+    return spin_wait([e] { return hipEventQuery(e); }, [e] { return hipEventSynchronize(e); });
+}
+
+// Reads the control block back with a stream synchronisation.  With a
+// timer, the event *mark is recorded behind the export, before the
+// synchronisation (a product's end event then costs no second round trip).
+static int read_ctl(Context *c, const Ctl *d, Ctl &h, Timer *tm = nullptr, int *mark = nullptr);
 
 // Waits for the export `seq` of the control block into the coherent host
 // block (the binning's scatter kernel writes it and then raises the flag at
@@ -530,7 +553,7 @@ static int wait_ctl(Context *c, const Ctl *d, unsigned long long seq, Ctl &h)
     return 0;
 }
 
-static int read_ctl(Context *c, const Ctl *d, Ctl &h)
+static int read_ctl(Context *c, const Ctl *d, Ctl &h, Timer *tm, int *mark)
 {
     static_assert(sizeof(Ctl) % 4 == 0 && sizeof(Ctl) <= 4096, "control block fits the host mirror");
     SP3MM_HIP(hipGetLastError());  // a kernel launch that was rejected
@@ -540,7 +563,8 @@ static int read_ctl(Context *c, const Ctl *d, Ctl &h)
     } else {
         SP3MM_HIP(hipMemcpyAsync(c->host_small, d, sizeof(Ctl), hipMemcpyDeviceToHost, c->stream));
     }
-    SP3MM_HIP(hipStreamSynchronize(c->stream));
+    if (tm && mark) *mark = tm->mark();
+    SP3MM_HIP(spin_stream(c->stream));
     memcpy(&h, c->host_small, sizeof(Ctl));
     return 0;
 }
@@ -1161,6 +1185,7 @@ struct FusedPlan {
     }
     int64_t nnz = 0, stot = 0;
     long long mac1 = 0;
+    int e_end = -1;  // event behind the final read-back of build() (timer of build())
     int slog1 = 0, slog2 = 0, dlog1 = 0, dlog2 = 0;
     Buf ctl, ub, rown, lists, retry, slotn, fix, tiles, scol, sval, binscr, spill;
     // long rows in one pass (k_window_direct slot mode): a scratch slot of
@@ -1479,7 +1504,7 @@ struct FusedPlan {
             launch_tail(c, s);
             spec = false;
             defer_retry = false;
-            if (read_ctl(c, d(), h)) return -1;
+            if (read_ctl(c, d(), h, &tm, &e_end)) return -1;
             if (!h.err && h.nspill == 0 && h.nretry == 0 && h.nnz <= ccap) {
                 nretry = h.nretry;
                 nspill = 0;
@@ -1532,7 +1557,7 @@ struct FusedPlan {
         }
         launch_tail(c, s);
         launch_fixups(c, s, false);
-        if (read_ctl(c, d(), h)) return -1;
+        if (read_ctl(c, d(), h, &tm, &e_end)) return -1;
         if (h.err) {
             set_error("direct product: %s", err_text(h.err));
             return -1;
@@ -1633,8 +1658,8 @@ static int rap_direct_fused(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3m
         }
         return 0;
     }
-    const int e1 = tm.mark();
-    (void)hipEventSynchronize(tm.ev[e1]);
+    const int e1 = plan.e_end >= 0 ? plan.e_end : tm.mark();
+    (void)spin_event(tm.ev[e1]);
     C = plan.C;
     memset(&plan.C, 0, sizeof plan.C);
     if (st) {

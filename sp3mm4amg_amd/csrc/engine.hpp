@@ -86,6 +86,11 @@ ConsState *cons_state(Context *c);
 // Waits for the library stream and reports (and clears) an error that a
 // stream-ordered consumer call left in the device error word.
 int consumer_wait(Context *c, double *dot_out = nullptr);
+// Waits for a stream (an event) by polling for up to ~2 ms, then blocks.  A
+// blocking wait returns some 20 us after the GPU is done; a product's final
+// wait is on its critical path.
+hipError_t spin_stream(hipStream_t s);
+hipError_t spin_event(hipEvent_t e);
 
 // Queues `sweeps` damped Jacobi sweeps (x in place, work as the other buffer).
 void queue_jacobi(const sp3mm_dmat &A, const double *b, double *x, double *work, double omega, int sweeps, int *err,
