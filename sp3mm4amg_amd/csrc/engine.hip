@@ -1379,18 +1379,14 @@ struct FusedPlan {
         hipStream_t s = c->stream;
         const int64_t M = R.M;
         fused_bins(c->max_lds, mixed, tu, thr, cfg);
+        // the binning's buffers first: its kernels start while the rest of
+        // the scratch is allocated
         ctl = Buf(sizeof(Ctl));
         ub = Buf((size_t)(M + 1) * 8);
         rown = Buf((size_t)(M + 1) * 4);
         lists = Buf((size_t)(thr.nb + 1) * (M ? M : 1) * 4);
-        retry = Buf((size_t)(M ? M : 1) * 4);
-        slotn = Buf((size_t)(M ? M : 1) * 4);
-        fix = Buf((size_t)(M ? M : 1) * 4);
-        tiles = Buf((size_t)scan_tiles(M) * 8);
         binscr = Buf((size_t)bin_scratch_ints(M) * 4);
-        spill = Buf((size_t)(M ? M : 1) * 4);
-        if (!ctl.p || !ub.p || !rown.p || !lists.p || !retry.p || !slotn.p || !fix.p || !tiles.p || !binscr.p ||
-            !spill.p) {
+        if (!ctl.p || !ub.p || !rown.p || !lists.p || !binscr.p) {
             set_error("scratch allocation failed");
             return -1;
         }
@@ -1400,10 +1396,30 @@ struct FusedPlan {
         dlog1 = sub_log(A, kDenseThreads);
         dlog2 = sub_log(P, kDenseThreads);
         // the likely bin starts before the populations are read back, so the
-        // GPU does not idle through the read-back; its slots are allocated
-        // first, so that nothing but launches sits between the binning
-        // kernels and its launch
+        // GPU does not idle through the read-back
         pred = predict_bin(c);
+        if (pred >= 0) {
+            mlen = Buf((size_t)kMaxLenBlocks * 2 * 4);
+            if (!mlen.p) (void)hipGetLastError();  // binning measures every row then
+        }
+        const int e0 = mk(tm);
+        // the binning clears the control block, and its scatter kernel sends
+        // the populations to the host while the likely bin already runs
+        SP3MM_HIP(hipGetLastError());
+        const unsigned long long seq = c->poll_host ? ++c->poll_seq : 0;
+        launch_binning(c, s, seq);
+        const int eu = mk(tm);
+        retry = Buf((size_t)(M ? M : 1) * 4);
+        slotn = Buf((size_t)(M ? M : 1) * 4);
+        fix = Buf((size_t)(M ? M : 1) * 4);
+        tiles = Buf((size_t)scan_tiles(M) * 8);
+        spill = Buf((size_t)(M ? M : 1) * 4);
+        if (!retry.p || !slotn.p || !fix.p || !tiles.p || !spill.p) {
+            set_error("scratch allocation failed");
+            return -1;
+        }
+        // the likely bin's slots; without them it runs with the other bins
+        // after the read-back (its rows are listed like any bin's)
         if (pred >= 0) {
             const size_t cap = (size_t)M * cfg[pred].H2;
             pcol = Buf(cap * 4);
@@ -1417,17 +1433,6 @@ struct FusedPlan {
                 pred = -1;
             }
         }
-        if (pred >= 0) {
-            mlen = Buf((size_t)kMaxLenBlocks * 2 * 4);
-            if (!mlen.p) (void)hipGetLastError();  // binning measures every row then
-        }
-        const int e0 = mk(tm);
-        // the binning clears the control block, and its scatter kernel sends
-        // the populations to the host while the likely bin already runs
-        SP3MM_HIP(hipGetLastError());
-        const unsigned long long seq = c->poll_host ? ++c->poll_seq : 0;
-        launch_binning(c, s, seq);
-        const int eu = mk(tm);
         int en0 = -1;
         if (pred >= 0) {
             en0 = mk(tm);
