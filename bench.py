@@ -122,11 +122,12 @@ def time_form(lib, dR, dA, dP, steps, warmup, **kw):
     total_ms = []
     t0 = time.perf_counter()
     for _ in range(steps):
-        out, st = lib.rap(dR, dA, dP, **kw)
-        total_ms.append(st["ms_total"])
+        out, st = lib.rap(dR, dA, dP, raw_stats=True, **kw)
+        total_ms.append(st.ms_total)
         out.free()
     lib.lib().sp3mm_sync()
     wall = (time.perf_counter() - t0) * 1e3 / steps
+    st = st.as_dict()
     main_ms = main_kernel_ms(lib, dR, dA, dP, steps, **kw)
     return wall, float(np.mean(main_ms)), float(np.median(total_ms)), st, total_ms
 

@@ -1243,16 +1243,25 @@ struct FusedPlan {
                                 retry.as<int32_t>(), &d()->nretry, &d()->err, max_blocks(c), s, f32, tu);
     }
 
-    void launch_tail(Context *c, hipStream_t s)
+    // export_ctl: the first gather also copies the control block to the host
+    // mirror as it starts (every count in it is final by then, and a gather
+    // can only add an overflow bit, which nnz <= ccap rules out); returns
+    // whether it does
+    bool launch_tail(Context *c, hipStream_t s, bool export_ctl = false)
     {
         const int64_t M = R.M;
+        bool exported = false;
         for (int i = 0; i < thr.nb; i++) {
             if (cnt[i] == 0) continue;
+            const bool x = export_ctl && !exported && c->host_small_dev;
             launch_gather_rows(lists.as<int32_t>() + (int64_t)i * M, d()->counts + i, cnt[i], slot_n(i), cfg[i].H2,
                                slot_col(i), slot_val(i), C.rp, C.col, static_cast<double *>(C.val), ccap,
-                               &d()->err, max_blocks(c), s);
+                               &d()->err, max_blocks(c), s, x ? d() : nullptr, x ? c->host_small_dev : nullptr,
+                               (int)(sizeof(Ctl) / 4));
+            exported = exported || x;
         }
         launch_long(c, s, true);
+        return exported;
     }
 
     // rows beyond the hash tables (long rows of the last bin, and rows whose
@@ -1506,10 +1515,18 @@ struct FusedPlan {
             else if (alloc_out(c, stot))
                 return -1;
             spec = true;
-            launch_tail(c, s);
+            const bool early = launch_tail(c, s, true);
             spec = false;
             defer_retry = false;
-            if (read_ctl(c, d(), h, &tm, &e_end)) return -1;
+            if (early) {
+                // the control block is in the host mirror once the stream ends
+                SP3MM_HIP(hipGetLastError());
+                e_end = tm.mark();
+                SP3MM_HIP(spin_stream(s));
+                memcpy(&h, c->host_small, sizeof(Ctl));
+            } else if (read_ctl(c, d(), h, &tm, &e_end)) {
+                return -1;
+            }
             if (!h.err && h.nspill == 0 && h.nretry == 0 && h.nnz <= ccap) {
                 nretry = h.nretry;
                 nspill = 0;

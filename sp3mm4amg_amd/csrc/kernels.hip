@@ -1087,8 +1087,13 @@ __global__ void __launch_bounds__(256) k_gather_rows(const int32_t *__restrict__
                                                       const double *__restrict__ sval,
                                                       const int64_t *__restrict__ crp, int32_t *__restrict__ ccol,
                                                       double *__restrict__ cval, int64_t out_cap,
-                                                      int *__restrict__ err, int cap_rows)
+                                                      int *__restrict__ err, int cap_rows,
+                                                      const uint32_t *__restrict__ xsrc, uint32_t *xdst, int xwords)
 {
+    // xdst (optional): workgroup 0 first copies xwords words from xsrc (a
+    // control block the kernels before this one completed) to host memory
+    if (xdst && blockIdx.x == 0)
+        for (int i = threadIdx.x; i < xwords; i += blockDim.x) xdst[i] = xsrc[i];
     const int gpb = blockDim.x / L;
     const int hl = threadIdx.x & (L - 1);
     const int n = clamp_count(count, cap_rows, err);
@@ -2506,21 +2511,24 @@ void launch_hash_score(const SpOperand &R, const SpOperand &A, const SpOperand &
 
 void launch_gather_rows(const int32_t *rows, const int *count, int rows_host, const int32_t *slotn, int cap,
                         const int32_t *scol, const double *sval, const int64_t *crp, int32_t *ccol, double *cval,
-                        int64_t out_cap, int *err, int max_blocks, hipStream_t st)
+                        int64_t out_cap, int *err, int max_blocks, hipStream_t st, const void *xsrc, void *xdst,
+                        int xwords)
 {
     const int L = cap <= 128 ? 16 : cap <= 4096 ? 32 : 256;
     unsigned grid = (unsigned)((rows_host + 256 / L - 1) / (256 / L));
     if (grid > (unsigned)max_blocks) grid = (unsigned)max_blocks;
     if (grid < 1) grid = 1;
+    const auto *xs = static_cast<const uint32_t *>(xsrc);
+    auto *xd = static_cast<uint32_t *>(xdst);
     if (L == 256)
         hipLaunchKernelGGL(k_gather_rows<256>, dim3(grid), dim3(256), 0, st, rows, count, slotn, cap, scol, sval,
-                           crp, ccol, cval, out_cap, err, rows_host);
+                           crp, ccol, cval, out_cap, err, rows_host, xs, xd, xwords);
     else if (L == 16)
         hipLaunchKernelGGL(k_gather_rows<16>, dim3(grid), dim3(256), 0, st, rows, count, slotn, cap, scol, sval, crp,
-                           ccol, cval, out_cap, err, rows_host);
+                           ccol, cval, out_cap, err, rows_host, xs, xd, xwords);
     else
         hipLaunchKernelGGL(k_gather_rows<32>, dim3(grid), dim3(256), 0, st, rows, count, slotn, cap, scol, sval, crp,
-                           ccol, cval, out_cap, err, rows_host);
+                           ccol, cval, out_cap, err, rows_host, xs, xd, xwords);
 }
 
 // A replay computed the sizes its plan was built for: nnz, retried and

@@ -159,9 +159,12 @@ size_t spgemm_fused_lds(int G, int H);
 void launch_spgemm_fused(int G, int H, int slog, const SpOperand &A, const SpOperand &B, const int32_t *rows,
                          const int *count, int rows_host, int32_t *rownnz, int32_t *slotn, int32_t *scol,
                          double *sval, int *err, int max_blocks, hipStream_t st, int xcd_map);
+// xdst (optional): the kernel's workgroup 0 first copies xwords words from
+// xsrc to xdst (a control block into host memory, read after the stream ends)
 void launch_gather_rows(const int32_t *rows, const int *count, int rows_host, const int32_t *slotn, int cap,
                         const int32_t *scol, const double *sval, const int64_t *crp, int32_t *ccol, double *cval,
-                        int64_t out_cap, int *err, int max_blocks, hipStream_t st);
+                        int64_t out_cap, int *err, int max_blocks, hipStream_t st, const void *xsrc = nullptr,
+                        void *xdst = nullptr, int xwords = 0);
 // populations of the bins a plan launched (bins 0..n-1, n <= kMaxBins + 1)
 struct BinCounts {
     int n;

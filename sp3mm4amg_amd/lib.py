@@ -274,14 +274,15 @@ class DeviceCSR:
 
 
 def rap(R: DeviceCSR, A: DeviceCSR, P: DeviceCSR, path: int = DIRECT, precision: int = FP64,
-        guard: float = 0.0, timing: bool = False, rowsum_dev: int | None = None):
-    """R*A*P on device-resident operands; returns (DeviceCSR, stats dict)."""
+        guard: float = 0.0, timing: bool = False, rowsum_dev: int | None = None, raw_stats: bool = False):
+    """R*A*P on device-resident operands; returns (DeviceCSR, stats dict), or
+    the Stats structure itself with raw_stats (no dict built: timing loops)."""
     out = DMat()
     st = Stats()
     o = Opts(path, precision, guard, int(timing), 0, rowsum_dev)
     _check(lib().sp3mm_rap(C.byref(R.d), C.byref(A.d), C.byref(P.d), C.byref(out), C.byref(o),
                            C.byref(st)), "sp3mm_rap")
-    return DeviceCSR(out), st.as_dict()
+    return DeviceCSR(out), (st if raw_stats else st.as_dict())
 
 
 def rap_noalloc(R: DeviceCSR, A: DeviceCSR, P: DeviceCSR, rp: "DeviceArray", col: "DeviceArray",
