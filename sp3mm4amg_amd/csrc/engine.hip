@@ -723,7 +723,7 @@ static void spg_bins(size_t max_lds, BinThresholds &thr, std::vector<BinCfg> &cf
 // scan and a gather build the output.  Rows beyond the largest LDS table run
 // on the dense long-row kernels (count, then numeric).
 static int spgemm_onepass(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &C, sp3mm_stats *st, Timer *tm,
-                          double *ms_num)
+                          double *ms_num, unsigned hmul)
 {
     Context *c = ctx();
     if (!c) return -1;
@@ -767,7 +767,7 @@ static int spgemm_onepass(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &
         launch_spgemm_fused(cfg[i].G, cfg[i].H, sub_log(B, cfg[i].G), a, b, lists.as<int32_t>() + (int64_t)i * M,
                             d->counts + i, h.counts[i], rown.as<int32_t>(), slotn.as<int32_t>() + nbase[i],
                             scol.as<int32_t>() + soff[i], sval.as<double>() + soff[i], &d->err, max_blocks(c), s,
-                            c->tune.xcd_map);
+                            c->tune.xcd_map, hmul);
     }
     const int t1 = tm ? tm->mark() : -1;
     const int nlong = h.counts[thr.nb];
@@ -843,13 +843,14 @@ static int spgemm_onepass(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &
     return 0;
 }
 
-// the one-pass SpGEMM unless SP3MM_SPGEMM_TWOPASS is set (symbolic + numeric)
+// the one-pass SpGEMM unless SP3MM_SPGEMM_TWOPASS is set (symbolic + numeric);
+// hmul: the multiplier of the one-pass kernel's hash tables
 static int spgemm_any(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &C, sp3mm_stats *st, Timer *tm,
-                      double *ms_num)
+                      double *ms_num, unsigned hmul = kHashMul)
 {
     static const bool twopass = getenv("SP3MM_SPGEMM_TWOPASS") != nullptr;
     if (twopass) return spgemm_impl(A, B, C, st, tm, ms_num);
-    return spgemm_onepass(A, B, C, st, tm, ms_num);
+    return spgemm_onepass(A, B, C, st, tm, ms_num, hmul);
 }
 
 int spgemm(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &C, sp3mm_stats *st, bool symbolic_only)
@@ -893,11 +894,14 @@ int rap_pair(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, sp3m
     double ms1 = 0, ms2 = 0;
     sp3mm_dmat RA;
     memset(&RA, 0, sizeof RA);
-    if (spgemm_any(R, A, RA, &s1, &tm, &ms1)) {
+    // R*A's keys are A's columns, (R*A)*P's are P's: the installed
+    // schedule's multipliers for the direct kernel's two tables (scored for
+    // those key sets when the autotuner ran on these operands)
+    if (spgemm_any(R, A, RA, &s1, &tm, &ms1, c->tune.hash1)) {
         dmat_release(RA);
         return -1;
     }
-    const int rc = spgemm_any(RA, P, C, &s2, &tm, &ms2);
+    const int rc = spgemm_any(RA, P, C, &s2, &tm, &ms2, c->tune.hash2);
     const int64_t mid = RA.nnz;
     dmat_release(RA);
     if (rc) {

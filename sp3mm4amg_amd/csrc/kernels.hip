@@ -967,7 +967,8 @@ __global__ void __launch_bounds__(256) k_hash_score(SpOperand R, SpOperand A, Sp
 // SpGEMM C = A*B in one pass
 // ---------------------------------------------------------------------------
 // Row r of A*B is accumulated in an LDS table of H >= its upper bound slots
-// (branch-free expansion, so it cannot fill), compacted, rank-sorted and
+// (branch-free expansion, so it cannot fill; home slots from the multiplier
+// hmul), compacted, rank-sorted and
 // written to its scratch slot of H entries (slot i of the bin list -> scratch
 // + i*H); its length goes to rownnz[r] and slotn[i].  After the scan of
 // rownnz, k_gather_rows moves every slot to its final place.
@@ -976,7 +977,7 @@ __global__ void __launch_bounds__(256) k_spgemm_fused(SpOperand A, SpOperand B, 
                                                        const int *__restrict__ count, int H, int slog,
                                                        int32_t *__restrict__ rownnz, int32_t *__restrict__ slotn,
                                                        int32_t *__restrict__ scol, double *__restrict__ sval,
-                                                       int *__restrict__ err, int nxcd)
+                                                       int *__restrict__ err, int nxcd, unsigned hmul)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int GPB = 256 / G;
@@ -1014,7 +1015,7 @@ __global__ void __launch_bounds__(256) k_spgemm_fused(SpOperand A, SpOperand B, 
         Group<G>::sync();
         bool full = false;
         expand_flat<G, double, double, U>(A.col, aval, rb, re, B.rp32, B.col, bval, slog, k, v, m, sh,
-                                          (unsigned)(H + t), full, fullflag);
+                                          (unsigned)(H + t), full, fullflag, hmul);
         const int64_t rbn = A.rp[rn], ren = A.rp[rn + 1];
         if (full) atomicOr(err, kErrHashFull);  // H >= the row's upper bound: cannot happen
         Group<G>::sync();
@@ -1064,13 +1065,14 @@ static unsigned grid_for(int rows, int G, int max_blocks);
 
 void launch_spgemm_fused(int G, int H, int slog, const SpOperand &A, const SpOperand &B, const int32_t *rows,
                          const int *count, int rows_host, int32_t *rownnz, int32_t *slotn, int32_t *scol,
-                         double *sval, int *err, int max_blocks, hipStream_t st, int xcd_map)
+                         double *sval, int *err, int max_blocks, hipStream_t st, int xcd_map, unsigned hmul)
 {
     const unsigned grid = grid_for(rows_host, G, max_blocks);
     const size_t lds = spgemm_fused_lds(G, H);
     slog = std::min(slog, ilog2((unsigned)G));
     const int nxcd = xcd_map && grid >= (unsigned)kXcds ? kXcds : 1;
-#define SP3MM_SPG_ARGS A, B, rows, count, H, slog, rownnz, slotn, scol, sval, err, nxcd
+    hmul |= 1u;
+#define SP3MM_SPG_ARGS A, B, rows, count, H, slog, rownnz, slotn, scol, sval, err, nxcd, hmul
     if (G == 32) hipLaunchKernelGGL((k_spgemm_fused<32, 4>), dim3(grid), dim3(256), lds, st, SP3MM_SPG_ARGS);
     else if (G == 64) hipLaunchKernelGGL((k_spgemm_fused<64, 4>), dim3(grid), dim3(256), lds, st, SP3MM_SPG_ARGS);
     else hipLaunchKernelGGL((k_spgemm_fused<256, 4>), dim3(grid), dim3(256), lds, st, SP3MM_SPG_ARGS);

@@ -158,7 +158,8 @@ void launch_hash_score(const SpOperand &R, const SpOperand &A, const SpOperand &
 size_t spgemm_fused_lds(int G, int H);
 void launch_spgemm_fused(int G, int H, int slog, const SpOperand &A, const SpOperand &B, const int32_t *rows,
                          const int *count, int rows_host, int32_t *rownnz, int32_t *slotn, int32_t *scol,
-                         double *sval, int *err, int max_blocks, hipStream_t st, int xcd_map);
+                         double *sval, int *err, int max_blocks, hipStream_t st, int xcd_map,
+                         unsigned hmul = kHashMul);
 // xdst (optional): the kernel's workgroup 0 first copies xwords words from
 // xsrc to xdst (a control block into host memory, read after the stream ends)
 void launch_gather_rows(const int32_t *rows, const int *count, int rows_host, const int32_t *slotn, int cap,
