@@ -654,13 +654,13 @@ def main():
     if not args.no_cpu:
         log("timing the reference CPU path on samples ...")
         p1 = problems.poisson2d_sa(64)
-        work = [("config2", f"{prob['name']} (FP64)", [(R, A, P)], 18.0, 3),
-                ("config1", f"{p1['name']} (FP64, whole product, 40 timed runs)", [(p1["R"], p1["A"], p1["P"])],
+        work = [("poisson3d128", f"{prob['name']} (FP64)", [(R, A, P)], 18.0, 3),
+                ("poisson2d64", f"{p1['name']} (FP64, whole product, 40 timed runs)", [(p1["R"], p1["A"], p1["P"])],
                  0.0, 40)]
         if stencil27 is not None:
-            work.append(("config3", "stencil27_100^3_sa3x3x3_seed3 (FP64 values)", [stencil27], 8.0, 3))
+            work.append(("stencil27_100", "stencil27_100^3_sa3x3x3_seed3 (FP64 values)", [stencil27], 8.0, 3))
         if batch_items is not None:
-            work.append(("config5", "5-level 96^3 hierarchy + 256 2D problems (FP64, every product sampled)",
+            work.append(("batch261", "5-level 96^3 hierarchy + 256 2D problems (FP64, every product sampled)",
                          batch_items, 8.0, 3))
         rec["cpu_baseline"] = cpu_baseline(work)
     for m in (dR, dA, dP):
