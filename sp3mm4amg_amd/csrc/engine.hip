@@ -2178,7 +2178,8 @@ static bool likely_tables(Context *c, const Tuning &t, const sp3mm_dmat &R, cons
 // stage 1 (rank1) and stage 2 (rank2); equal scores keep the list order, so
 // kHashMul leads a tie.
 static int score_hashes(Context *c, const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int H1, int H2,
-                        std::vector<unsigned> &rank1, std::vector<unsigned> &rank2)
+                        std::vector<unsigned> &rank1, std::vector<unsigned> &rank2,
+                        std::vector<int> *nkeys = nullptr)
 {
     const std::vector<unsigned> &cand = hash_candidates();
     const int nc = (int)cand.size();
@@ -2190,8 +2191,8 @@ static int score_hashes(Context *c, const sp3mm_dmat &R, const sp3mm_dmat &A, co
         return 0;
     std::vector<int32_t> rows((size_t)nrows);
     for (int i = 0; i < nrows; i++) rows[(size_t)i] = (int32_t)(((2 * (int64_t)i + 1) * R.M) / (2 * (int64_t)nrows));
-    Buf drows((size_t)nrows * 4), dcand((size_t)nc * 4), dscore((size_t)nc * 2 * 8);
-    if (!drows.p || !dcand.p || !dscore.p) {
+    Buf drows((size_t)nrows * 4), dcand((size_t)nc * 4), dscore((size_t)nc * 2 * 8), dkeys((size_t)nrows * 2 * 4);
+    if (!drows.p || !dcand.p || !dscore.p || !dkeys.p) {
         set_error("hash scoring: allocation failed");
         return -1;
     }
@@ -2200,10 +2201,14 @@ static int score_hashes(Context *c, const sp3mm_dmat &R, const sp3mm_dmat &A, co
     SP3MM_HIP(hipMemcpyAsync(dcand.p, cand.data(), (size_t)nc * 4, hipMemcpyHostToDevice, s));
     auto *sc = dscore.as<unsigned long long>();
     launch_hash_score(operand(R), operand(A), operand(P), drows.as<int32_t>(), nrows, dcand.as<unsigned>(), nc, H1,
-                      H2, sc, sc + nc, s);
+                      H2, sc, sc + nc, dkeys.as<int>(), s);
     SP3MM_HIP(hipGetLastError());
     std::vector<unsigned long long> h((size_t)nc * 2);
     SP3MM_HIP(hipMemcpyAsync(h.data(), sc, (size_t)nc * 2 * 8, hipMemcpyDeviceToHost, s));
+    if (nkeys) {
+        nkeys->assign((size_t)nrows * 2, 0);
+        SP3MM_HIP(hipMemcpyAsync(nkeys->data(), dkeys.p, (size_t)nrows * 2 * 4, hipMemcpyDeviceToHost, s));
+    }
     SP3MM_HIP(hipStreamSynchronize(s));
     for (int st = 0; st < 2; st++) {
         std::vector<int> idx((size_t)nc);
@@ -2358,6 +2363,36 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
         }
         return 0;
     };
+    // Distinct key counts of the scoring's sample rows: a candidate whose
+    // likely-bin tables would not hold them for more than a tenth of the
+    // sampled rows is not timed (such rows go to the long-row kernels, at up
+    // to hundreds of products' time each).
+    std::vector<int> sample_keys;
+    {
+        int H1 = 0, H2 = 0;
+        std::vector<unsigned> r1, r2;
+        if (likely_tables(c, cur, R, A, H1, H2) && score_hashes(c, R, A, P, H1, H2, r1, r2, &sample_keys)) {
+            c->tune = saved;
+            return -1;
+        }
+    }
+    auto would_fill = [&](const Tuning &t) -> bool {
+        if (sample_keys.empty()) return false;
+        BinThresholds thr{};
+        std::vector<FusedCfg> cfg;
+        fused_bins(c->max_lds, prec == SP3MM_PREC_MIXED, t, thr, cfg);
+        const int b = likely_bin(thr, R, A);
+        if (b < 0) return false;
+        const FusedCfg &f = cfg[(size_t)b];
+        const int cap1 = f.G > kWave ? f.H1 - f.H1 / 8 : f.H1, cap2 = f.G > kWave ? f.H2 - f.H2 / 8 : f.H2;
+        int n = 0, over = 0;
+        for (size_t i = 0; i + 1 < sample_keys.size(); i += 2) {
+            if (sample_keys[i] == INT32_MAX || sample_keys[i + 1] == INT32_MAX) continue;
+            n++;
+            over += sample_keys[i] > cap1 || sample_keys[i + 1] > cap2;
+        }
+        return n > 0 && 10 * over > n;
+    };
     bool hashed = false;
     // coordinate descent from `cur` (measured: cur_ms)
     auto descend = [&]() -> int {
@@ -2403,7 +2438,7 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
                     Tuning t = cur;
                     t.*ax.field = v;
                     if (hashed && !reassociated && rehash(t)) return -1;
-                    if (!tuning_valid(t)) continue;
+                    if (!tuning_valid(t) || would_fill(t)) continue;
                     if (step(t, took)) {
                         return -1;
                     }
@@ -2432,7 +2467,7 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
     mesh.h2_div = 4;
     const Tuning end1 = cur;
     const double end1_ms = cur_ms;
-    if (!reassociated && !same_tuning(mesh, saved)) {
+    if (!reassociated && !same_tuning(mesh, saved) && !would_fill(mesh)) {
         cur = mesh;
         if (measure(cur, cur_ms) || descend()) {
             c->tune = saved;

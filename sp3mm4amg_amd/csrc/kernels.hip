@@ -873,13 +873,15 @@ __global__ void __launch_bounds__(256, 6) k_direct_fused(SpOperand R, SpOperand 
 // linear probing that sum does not depend on the insert order, so the scores
 // are deterministic.  A row with more distinct keys at a stage than its
 // dedup table takes (kScoreDedup / 2), or than 7/8 of H1 (H2), is skipped for
-// that stage by every candidate.
+// that stage by every candidate.  nkeys[2i], nkeys[2i+1] receive sample row
+// i's distinct stage-1 and stage-2 key counts.
 constexpr int kScoreDedup = 2048;
 __global__ void __launch_bounds__(256) k_hash_score(SpOperand R, SpOperand A, SpOperand P,
                                                    const int32_t *__restrict__ rows, const unsigned *__restrict__ cand,
                                                    int ncand, int H1, int H2,
                                                    unsigned long long *__restrict__ score1,
-                                                   unsigned long long *__restrict__ score2)
+                                                   unsigned long long *__restrict__ score2,
+                                                   int *__restrict__ nkeys)
 {
     __shared__ int d1[kScoreDedup], d2[kScoreDedup];
     __shared__ int red[Group<256>::kWaves + 1];
@@ -920,6 +922,11 @@ __global__ void __launch_bounds__(256) k_hash_score(SpOperand R, SpOperand A, Sp
     }
     __syncthreads();
     const int n2 = (over || n1 > kScoreDedup / 2) ? kScoreDedup : compact_table<256, double>(d2, nullptr, kScoreDedup, red);
+    // the row's distinct key counts (INT32_MAX: beyond the dedup tables)
+    if (t == 0) {
+        nkeys[2 * blockIdx.x] = n1 <= kScoreDedup / 2 ? n1 : INT32_MAX;
+        nkeys[2 * blockIdx.x + 1] = n2 <= kScoreDedup / 2 ? n2 : INT32_MAX;
+    }
     const bool use1 = n1 <= kScoreDedup / 2 && n1 <= H1 - H1 / 8;
     const bool use2 = n2 <= kScoreDedup / 2 && n2 <= H2 - H2 / 8;
     int *t1 = tabs + w * (H1 + H2), *t2 = t1 + H1;
@@ -2501,14 +2508,14 @@ size_t hash_score_lds(int H1, int H2)
 
 void launch_hash_score(const SpOperand &R, const SpOperand &A, const SpOperand &P, const int32_t *rows, int nrows,
                        const unsigned *cand, int ncand, int H1, int H2, unsigned long long *score1,
-                       unsigned long long *score2, hipStream_t st)
+                       unsigned long long *score2, int *nkeys, hipStream_t st)
 {
     (void)hipMemsetAsync(score1, 0, sizeof(unsigned long long) * ncand, st);
     (void)hipMemsetAsync(score2, 0, sizeof(unsigned long long) * ncand, st);
     if (nrows <= 0) return;
     const size_t lds = (size_t)Group<256>::kWaves * (H1 + H2) * sizeof(int);
     hipLaunchKernelGGL(k_hash_score, dim3(nrows), dim3(256), lds, st, R, A, P, rows, cand, ncand, H1, H2, score1,
-                       score2);
+                       score2, nkeys);
 }
 
 void launch_gather_rows(const int32_t *rows, const int *count, int rows_host, const int32_t *slotn, int cap,

@@ -147,13 +147,14 @@ void launch_direct_fused(int G, int H1, int H2, int slog1, int slog2, const SpOp
                          const Tuning &tu);
 // Scores of candidate hash multipliers for the direct kernel's tables of H1 and
 // H2 slots (both at most kScoreTable) on sample rows of R (see k_hash_score):
-// score1[c] / score2[c] (zeroed here) receive the summed home-slot distances.
-// The launch takes hash_score_lds(H1, H2) bytes of LDS per workgroup.
+// score1[c] / score2[c] (zeroed here) receive the summed home-slot distances,
+// nkeys (2 * nrows) every sample row's distinct stage-1 and stage-2 key
+// counts.  The launch takes hash_score_lds(H1, H2) bytes of LDS per workgroup.
 size_t hash_score_lds(int H1, int H2);
 constexpr int kScoreTable = 4096;
 void launch_hash_score(const SpOperand &R, const SpOperand &A, const SpOperand &P, const int32_t *rows, int nrows,
                        const unsigned *cand, int ncand, int H1, int H2, unsigned long long *score1,
-                       unsigned long long *score2, hipStream_t st);
+                       unsigned long long *score2, int *nkeys, hipStream_t st);
 // one-pass SpGEMM rows into scratch slots of H entries (see k_spgemm_fused)
 size_t spgemm_fused_lds(int G, int H);
 void launch_spgemm_fused(int G, int H, int slog, const SpOperand &A, const SpOperand &B, const int32_t *rows,
