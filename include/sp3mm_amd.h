@@ -456,6 +456,22 @@ int sp3mm_autotune(const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P
 int sp3mm_batch_autotune(sp3mm_batch *b, int32_t reps, sp3mm_tuning *best,
                          double *ms_best, double *ms_default);
 
+/* Scores of the candidate hash multipliers the autotuner ranks (no
+ * counterpart in the reference; exposed for inspection and tests).  For
+ * tables of H1 and H2 slots (powers of two in [64, 4096]) and the rows
+ * rows[0..nrows) of R (host array): cand[c] receives candidate c's
+ * multiplier, score1[c] / score2[c] the summed distances of the rows'
+ * distinct stage-1 keys (columns of the rows of A that R's row selects) /
+ * stage-2 keys (columns of the rows of P that those select) from their home
+ * slots under linear probing, nkeys[2i] and nkeys[2i+1] row i's distinct key
+ * counts (INT32_MAX beyond 1024).  A row with more than 1024 distinct keys at
+ * a stage, or more than 7/8 of the stage's table, adds nothing to that
+ * stage's scores.  Returns the number of candidates (cap must hold them), or
+ * -1. */
+int32_t sp3mm_hash_scores(const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P, int32_t H1, int32_t H2,
+                          const int32_t *rows, int32_t nrows, int32_t cap, uint32_t *cand, uint64_t *score1,
+                          uint64_t *score2, int32_t *nkeys);
+
 /* Device memory from the library's pool (e.g. for sp3mm_opts.rowsum), its
  * release, and blocking copies between host and device. */
 void *sp3mm_device_alloc(int64_t bytes);

@@ -690,6 +690,40 @@ int sp3mm_autotune(const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P
     return 0;
 }
 
+int32_t sp3mm_hash_scores(const sp3mm_dmat *R, const sp3mm_dmat *A, const sp3mm_dmat *P, int32_t H1, int32_t H2,
+                          const int32_t *rows, int32_t nrows, int32_t cap, uint32_t *cand, uint64_t *score1,
+                          uint64_t *score2, int32_t *nkeys)
+{
+    clear_error();
+    if (!R || !A || !P || !rows || nrows <= 0 || !cand || !score1 || !score2 || !nkeys) {
+        set_error("hash_scores: null argument or no rows");
+        return -1;
+    }
+    if (R->N != A->M || A->N != P->M) {
+        set_error("hash_scores: inner dimensions differ");
+        return -1;
+    }
+    Context *c = ctx();
+    if (!c) return -1;
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    const std::vector<unsigned> &list = hash_candidate_list();
+    if (cap < (int32_t)list.size()) {
+        set_error("hash_scores: room for %d candidates, %zu needed", cap, list.size());
+        return -1;
+    }
+    std::vector<unsigned long long> h;
+    std::vector<int> keys;
+    if (hash_scores(c, *R, *A, *P, H1, H2, rows, nrows, h, &keys)) return -1;
+    const size_t nc = list.size();
+    for (size_t i = 0; i < nc; i++) {
+        cand[i] = list[i];
+        score1[i] = h[i];
+        score2[i] = h[nc + i];
+    }
+    for (size_t i = 0; i < keys.size(); i++) nkeys[i] = keys[i];
+    return (int32_t)nc;
+}
+
 void *sp3mm_device_alloc(int64_t bytes)
 {
     Context *c = ctx();

@@ -2173,6 +2173,49 @@ static bool likely_tables(Context *c, const Tuning &t, const sp3mm_dmat &R, cons
     return H1 <= kScoreTable && H2 <= kScoreTable;
 }
 
+// Scores of the candidates for tables of H1 and H2 slots on the given rows of
+// R (k_hash_score): h[c] / h[ncand + c] the summed home-slot distances at
+// stage 1 / 2, nkeys (optional) the rows' distinct key counts.
+int hash_scores(Context *c, const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int H1, int H2,
+                const int32_t *rows, int nrows, std::vector<unsigned long long> &h, std::vector<int> *nkeys)
+{
+    const std::vector<unsigned> &cand = hash_candidates();
+    const int nc = (int)cand.size();
+    if (nrows <= 0 || !A.rp32 || !P.rp32 || H1 < 64 || H2 < 64 || (H1 & (H1 - 1)) || (H2 & (H2 - 1)) ||
+        H1 > kScoreTable || H2 > kScoreTable || hash_score_lds(H1, H2) > c->lds_device) {
+        set_error("hash scoring: tables of %d and %d slots (powers of two in [64, %d]), %d rows", H1, H2,
+                  kScoreTable, nrows);
+        return -1;
+    }
+    for (int i = 0; i < nrows; i++)
+        if (rows[i] < 0 || rows[i] >= R.M) {
+            set_error("hash scoring: row %d out of range", rows[i]);
+            return -1;
+        }
+    Buf drows((size_t)nrows * 4), dcand((size_t)nc * 4), dscore((size_t)nc * 2 * 8), dkeys((size_t)nrows * 2 * 4);
+    if (!drows.p || !dcand.p || !dscore.p || !dkeys.p) {
+        set_error("hash scoring: allocation failed");
+        return -1;
+    }
+    hipStream_t s = c->stream;
+    SP3MM_HIP(hipMemcpyAsync(drows.p, rows, (size_t)nrows * 4, hipMemcpyHostToDevice, s));
+    SP3MM_HIP(hipMemcpyAsync(dcand.p, cand.data(), (size_t)nc * 4, hipMemcpyHostToDevice, s));
+    auto *sc = dscore.as<unsigned long long>();
+    launch_hash_score(operand(R), operand(A), operand(P), drows.as<int32_t>(), nrows, dcand.as<unsigned>(), nc, H1,
+                      H2, sc, sc + nc, dkeys.as<int>(), s);
+    SP3MM_HIP(hipGetLastError());
+    h.assign((size_t)nc * 2, 0);
+    SP3MM_HIP(hipMemcpyAsync(h.data(), sc, (size_t)nc * 2 * 8, hipMemcpyDeviceToHost, s));
+    if (nkeys) {
+        nkeys->assign((size_t)nrows * 2, 0);
+        SP3MM_HIP(hipMemcpyAsync(nkeys->data(), dkeys.p, (size_t)nrows * 2 * 4, hipMemcpyDeviceToHost, s));
+    }
+    SP3MM_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+const std::vector<unsigned> &hash_candidate_list() { return hash_candidates(); }
+
 // For tables of H1 and H2 slots: the candidates ranked by the summed
 // home-slot distance of the keys of 512 sample rows of R (evenly spaced), at
 // stage 1 (rank1) and stage 2 (rank2); equal scores keep the list order, so
@@ -2191,25 +2234,8 @@ static int score_hashes(Context *c, const sp3mm_dmat &R, const sp3mm_dmat &A, co
         return 0;
     std::vector<int32_t> rows((size_t)nrows);
     for (int i = 0; i < nrows; i++) rows[(size_t)i] = (int32_t)(((2 * (int64_t)i + 1) * R.M) / (2 * (int64_t)nrows));
-    Buf drows((size_t)nrows * 4), dcand((size_t)nc * 4), dscore((size_t)nc * 2 * 8), dkeys((size_t)nrows * 2 * 4);
-    if (!drows.p || !dcand.p || !dscore.p || !dkeys.p) {
-        set_error("hash scoring: allocation failed");
-        return -1;
-    }
-    hipStream_t s = c->stream;
-    SP3MM_HIP(hipMemcpyAsync(drows.p, rows.data(), (size_t)nrows * 4, hipMemcpyHostToDevice, s));
-    SP3MM_HIP(hipMemcpyAsync(dcand.p, cand.data(), (size_t)nc * 4, hipMemcpyHostToDevice, s));
-    auto *sc = dscore.as<unsigned long long>();
-    launch_hash_score(operand(R), operand(A), operand(P), drows.as<int32_t>(), nrows, dcand.as<unsigned>(), nc, H1,
-                      H2, sc, sc + nc, dkeys.as<int>(), s);
-    SP3MM_HIP(hipGetLastError());
-    std::vector<unsigned long long> h((size_t)nc * 2);
-    SP3MM_HIP(hipMemcpyAsync(h.data(), sc, (size_t)nc * 2 * 8, hipMemcpyDeviceToHost, s));
-    if (nkeys) {
-        nkeys->assign((size_t)nrows * 2, 0);
-        SP3MM_HIP(hipMemcpyAsync(nkeys->data(), dkeys.p, (size_t)nrows * 2 * 4, hipMemcpyDeviceToHost, s));
-    }
-    SP3MM_HIP(hipStreamSynchronize(s));
+    std::vector<unsigned long long> h;
+    if (hash_scores(c, R, A, P, H1, H2, rows.data(), nrows, h, nkeys)) return -1;
     for (int st = 0; st < 2; st++) {
         std::vector<int> idx((size_t)nc);
         for (int i = 0; i < nc; i++) idx[(size_t)i] = i;

@@ -90,6 +90,11 @@ int consumer_wait(Context *c, double *dot_out = nullptr);
 // blocking wait returns some 20 us after the GPU is done; a product's final
 // wait is on its critical path.
 hipError_t spin_stream(hipStream_t s);
+// the hash multipliers the autotuner scores, and their raw scores on given
+// rows of R (see k_hash_score)
+const std::vector<unsigned> &hash_candidate_list();
+int hash_scores(Context *c, const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int H1, int H2,
+                const int32_t *rows, int nrows, std::vector<unsigned long long> &h, std::vector<int> *nkeys);
 hipError_t spin_event(hipEvent_t e);
 
 // Queues `sweeps` damped Jacobi sweeps (x in place, work as the other buffer).

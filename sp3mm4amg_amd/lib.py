@@ -186,6 +186,9 @@ def lib():
         L.sp3mm_get_stream.restype = C.c_void_p
         L.sp3mm_copy_probe.argtypes = [C.c_int64, C.c_int32]
         L.sp3mm_copy_probe.restype = C.c_double
+        L.sp3mm_hash_scores.argtypes = [P(DMat), P(DMat), P(DMat), C.c_int32, C.c_int32, C.c_void_p,
+                                        C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.sp3mm_hash_scores.restype = C.c_int32
         _LIB = L
     return _LIB
 
@@ -608,6 +611,26 @@ def autotune(R: DeviceCSR, A: DeviceCSR, P: DeviceCSR, precision: int = FP64, gu
     _check(lib().sp3mm_autotune(C.byref(R.d), C.byref(A.d), C.byref(P.d), C.byref(o), int(reps),
                                 C.byref(t), C.byref(mb), C.byref(md)), "sp3mm_autotune")
     return t.as_dict(), mb.value, md.value
+
+
+def hash_scores(R: DeviceCSR, A: DeviceCSR, P: DeviceCSR, H1: int, H2: int, rows):
+    """The autotuner's hash-multiplier scores on the given rows of R: returns
+    (multipliers, stage-1 scores, stage-2 scores, per-row distinct key counts
+    as an (nrows, 2) array), all numpy arrays."""
+    import numpy as np
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    cap = 4096
+    cand = np.zeros(cap, np.uint32)
+    s1 = np.zeros(cap, np.uint64)
+    s2 = np.zeros(cap, np.uint64)
+    nk = np.zeros((len(rows), 2), np.int32)
+    n = lib().sp3mm_hash_scores(C.byref(R.d), C.byref(A.d), C.byref(P.d), int(H1), int(H2),
+                                rows.ctypes.data_as(C.c_void_p), len(rows), cap,
+                                cand.ctypes.data_as(C.c_void_p), s1.ctypes.data_as(C.c_void_p),
+                                s2.ctypes.data_as(C.c_void_p), nk.ctypes.data_as(C.c_void_p))
+    if n < 0:
+        raise RuntimeError(f"sp3mm_hash_scores failed: {last_error()}")
+    return cand[:n], s1[:n], s2[:n], nk
 
 
 def copy_probe(nbytes: int, reps: int = 20) -> float:
