@@ -2140,6 +2140,8 @@ __device__ __forceinline__ void block_minmax(int &lo, int &hi, int *sh)
     __syncthreads();
 }
 
+constexpr int kWinSub2 = 8;  // lanes per forwarded entry in k_window_direct's stage 2
+
 template <bool NUM, typename VT>
 __global__ void __launch_bounds__(kWin) k_window_direct(SpOperand R, SpOperand A, SpOperand P,
                                                            const int32_t *__restrict__ rows,
@@ -2258,39 +2260,46 @@ __global__ void __launch_bounds__(kWin) k_window_direct(SpOperand R, SpOperand A
         if constexpr (NUM)
             for (int j = t; j < span2; j += kWin) V2[j] = 0.0;
         __syncthreads();
-        // stage 2: every present entry of the R*A row is forwarded by one lane
+        // stage 2: sub-groups of kWinSub2 lanes walk the P rows of the present
+        // entries (positions strided over the sub-groups), two entries per
+        // lane per step.  A lane per present entry walking its P row alone
+        // made every load of a wavefront touch 64 rows (the 1728-row level
+        // of a 96^3 hierarchy took 0.95 instead of 0.70 ms per product).
         double ps = 0.0;
-        for (int j = t; j < span1; j += kWin) {
-            if (!(B[j >> 5] & (1u << (j & 31)))) continue;
-            const int c = lo1 + j;
-            double x = 0.0;
-            if constexpr (NUM) x = V[j];
-            const int p1 = P.rp32[c + 1];
-            // four entries of the P row at a time (loads in flight together)
-            for (int q0 = P.rp32[c]; q0 < p1; q0 += 4) {
-                int jj[4];
-                double pv[4];
+        {
+            constexpr int S2 = kWinSub2;
+            const int sub = t & (S2 - 1), ngrp = kWin / S2;
+            for (int j = t / S2; j < span1; j += ngrp) {
+                if (!(B[j >> 5] & (1u << (j & 31)))) continue;
+                const int c = lo1 + j;
+                double x = 0.0;
+                if constexpr (NUM) x = V[j];
+                const int p1 = P.rp32[c + 1];
+                for (int q0 = P.rp32[c] + sub; q0 < p1; q0 += 2 * S2) {
+                    int jj[2];
+                    double pv[2];
 #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const bool ok = q0 + u < p1;
-                    jj[u] = ok ? P.col[q0 + u] - lo2 : 0;
-                    pv[u] = 0.0;
-                    if constexpr (NUM) pv[u] = ok ? (double)pval[q0 + u] : 0.0;
-                }
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    if (q0 + u >= p1) break;
-                    const int j2 = jj[u];
-                    if ((unsigned)j2 >= (unsigned)span2) {  // an unsorted P row
-                        atomicOr(err, kErrBadInput);
-                        continue;
+                    for (int u = 0; u < 2; u++) {
+                        const bool ok = q0 + u * S2 < p1;
+                        jj[u] = ok ? P.col[q0 + u * S2] - lo2 : 0;
+                        pv[u] = 0.0;
+                        if constexpr (NUM) pv[u] = ok ? (double)pval[q0 + u * S2] : 0.0;
                     }
-                    const unsigned bit2 = 1u << (j2 & 31);
-                    if (!(B2[j2 >> 5] & bit2)) atomicOr(B2 + (j2 >> 5), bit2);
-                    if constexpr (NUM) {
-                        const double pr = x * pv[u];
-                        atomicAdd(V2 + j2, pr);
-                        ps += pr;
+#pragma unroll
+                    for (int u = 0; u < 2; u++) {
+                        if (q0 + u * S2 >= p1) break;
+                        const int j2 = jj[u];
+                        if ((unsigned)j2 >= (unsigned)span2) {  // an unsorted P row
+                            atomicOr(err, kErrBadInput);
+                            continue;
+                        }
+                        const unsigned bit2 = 1u << (j2 & 31);
+                        if (!(B2[j2 >> 5] & bit2)) atomicOr(B2 + (j2 >> 5), bit2);
+                        if constexpr (NUM) {
+                            const double pr = x * pv[u];
+                            atomicAdd(V2 + j2, pr);
+                            ps += pr;
+                        }
                     }
                 }
             }
