@@ -745,7 +745,8 @@ static int spgemm_onepass(const sp3mm_dmat &A, const sp3mm_dmat &B, sp3mm_dmat &
     Ctl h;
     SP3MM_HIP(hipMemsetAsync(d, 0, sizeof(Ctl), s));
     launch_ub_bin(a, b, ub.as<int64_t>(), rown.as<int32_t>(), lists.as<int32_t>(), d->counts, thr, d->totals,
-                  binscr.as<int>(), s);
+                  binscr.as<int>(), s, 0, nullptr, nullptr, nullptr, 0, nullptr, 0, 0, nullptr, -1, 0, false, nullptr,
+                  M > 0 && A.nnz >= (int64_t)kWaveUbRow * M);
     if (read_ctl(c, d, h)) return -1;
     const long long mac = (long long)h.totals[0];
     std::vector<int64_t> soff(thr.nb + 1, 0), nbase(thr.nb + 1, 0);
@@ -1357,7 +1358,7 @@ struct FusedPlan {
         launch_ub_bin(r(), a(), ub.as<int64_t>(), rown.as<int32_t>(), lists.as<int32_t>(), d()->rawc, thr,
                       d()->totals, binscr.as<int>(), s, kMergeRows, d()->counts, d(), host,
                       (int)(sizeof(Ctl) / 4), flag, seq, span_lim, &d()->pred_count, pred, big_from,
-                      reassoc_possible(c), one ? &ob : nullptr);
+                      reassoc_possible(c), one ? &ob : nullptr, R.M > 0 && R.nnz >= (int64_t)kWaveUbRow * R.M);
     }
 
     // R*(A*P) instead of the direct forwarding: when at least half of R's

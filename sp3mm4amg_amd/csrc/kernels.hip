@@ -138,13 +138,42 @@ __global__ void __launch_bounds__(256) k_row_maxlen(const IA *__restrict__ arp, 
     }
 }
 
+// Upper bounds ub(r) = sum of nnz(B_k) over the entries k of row r of A, one
+// wavefront per row (four entries per lane in flight): for operators whose
+// rows run to hundreds of entries, where one thread per row would walk its
+// row alone (on the 216-row level of a 96^3 hierarchy, rows of up to 1004
+// entries, that took 67 us).
+__global__ void __launch_bounds__(256) k_row_ub_wave(const int64_t *__restrict__ arp,
+                                                      const int32_t *__restrict__ acol,
+                                                      const int32_t *__restrict__ brp, int64_t M,
+                                                      int64_t *__restrict__ ub)
+{
+    const int64_t r = (int64_t)blockIdx.x * (256 / kWave) + threadIdx.x / kWave;
+    if (r >= M) return;  // whole wavefronts leave together
+    const int l = lane();
+    const int64_t a0 = arp[r], a1 = arp[r + 1];
+    long long s = 0;
+    for (int64_t e = a0 + l; e < a1; e += 4 * kWave) {
+        int k[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) k[u] = acol[e + u * kWave < a1 ? e + u * kWave : a0];
+        int len[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) len[u] = brp[k[u] + 1] - brp[k[u]];
+#pragma unroll
+        for (int u = 0; u < 4; u++) s += e + u * kWave < a1 ? len[u] : 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (l == 0) ub[r] = s;
+}
+
 __global__ void __launch_bounds__(kBinThreads) k_bin_hist(const int64_t *__restrict__ arp, const int32_t *__restrict__ acol,
                                                            const int32_t *__restrict__ brp, const int32_t *__restrict__ meas1,
                                                            const int32_t *__restrict__ meas2, int64_t M,
                                                            int64_t *__restrict__ ub, int32_t *__restrict__ rownnz,
                                                            int *__restrict__ hist, unsigned long long *__restrict__ part,
                                                            BinThresholds thr, uint32_t *__restrict__ zero, int zero_words,
-                                                           int span_lim, OneBin ob)
+                                                           int span_lim, OneBin ob, int pre_ub)
 {
     __shared__ int h[kMaxBins + 1];
     // one-bin mode: the longest rows bound every row's ub by ob.bound, so
@@ -199,8 +228,14 @@ __global__ void __launch_bounds__(kBinThreads) k_bin_hist(const int64_t *__restr
         // (the scatter lists the rows in row order without reading them)
         if (threadIdx.x == 0) h[bin_of(ob.u, thr)] = (int)min((int64_t)kBinThreads, M - blockIdx.x * (int64_t)kBinThreads);
     } else if (r < M) {
-        u = row_measure(r, arp, acol, brp, meas1, meas2, kspan);
-        if (ub) ub[r] = u;
+        if (pre_ub) {  // k_row_ub_wave measured the row
+            u = ub[r];
+            const int64_t a0 = arp[r], a1 = arp[r + 1];
+            kspan = a1 > a0 ? acol[a1 - 1] - acol[a0] : 0;
+        } else {
+            u = row_measure(r, arp, acol, brp, meas1, meas2, kspan);
+            if (ub) ub[r] = u;
+        }
         if (u <= 0 && rownnz) rownnz[r] = 0;
         const int b = bin_of(u, thr);
         if (b >= 0) atomicAdd(h + b, 1);
@@ -1916,9 +1951,16 @@ int64_t bin_scratch_ints(int64_t M) { return (kMaxBins + 1) * ((M + kBinThreads 
 static void launch_bins(const SpOperand *A, const SpOperand *B, const int32_t *m1, const int32_t *m2, int64_t M,
                         int64_t *ub, int32_t *rownnz, int32_t *lists, int *counts, const BinThresholds &thr,
                         unsigned long long *totals, int *scratch, hipStream_t st, int merge_lim = 0,
-                        int *mcounts = nullptr, const CtlExport *ex = nullptr, const OneBin *one = nullptr)
+                        int *mcounts = nullptr, const CtlExport *ex = nullptr, const OneBin *one = nullptr,
+                        bool wave_ub = false)
 {
     if (M == 0) return;
+    // long rows of A: their bounds measured a wavefront per row first (not
+    // when the one-bin test may make measuring unnecessary)
+    const bool pre = wave_ub && A && B && ub && !one;
+    if (pre)
+        hipLaunchKernelGGL(k_row_ub_wave, dim3(blocks_for(M, 256 / kWave)), dim3(256), 0, st, A->rp, A->col,
+                           B->rp32, M, ub);
     OneBin ob{};
     ob.word = -1;
     if (one && A && B) {
@@ -1939,7 +1981,7 @@ static void launch_bins(const SpOperand *A, const SpOperand *B, const int32_t *m
     uint32_t *zero = ex && ex->clear ? const_cast<uint32_t *>(ex->src) : nullptr;
     hipLaunchKernelGGL(k_bin_hist, dim3(nblk), dim3(kBinThreads), 0, st, A ? A->rp : nullptr, A ? A->col : nullptr,
                        B ? B->rp32 : nullptr, m1, m2, M, ub, rownnz, hist, part, thr, zero, zero ? ex->words : 0,
-                       ex ? ex->span_lim : 0, ob);
+                       ex ? ex->span_lim : 0, ob, pre ? 1 : 0);
     hipLaunchKernelGGL(k_bin_offsets, dim3(thr.nb + 2), dim3(1024), 0, st, hist, nblk, thr.nb + 1, part, counts,
                        totals);
     CtlExport e{};
@@ -1952,13 +1994,14 @@ void launch_ub_bin(const SpOperand &A, const SpOperand &B, int64_t *ub, int32_t 
                    int32_t *lists, int *counts, const BinThresholds &thr,
                    unsigned long long *totals, int *scratch, hipStream_t st, int merge_lim, int *mcounts,
                    const void *ctl, void *ctl_host, int ctl_words, unsigned long long *flag, unsigned long long seq,
-                   int span_lim, int *pred_count, int pred, int big_from, bool reassoc_possible, const OneBin *one)
+                   int span_lim, int *pred_count, int pred, int big_from, bool reassoc_possible, const OneBin *one,
+                   bool wave_ub)
 {
     CtlExport e{static_cast<const uint32_t *>(ctl), static_cast<uint32_t *>(ctl_host), ctl_words, flag, seq,
                 ctl != nullptr, span_lim, pred_count, pred, big_from, reassoc_possible, totals,
                 ctl && one ? one->word : -1};
     launch_bins(&A, &B, nullptr, nullptr, A.M, ub, rownnz, lists, counts, thr, totals, scratch, st, merge_lim,
-                mcounts, ctl ? &e : nullptr, ctl ? one : nullptr);
+                mcounts, ctl ? &e : nullptr, ctl ? one : nullptr, wave_ub);
 }
 
 void launch_ub_colparts(const int64_t *arp, const int32_t *acol, int64_t M, const int64_t *boff, int G,
@@ -2387,7 +2430,10 @@ void launch_window_direct(bool num, int blocks, size_t lds, const SpOperand &R, 
 {
     const int cap = window_cap(lds);
     const size_t bytes = (size_t)cap * 8 + (size_t)cap / 32 * 4;
-    if (slog1 > 6) slog1 = 6;
+    // at most 16 lanes per R entry: 64 entries of R's row in flight per
+    // workgroup (64 lanes each, 16 in flight, made the 1728-row level of a
+    // 96^3 hierarchy 6% slower)
+    if (slog1 > 4) slog1 = 4;
 #define SP3MM_WIN_ARGS R, A, P, rows, count, cap, slog1, spill, nspill, rownnz, crp, ccol, cval, rowsum, out_cap, \
                        err, slotn, scol, sval, cap_rows
     if (f32in) {

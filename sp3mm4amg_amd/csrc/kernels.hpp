@@ -17,6 +17,7 @@ struct SpOperand {
 };
 
 constexpr int kMaxBins = 14;
+constexpr int kWaveUbRow = 64;
 
 // Default multiplier of the LDS hash tables (slot = top bits of key * mul):
 // 2^32 / golden ratio, rounded to odd.
@@ -62,7 +63,9 @@ void launch_ub_bin(const SpOperand &A, const SpOperand &B, int64_t *ub, int32_t 
                    int *mcounts = nullptr, const void *ctl = nullptr, void *ctl_host = nullptr, int ctl_words = 0,
                    unsigned long long *flag = nullptr, unsigned long long seq = 0, int span_lim = 0,
                    int *pred_count = nullptr, int pred = -1, int big_from = 0, bool reassoc_possible = false,
-                   const OneBin *one = nullptr);
+                   const OneBin *one = nullptr, bool wave_ub = false);
+// wave_ub: the bounds are measured a wavefront per row of A first (rows of
+// A averaging kWaveUbRow entries or more), then binned
 void launch_ub_colparts(const int64_t *arp, const int32_t *acol, int64_t M, const int64_t *boff, int G,
                         int64_t *out, unsigned long long *total, hipStream_t st);
 void launch_rebin(const int32_t *m1, const int32_t *m2, int64_t M, int32_t *lists, int *counts,
