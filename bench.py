@@ -243,7 +243,7 @@ def batch_leg(lib, problems, reps, tune=True):
         lib.set_tuning(None)
         b = lib.Batch([m[0] for m in mats], [m[1] for m in mats], [m[2] for m in mats], graph=True,
                       precision=prec)
-        schedule = b.autotune(reps=3)[0] if tune else None
+        schedule = b.autotune(reps=5)[0] if tune else None
         # per-product loop
         for (dR, dA, dP) in mats:
             out, _ = lib.rap(dR, dA, dP, **kw)

@@ -434,11 +434,11 @@ typedef struct {
                             contiguous part of a bin's rows, 0: rows interleaved over all
                             workgroups */
     uint32_t hash1, hash2; /* multipliers of the stage-1 / stage-2 LDS hash tables of the
-                              FP64-accumulating direct kernel (slot = top bits of
-                              key * multiplier); odd, 0: the default 0x9E3779B1.  The
-                              autotuner scores candidates on sample rows of the
-                              operands and takes the ones that leave the fewest keys off
-                              their home slots */
+                              direct kernels (and of the pair path's two SpGEMMs; slot =
+                              top bits of key * multiplier); odd, 0: the default
+                              0x9E3779B1.  The autotuner scores candidates on sample
+                              rows of the operands and takes the ones that leave the
+                              fewest keys off their home slots */
 } sp3mm_tuning;
 int sp3mm_get_tuning(sp3mm_tuning *t);
 /* NULL restores the defaults. */

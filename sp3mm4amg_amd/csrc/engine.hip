@@ -2162,11 +2162,12 @@ static const std::vector<unsigned> &hash_candidates()
 
 // Tables of the likely bin under schedule t (false: no such bin, or tables
 // beyond what k_hash_score takes)
-static bool likely_tables(Context *c, const Tuning &t, const sp3mm_dmat &R, const sp3mm_dmat &A, int &H1, int &H2)
+static bool likely_tables(Context *c, const Tuning &t, const sp3mm_dmat &R, const sp3mm_dmat &A, int &H1, int &H2,
+                          bool mixed = false)
 {
     BinThresholds thr{};
     std::vector<FusedCfg> cfg;
-    fused_bins(c->max_lds, false, t, thr, cfg);
+    fused_bins(c->max_lds, mixed, t, thr, cfg);
     const int b = likely_bin(thr, R, A);
     if (b < 0) return false;
     H1 = cfg[b].H1;
@@ -2363,14 +2364,13 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
     // multipliers are taken (`hashed`), every candidate schedule gets the
     // best-ranked for its own tables, and each pass times the next kHashTries
     // of each stage's ranking against the current ones: multipliers that leave
-    // every key on its home slot can still differ in LDS bank conflicts.  The
-    // float-pair kernel keeps the default.
+    // every key on its home slot can still differ in LDS bank conflicts.
     constexpr int kHashTries = 3;
     std::map<std::pair<int, int>, std::pair<std::vector<unsigned>, std::vector<unsigned>>> hcache;
     auto ranked = [&](const Tuning &t, const std::vector<unsigned> *&r1, const std::vector<unsigned> *&r2) -> int {
         r1 = r2 = nullptr;
         int H1 = 0, H2 = 0;
-        if (prec == SP3MM_PREC_MIXED || !likely_tables(c, t, R, A, H1, H2)) return 0;
+        if (!likely_tables(c, t, R, A, H1, H2, prec == SP3MM_PREC_MIXED)) return 0;
         auto it = hcache.find({H1, H2});
         if (it == hcache.end()) {
             std::vector<unsigned> a, b;
@@ -2398,7 +2398,8 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
     {
         int H1 = 0, H2 = 0;
         std::vector<unsigned> r1, r2;
-        if (likely_tables(c, cur, R, A, H1, H2) && score_hashes(c, R, A, P, H1, H2, r1, r2, &sample_keys)) {
+        if (likely_tables(c, cur, R, A, H1, H2, prec == SP3MM_PREC_MIXED) &&
+            score_hashes(c, R, A, P, H1, H2, r1, r2, &sample_keys)) {
             c->tune = saved;
             return -1;
         }

@@ -138,12 +138,12 @@ size_t mixed_lds(int G, int H1, int H2)
 // of 2^slog lanes per left entry, U left entries per lane per pass, the first
 // probe of every insert issued back to back); `loadl(e)` and `loadb(q)` fetch
 // the left and right payloads and `acc(slot, left, right, col)` accumulates one
-// product into an occupied slot.
+// product into an occupied slot.  Home slots come from the multiplier hmul.
 template <int G, int U, typename LoadL, typename LoadB, typename Acc>
 __device__ __forceinline__ void expand_mixed(const int32_t *lcol, int64_t e0, int64_t e1,
                                              const int32_t *__restrict__ brp, const int32_t *__restrict__ bcol,
                                              int slog, int *keys, unsigned mask, int shift, bool &full,
-                                             int *fullflag, LoadL loadl, LoadB loadb, Acc acc)
+                                             int *fullflag, LoadL loadl, LoadB loadb, Acc acc, unsigned hmul)
 {
     using LT = decltype(loadl(int64_t{}));
     using BT = decltype(loadb(int{}));
@@ -182,7 +182,7 @@ __device__ __forceinline__ void expand_mixed(const int32_t *lcol, int64_t e0, in
             }
 #pragma unroll
             for (int u = 0; u < U; u++) {
-                sl[u] = hslot(c[u], shift) & mask;
+                sl[u] = hslot(c[u], shift, hmul) & mask;
                 old[u] = c[u] != kEmpty ? atomicCAS(keys + sl[u], kEmpty, c[u]) : kEmpty;
             }
             __builtin_amdgcn_sched_barrier(0);  // the U swaps stay in flight together
@@ -238,7 +238,8 @@ __global__ void __launch_bounds__(256) k_direct_mixed(SpOperand R, SpOperand A, 
                                                        double *__restrict__ rowsum,
                                                        int32_t *__restrict__ retry, int *__restrict__ nretry,
                                                        int32_t *__restrict__ fix, int *__restrict__ nfix,
-                                                       int *__restrict__ err, int nxcd, int cap_rows)
+                                                       int *__restrict__ err, int nxcd, int cap_rows,
+                                                       unsigned hm1, unsigned hm2)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int GPB = 256 / G;
@@ -314,7 +315,8 @@ __global__ void __launch_bounds__(256) k_direct_mixed(SpOperand R, SpOperand A, 
                 const FF pr = two_prod(x, w);
                 ff_atomic_accum(h1 + sl, l1 + sl, pr);
                 atomicAdd(a1 + sl, fabsf(pr.h));
-            });
+            },
+            hm1);
         const int64_t rbn = R.rp[rn], ren = R.rp[rn + 1];
         Group<G>::sync();
         const int nn1 = compact3<G>(k1, h1, l1, a1, H1, ired);
@@ -337,7 +339,8 @@ __global__ void __launch_bounds__(256) k_direct_mixed(SpOperand R, SpOperand A, 
                 const double pd = m.xd * (double)w;
                 rs += pd;
                 if (c == r) atomicAdd(dg, pd);
-            });
+            },
+            hm2);
         const long long anyfull = group_max<G>((full1 || full2) ? 1 : 0, red);
         if (anyfull) {
             if (t == 0) {
@@ -411,7 +414,8 @@ void launch_direct_mixed(int G, int H1, int H2, int slog1, int slog2, float guar
     auto fn = G == 32 ? k_direct_mixed<32> : G == 64 ? k_direct_mixed<64> : k_direct_mixed<256>;
     const int nxcd = tu.xcd_map && grid >= (unsigned)kXcds ? kXcds : 1;
     hipLaunchKernelGGL(fn, dim3(grid), dim3(256), lds, st, R, A, P, rows, count, H1, H2, slog1, slog2, guard,
-                       rownnz, slotn, scol, sval, rowsum, retry, nretry, fix, nfix, err, nxcd, rows_host);
+                       rownnz, slotn, scol, sval, rowsum, retry, nretry, fix, nfix, err, nxcd, rows_host,
+                       tu.hash1 | 1u, tu.hash2 | 1u);
 }
 
 }  // namespace sp3mm
