@@ -1463,7 +1463,9 @@ __global__ void __launch_bounds__(kDense) k_dense_spgemm(SpOperand A, SpOperand 
 // (cursor + sub + v*kWinSub) at once, the positions whose column falls inside
 // the window form a prefix of the sub-group's kWinVec*kWinSub positions (the
 // right row is sorted), and their count advances the cursor; a step with a
-// position past the window ends the entry's slice.
+// position past the window ends the entry's slice.  Each sub-group loads its
+// next round before running this round's atomics, so a round's loads wait
+// behind the atomics instead of after them.
 //
 // Count pass (NUM false): presence bits only, one window spanning all of B's
 // columns (they fit in LDS as bits), the row length to rownnz[r].  Numeric
@@ -1628,20 +1630,35 @@ __global__ void __launch_bounds__(kDense) k_spgemm_window(SpOperand A, SpOperand
                     }
                 }
                 bool more = ok && q < qe;
-                while (ballot(more)) {
-                    int c[kWinVec];
-                    double bv[kWinVec];
+                // rounds of kWinStep positions, loaded one round ahead into
+                // two register sets in turn: a sub-group's next round starts
+                // kWinStep positions on when this round was all inside the
+                // window, and its loads are in flight while this round's
+                // atomics run
+                int ca[kWinVec], cb[kWinVec];
+                double va[kWinVec], vb[kWinVec];
+                auto load = [&](int at, int *c, double *bv, unsigned &m) {
+                    m = 0u;
 #pragma unroll
                     for (int v = 0; v < kWinVec; v++) {
-                        const int qq = q + v * kWinSub + sub;
+                        const int qq = at + v * kWinSub + sub;
                         const bool valid = more && qq < qe;
-                        c[v] = valid ? B.col[qq] : INT_MAX;
-                        if constexpr (NUM) bv[v] = valid ? bval[qq] : 0.0;
+                        // unconditional loads (a valid index either way) and
+                        // the validity kept apart, so that neither a branch
+                        // nor a select on the loaded values comes before the
+                        // next round's loads are issued
+                        const int qs = valid ? qq : 0;
+                        m |= valid ? 1u << v : 0u;
+                        c[v] = B.col[qs];
+                        bv[v] = 0.0;
+                        if constexpr (NUM) bv[v] = bval[qs];
                     }
+                };
+                auto round = [&](const int *c, const double *bv, unsigned m) {
                     int cnt = 0;
 #pragma unroll
                     for (int v = 0; v < kWinVec; v++) {
-                        const bool in = c[v] < w1;
+                        const bool in = ((m >> v) & 1u) && c[v] < w1;
                         if (in) {
                             const int cw = c[v] - w0;
                             atomicOr(wbits + (cw >> 5), 1u << (cw & 31));
@@ -1651,6 +1668,18 @@ __global__ void __launch_bounds__(kDense) k_spgemm_window(SpOperand A, SpOperand
                     }
                     q += cnt;
                     more = more && cnt == kWinStep;
+                };
+                unsigned ma, mb;
+                if (ballot(more)) {  // some lane has entries: B is not empty
+                    load(q, ca, va, ma);
+                    for (;;) {
+                        load(q + kWinStep, cb, vb, mb);
+                        round(ca, va, ma);
+                        if (!ballot(more)) break;
+                        load(q + kWinStep, ca, va, ma);
+                        round(cb, vb, mb);
+                        if (!ballot(more)) break;
+                    }
                 }
                 if (ok && sub == 0 && e < kWinEntries) ecur[e] = q;
             }
