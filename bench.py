@@ -177,7 +177,7 @@ def mixed_leg(lib, problems, steps, warmup, tune, n=100):
         lib.set_tuning(None)
         sched = None
         if tune:
-            sched, _, _ = lib.autotune(*d, precision=prec, reps=3)
+            sched, _, _ = lib.autotune(*d, precision=prec, reps=7)
         wall, main, dev, st, devs = time_form(lib, *d, steps, warmup, path=lib.DIRECT, precision=prec)
         got, _ = lib.rap(*d, path=lib.DIRECT, precision=prec)
         b = got.download()

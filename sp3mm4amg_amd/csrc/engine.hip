@@ -2266,8 +2266,9 @@ static bool same_tuning(const Tuning &x, const Tuning &y)
 // that looks faster at all is timed again interleaved with the current
 // schedule, and taken only when that shows it at least 1% faster.  Each pass
 // first tries the hash multipliers ranked on the operands (score_hashes).
-// The descent runs from the installed schedule and again from a preset for
-// mesh operators; the faster end point wins.
+// The descent runs from the installed schedule and again from two presets
+// (mesh operators; tables at an eighth of the bound); the fastest end point
+// wins.
 // --------------------------------------------------------------------------
 int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int prec, double guard, int reps,
              Tuning &best, double &ms_best, double &ms_default)
@@ -2481,43 +2482,63 @@ int autotune(const sp3mm_dmat &R, const sp3mm_dmat &A, const sp3mm_dmat &P, int 
         c->tune = saved;
         return -1;
     }
-    // A second descent from a schedule that suits mesh operators (three-lane
-    // stage-2 batches, four-lane sub-groups, the stage-1 table at half the
-    // bound): on the 128^3 7-point product one descent from the defaults
-    // settled once at 0.63 ms where others reach 0.55 ms (u2 = 3 and
-    // slog2_cap = 2 pay only together).  The better end point is kept after
-    // an interleaved re-timing.
-    Tuning mesh{};
-    mesh.u1 = 4;
-    mesh.u2 = 3;
-    mesh.slog2_cap = 2;
-    mesh.h1_shift = 1;
-    mesh.h2_div = 4;
-    const Tuning end1 = cur;
-    const double end1_ms = cur_ms;
-    if (!reassociated && !same_tuning(mesh, saved) && !would_fill(mesh)) {
-        cur = mesh;
-        if (measure(cur, cur_ms) || descend()) {
+    // Further descents from presets, each end point kept only when an
+    // interleaved re-timing shows it faster than the best so far:
+    // * a schedule that suits mesh operators (three-lane stage-2 batches,
+    //   four-lane sub-groups, the stage-1 table at half the bound): on the
+    //   128^3 7-point product one descent from the defaults settled once at
+    //   0.63 ms where others reach 0.55 ms (u2 = 3 and slog2_cap = 2 pay only
+    //   together);
+    // * tables at an eighth of the bound (stage 1) and of the stage-2 bound
+    //   over 8: on the 27-point 100^3 product with FP32 operands a descent
+    //   from the other starts settled at 0.566 ms with quarter-bound tables,
+    //   where this one reaches 0.485 ms (h1_shift, h2_div and the stage-2
+    //   multiplier pay only together).
+    Tuning presets[2]{};
+    presets[0].u1 = 4;
+    presets[0].u2 = 3;
+    presets[0].slog2_cap = 2;
+    presets[0].h1_shift = 1;
+    presets[0].h2_div = 4;
+    presets[1].u1 = 4;
+    presets[1].u2 = 2;
+    presets[1].slog2_cap = 2;
+    presets[1].h1_shift = 3;
+    presets[1].h2_div = 8;
+    Tuning endb = cur;
+    double endb_ms = cur_ms;
+    for (const Tuning &pre : presets) {
+        if (reassociated || same_tuning(pre, saved) || would_fill(pre)) continue;
+        // a start more than 3x behind the best end point is not descended
+        // from (tables that fill send rows to the long-row kernels, and a
+        // descent among such schedules costs many slow products; a start
+        // with unscored multipliers can be 1.5x behind and still descend to
+        // the best end point)
+        cur = pre;
+        if (measure(cur, cur_ms, 3.0 * endb_ms)) {
             c->tune = saved;
             return -1;
         }
-        if (!same_tuning(cur, end1)) {
-            double ms2 = 0, ms1 = 0;
-            if (confirm(cur, end1, ms2, ms1)) {
-                c->tune = saved;
-                return -1;
-            }
-            if (ms1 <= ms2) {
-                cur = end1;
-                cur_ms = ms1;
-            } else {
-                cur_ms = ms2;
-            }
+        if (cur_ms > 3.0 * endb_ms) continue;
+        if (descend()) {
+            c->tune = saved;
+            return -1;
         }
-    } else {
-        cur = end1;
-        cur_ms = end1_ms;
+        if (same_tuning(cur, endb)) continue;
+        double ms2 = 0, ms1 = 0;
+        if (confirm(cur, endb, ms2, ms1)) {
+            c->tune = saved;
+            return -1;
+        }
+        if (ms2 < ms1) {
+            endb = cur;
+            endb_ms = ms2;
+        } else {
+            endb_ms = ms1;
+        }
     }
+    cur = endb;
+    cur_ms = endb_ms;
     c->tune = cur;
     best = cur;
     ms_best = cur_ms;
